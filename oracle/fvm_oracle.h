@@ -1,0 +1,76 @@
+/*
+ * fvm_oracle.h -- CPU restatement of the FVM-3D-F90 explicit residual + RK
+ * path.  TEST INFRASTRUCTURE ONLY: the test suite, the smoke check and the
+ * CPU-baseline leg of bench.py load liboracle.so as the checker.  Nothing in
+ * the product links or calls it.
+ *
+ * Scheme codes are the reference's (src/defs/type_defs_scheme.f90:11-36);
+ * codes >= 1000 are additions of this build with no reference counterpart.
+ */
+#ifndef FVM_ORACLE_H
+#define FVM_ORACLE_H
+#include <stdint.h>
+
+#define ORC_RK_SSP3       101   /* "Runge-Kutta": Shu-Osher SSP-RK3       */
+#define ORC_RK4           1001  /* classical 4-stage RK (no reference)     */
+#define ORC_SIMPLE        200
+#define ORC_MUSCL         201
+#define ORC_WENO          202
+#define ORC_LIM_NONE      250
+#define ORC_LIM_MINMOD    251
+#define ORC_LIM_VANLEER   252
+#define ORC_LIM_VANALBADA 253
+#define ORC_ROE           300
+#define ORC_VANLEER       301
+#define ORC_AUSM          303
+#define ORC_AUSMPLUS      1303  /* Liou's AUSM+ (no reference)             */
+
+#define ORC_BC_EXTRAP     0     /* zeroth-order copy of the boundary vertex */
+#define ORC_BC_PERIODIC   1     /* nodal periodic: vertex n == vertex 0     */
+
+/* ---- per-pencil scheme routines (same argument meaning as the reference) */
+void orc_recon_simple(int64_t nface, int64_t dghost, int64_t npg,
+                      const double *pg, double *pr, int wind);
+void orc_recon_muscl(int64_t nface, int64_t dghost, double kappa, int limiter,
+                     int64_t npg, const double *pg, double *pr, int wind);
+void orc_recon_weno(int64_t nface, int neq, int64_t dghost, int64_t npg,
+                    const double *qg, double *qr, int wind, int textbook);
+void orc_flux_roe(int64_t nface, int neq, double gamma, const double *norm,
+                  const double *ql, const double *qr, double *flux, double entfix);
+void orc_flux_vanleer(int64_t nface, int neq, double gamma, const double *norm,
+                      const double *ql, const double *qr, double *flux);
+void orc_flux_ausm(int64_t nface, int neq, double gamma, const double *norm,
+                   const double *ql, const double *qr, double *flux);
+void orc_flux_ausmplus(int64_t nface, int neq, double gamma, const double *norm,
+                       const double *ql, const double *qr, double *flux);
+void orc_runge_kutta(int m, double dt, int64_t n, double *u0, double *um,
+                     const double *rm);
+
+/* ---- whole-block solver ------------------------------------------------ */
+typedef struct orc_config {
+    int dim;              /* 2: sweep i,j (k planes independent); 3: i,j,k   */
+    int recon, limiter, flux, rk;
+    int weno_textbook;    /* 0: reference WENO (IS_1 '+', unmirrored right)  */
+    int p_from_q;         /* 1: first stage uses convert(q), 0: IC primitives */
+    int bc[6];            /* i-,i+,j-,j+,k-,k+                               */
+    double kappa, entfix, gamma;
+    double dr[3];
+} orc_config;
+
+/* Runs nsteps steps of fixed dt (dt > 0) or CFL dt (dt <= 0, cfl > 0) on one
+ * block of n[0] x n[1] x n[2] real vertices.
+ *   p_real  AoS (5, n0, n1, n2) initial primitives
+ *   q_out   (n0+6, n1+6, n2+2gz, 5) Fortran order, gz = 3 if dim == 3 else 0
+ *   rhs_out (n0, n1, n2, 5) rhs of the last stage
+ *   res_out (6, nsteps) absolute residual norms of the last stage
+ *   dt_out  (nsteps) the dt used per step                                 */
+int orc_solve(const int64_t n[3], const orc_config *cfg, double dt, double cfl,
+              int nsteps, const double *p_real, double *q_out, double *rhs_out,
+              double *res_out, double *dt_out);
+
+/* Global CFL time step of a state (no reference counterpart):
+ *   dt = cfl / max_v sum_d (|u_d| + c) / dr_d,  c = sqrt(gamma p / rho)   */
+double orc_cfl_dt(const int64_t n[3], int dim, const double *q_real,
+                  double gamma, const double dr[3], double cfl);
+
+#endif
