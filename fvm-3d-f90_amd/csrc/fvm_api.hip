@@ -1,0 +1,609 @@
+// fvm_api.hip -- C ABI of the MI355X residual + RK path (include/fvm_gpu.h):
+// contexts, blocks, host transfers, step orchestration and the small
+// auxiliary kernels (ghost fill, CFL reduction, dt update).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm_gpu.h"
+#include "fvm_launch.hpp"
+
+using namespace fvm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(FVM_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                       \
+  } while (0)
+
+constexpr int kGhost = 3;   // ghost layers per side (src/base/defs_ghost_cell.f90:42)
+
+}  // namespace
+
+struct fvm_block {
+  fvm_ctx* ctx = nullptr;
+  int dim = 3, nc = 5;
+  BlockGeom g{};
+  double* qn = nullptr;   // u^n, then u^{n+1}
+  double* qa = nullptr;   // stage buffers
+  double* qb = nullptr;
+  double* acc = nullptr;  // RK4 accumulator
+  size_t state_elems = 0; // nc * cs
+  bool uploaded = false;
+  // launch shape
+  int ty = 4, chunk = 16;
+  int tiles_x = 0, tiles_y = 0, tiles_z = 0, nblocks = 0;
+};
+
+struct fvm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool scheme_set = false;
+  int recon = 0, flux = 0, rk = 0;
+  Params P{};
+  std::vector<fvm_block*> blocks;
+  double* d_dt = nullptr;                 // CFL dt of the current step
+  unsigned long long* d_smax = nullptr;   // wave-speed max of the current state
+  double* d_norm = nullptr;               // per-workgroup partial sums of rhs^2
+  size_t norm_cap = 0;                    // doubles
+  std::vector<double> h_norm;
+  double norms[6] = {0, 0, 0, 0, 0, 0};
+  int nan_flag = 0;
+  bool have_norms = false;
+  // stage-kernel timing (HIP events around every stage launch)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double stage_ms = 0.0;
+  int64_t stage_launches = 0;
+  double cell_updates = 0.0;
+};
+
+// ============================================================================
+// auxiliary kernels
+// ============================================================================
+namespace {
+
+// Ghost layers of one face region: extrapolation copies the boundary vertex
+// (src/base/defs_block_data.f90:226-256), periodic the vertex n periods away.
+__global__ void ghost_fill_kernel(double* __restrict__ q, BlockGeom g, int nc, int d, int side) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int a_dim = d == 0 ? 1 : 0, b_dim = d == 2 ? 1 : 2;
+  const int na = n[a_dim], nb = n[b_dim];
+  const int64_t total = (int64_t)na * nb * g.ng;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int aa = (int)(e % na);
+    const int bb = (int)((e / na) % nb);
+    const int l = 1 + (int)(e / ((int64_t)na * nb));
+    int x[3], s[3];
+    x[a_dim] = aa; x[b_dim] = bb;
+    s[a_dim] = aa; s[b_dim] = bb;
+    const bool periodic = g.bc[2 * d + side] == BC_PERIODIC;
+    if (side == 0) {
+      x[d] = -l;
+      s[d] = periodic ? n[d] - l : 0;
+    } else {
+      x[d] = n[d] - 1 + l;
+      s[d] = periodic ? l - 1 : n[d] - 1;
+    }
+    const int64_t di = gidx(g, x[0], x[1], x[2]), si = gidx(g, s[0], s[1], s[2]);
+    for (int c = 0; c < nc; ++c) q[di + c * g.cs] = q[si + c * g.cs];
+  }
+}
+
+// Max over real vertices of sum_d (|u_d| + c) / dr_d.
+template <int NC>
+__global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__ q, BlockGeom g,
+                                                     double gamma, unsigned long long* smax) {
+  __shared__ double red[4];
+  const int64_t total = (int64_t)g.nx * g.ny * g.nz;
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.nx);
+    const int j = (int)((e / g.nx) % g.ny);
+    const int k = (int)(e / ((int64_t)g.nx * g.ny));
+    const int64_t ci = gidx(g, i, j, k);
+    double v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = q[ci + c * g.cs];
+    const double w = wave_speed<NC>(v, g, gamma);
+    s = (w > s || w != w) ? w : s;
+  }
+  s = wg_max<256>(s, red);
+  if (threadIdx.x == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(s));
+}
+
+// dt = cfl / smax for the step about to start; clears the accumulator so the
+// step's last stage can collect the next maximum.
+__global__ void cfl_dt_kernel(unsigned long long* smax, double* dt, double cfl) {
+  const double s = __longlong_as_double((long long)*smax);
+  *dt = cfl / s;
+  *smax = 0ull;
+}
+
+}  // namespace
+
+// ============================================================================
+// helpers
+// ============================================================================
+namespace {
+
+int check_ctx(fvm_ctx* ctx) {
+  if (!ctx) return fail(FVM_EINVAL, "null context");
+  return FVM_OK;
+}
+
+int set_device(fvm_ctx* ctx) {
+  HIP_TRY(hipSetDevice(ctx->device));
+  return FVM_OK;
+}
+
+// Launch shape of a block: 3-D tiles of 62 x ty columns marching over
+// `chunk` planes; 2-D rows of 62 columns marching over `chunk` rows.
+void plan_launch(fvm_block* b) {
+  const BlockGeom& g = b->g;
+  b->tiles_x = (g.nx + 61) / 62;
+  if (b->dim == 2) {
+    b->tiles_y = 1;
+    // keep enough workgroups to fill the device on small grids
+    int chunk = 16;
+    while (chunk > 2 && (int64_t)b->tiles_x * ((g.ny + chunk - 1) / chunk) < 2048) chunk /= 2;
+    b->chunk = chunk;
+    b->tiles_z = (g.ny + chunk - 1) / chunk;
+  } else {
+    b->tiles_y = (g.ny + b->ty - 1) / b->ty;
+    int chunk = 32;
+    while (chunk > 4 && (int64_t)b->tiles_x * b->tiles_y * ((g.nz + chunk - 1) / chunk) < 2048)
+      chunk /= 2;
+    b->chunk = chunk;
+    b->tiles_z = (g.nz + chunk - 1) / chunk;
+  }
+  b->nblocks = b->tiles_x * b->tiles_y * b->tiles_z;
+}
+
+int ensure_norm_capacity(fvm_ctx* ctx, size_t doubles) {
+  if (doubles <= ctx->norm_cap) return FVM_OK;
+  if (ctx->d_norm) HIP_TRY(hipFree(ctx->d_norm));
+  ctx->d_norm = nullptr;
+  if (hipMalloc(&ctx->d_norm, doubles * sizeof(double)) != hipSuccess)
+    return fail(FVM_ENOMEM, "cannot allocate %zu norm partials", doubles);
+  ctx->norm_cap = doubles;
+  return FVM_OK;
+}
+
+int fill_ghosts(fvm_block* b, double* q) {
+  for (int d = 0; d < b->dim; ++d)
+    for (int side = 0; side < 2; ++side) {
+      hipLaunchKernelGGL(ghost_fill_kernel, dim3(256), dim3(256), 0, b->ctx->stream, q, b->g,
+                         b->nc, d, side);
+      HIP_TRY(hipGetLastError());
+    }
+  return FVM_OK;
+}
+
+int launch_stage(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
+  hipError_t e = hipErrorInvalidValue;
+  if (b->dim == 2) {
+    switch (ctx->recon) {
+      case RECON_SIMPLE: e = launch_stage2d_simple(ctx->flux, a, b->nblocks, ctx->stream); break;
+      case RECON_MUSCL: e = launch_stage2d_muscl(ctx->flux, a, b->nblocks, ctx->stream); break;
+      case RECON_WENO: e = launch_stage2d_weno(ctx->flux, a, b->nblocks, ctx->stream); break;
+    }
+  } else {
+    switch (ctx->recon) {
+      case RECON_SIMPLE: e = launch_stage3d_simple(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
+      case RECON_MUSCL: e = launch_stage3d_muscl(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
+      case RECON_WENO: e = launch_stage3d_weno(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
+    }
+  }
+  if (e != hipSuccess) return fail(FVM_EHIP, "stage launch: %s", hipGetErrorString(e));
+  return FVM_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* fvm_last_error(void) { return g_err.c_str(); }
+
+const char* fvm_version(void) { return "fvm-mi355x 0.1.0 (gfx950, fp64)"; }
+
+int fvm_ctx_create(int device, fvm_ctx** out) {
+  if (!out) return fail(FVM_EINVAL, "null output pointer");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(FVM_EINVAL, "device %d out of range (%d devices)", device, ndev);
+  fvm_ctx* ctx = new fvm_ctx();
+  ctx->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIP_TRY(hipMalloc(&ctx->d_dt, sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_smax, sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(ctx->d_smax, 0, sizeof(unsigned long long)));
+  *out = ctx;
+  g_err.clear();
+  return FVM_OK;
+}
+
+int fvm_ctx_destroy(fvm_ctx* ctx) {
+  if (!ctx) return FVM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (fvm_block* b : ctx->blocks) {
+    (void)hipFree(b->qn); (void)hipFree(b->qa); (void)hipFree(b->qb); (void)hipFree(b->acc);
+    delete b;
+  }
+  for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  (void)hipFree(ctx->d_dt);
+  (void)hipFree(ctx->d_smax);
+  (void)hipFree(ctx->d_norm);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FVM_OK;
+}
+
+int fvm_scheme_set(fvm_ctx* ctx, int recon, int limiter, int flux, int rk, double kappa,
+                   double entropy_fix, double gamma, int ref_compat) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (recon != RECON_SIMPLE && recon != RECON_MUSCL && recon != RECON_WENO)
+    return fail(FVM_EINVAL, "unknown reconstruction code %d", recon);
+  if (recon == RECON_MUSCL && limiter != LIM_NONE && limiter != LIM_MINMOD &&
+      limiter != LIM_VANLEER && limiter != LIM_VANALBADA && limiter != -1)
+    return fail(FVM_EINVAL, "unknown limiter code %d", limiter);
+  if (flux != FLUX_ROE && flux != FLUX_VANLEER && flux != FLUX_AUSM && flux != FLUX_AUSMPLUS)
+    return fail(FVM_EINVAL, "unknown flux code %d", flux);
+  if (rk != RK_SSP3 && rk != RK4) return fail(FVM_EINVAL, "unknown time scheme code %d", rk);
+  if (!(gamma > 1.0)) return fail(FVM_EINVAL, "gamma must exceed 1 (got %g)", gamma);
+  ctx->recon = recon;
+  ctx->flux = flux;
+  ctx->rk = rk;
+  ctx->P.gamma = gamma;
+  ctx->P.kappa = kappa;
+  ctx->P.entfix = entropy_fix;
+  ctx->P.g_gm1 = gamma / (gamma - 1.0);
+  ctx->P.limiter = limiter;
+  ctx->P.weno_textbook = ref_compat ? 0 : 1;
+  ctx->scheme_set = true;
+  return FVM_OK;
+}
+
+int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
+                     double dz, fvm_block** out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out) return fail(FVM_EINVAL, "null output pointer");
+  if (ni < 2 || nj < 2 || nk < 1) return fail(FVM_EINVAL, "block needs ni, nj >= 2 and nk >= 1 (got %lld %lld %lld)",
+                                              (long long)ni, (long long)nj, (long long)nk);
+  if (ni > (1 << 24) || nj > (1 << 24) || nk > (1 << 24))
+    return fail(FVM_EINVAL, "block extent too large");
+  const int dim = nk == 1 ? 2 : 3;
+  if (dim == 3 && nk < 2) return fail(FVM_EINVAL, "3-D block needs nk >= 2");
+  if (!(dx > 0) || !(dy > 0) || (dim == 3 && !(dz > 0)))
+    return fail(FVM_EINVAL, "vertex spacing must be positive");
+  if (int rc = set_device(ctx)) return rc;
+  fvm_block* b = new fvm_block();
+  b->ctx = ctx;
+  b->dim = dim;
+  b->nc = dim == 3 ? 5 : 4;
+  BlockGeom& g = b->g;
+  g.nx = (int)ni; g.ny = (int)nj; g.nz = (int)nk;
+  g.ox = kGhost; g.oy = kGhost; g.oz = dim == 3 ? kGhost : 0;
+  g.px = ni + 2 * kGhost;
+  g.pxy = g.px * (nj + 2 * kGhost);
+  const int64_t planes = dim == 3 ? nk + 2 * kGhost : 1;
+  g.cs = (g.pxy * planes + 63) / 64 * 64;   // 512-byte aligned components
+  g.dx = dx; g.dy = dy; g.dz = dim == 3 ? dz : 1.0;
+  for (int f = 0; f < 6; ++f) g.bc[f] = BC_EXTRAP;
+  g.ng = kGhost;
+  b->state_elems = (size_t)b->nc * g.cs;
+  const size_t bytes = b->state_elems * sizeof(double);
+  double** bufs[4] = {&b->qn, &b->qa, &b->qb, &b->acc};
+  for (double** p : bufs) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      for (double** r : bufs) if (*r) (void)hipFree(*r);
+      delete b;
+      return fail(FVM_ENOMEM, "cannot allocate %zu bytes of block state", bytes);
+    }
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, ctx->stream));
+  }
+  b->ty = 4;
+  plan_launch(b);
+  ctx->blocks.push_back(b);
+  *out = b;
+  return FVM_OK;
+}
+
+int fvm_block_destroy(fvm_block* b) {
+  if (!b) return FVM_OK;
+  fvm_ctx* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(b->qn); (void)hipFree(b->qa); (void)hipFree(b->qb); (void)hipFree(b->acc);
+  ctx->blocks.erase(std::remove(ctx->blocks.begin(), ctx->blocks.end(), b), ctx->blocks.end());
+  delete b;
+  return FVM_OK;
+}
+
+int fvm_block_set_bc(fvm_block* b, int face, int kind) {
+  if (!b) return fail(FVM_EINVAL, "null block");
+  if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
+  if (kind != FVM_BC_EXTRAP && kind != FVM_BC_PERIODIC) return fail(FVM_EINVAL, "unknown BC kind %d", kind);
+  if (b->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
+  b->g.bc[face] = kind;
+  // periodic pairs must agree
+  const int d = face / 2;
+  const int other = b->g.bc[2 * d + (1 - face % 2)];
+  if (kind == FVM_BC_PERIODIC && other != FVM_BC_PERIODIC) b->g.bc[2 * d + (1 - face % 2)] = FVM_BC_PERIODIC;
+  if (kind == FVM_BC_EXTRAP && other == FVM_BC_PERIODIC) b->g.bc[2 * d + (1 - face % 2)] = FVM_BC_EXTRAP;
+  if (b->uploaded) {
+    if (int rc = set_device(b->ctx)) return rc;
+    return fill_ghosts(b, b->qn);
+  }
+  return FVM_OK;
+}
+
+// host (Fortran order, 5 equations, host_ng ghosts per swept side) -> device
+int fvm_upload(fvm_block* b, const double* q, int host_ng) {
+  if (!b || !q) return fail(FVM_EINVAL, "null block or host array");
+  if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
+  fvm_ctx* ctx = b->ctx;
+  if (int rc = set_device(ctx)) return rc;
+  const BlockGeom& g = b->g;
+  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
+  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const int64_t hcs = hx * hy * hz;
+  std::vector<double> stage((size_t)b->state_elems, 0.0);
+  for (int c = 0; c < 5; ++c) {
+    int dc = c;
+    if (b->dim == 2) {
+      if (c == 3) {
+        for (int64_t k = 0; k < g.nz; ++k)
+          for (int64_t j = 0; j < g.ny; ++j)
+            for (int64_t i = 0; i < g.nx; ++i) {
+              const double v = q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + i + host_ng];
+              if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
+            }
+        continue;
+      }
+      if (c == 4) dc = 3;
+    }
+    for (int64_t k = 0; k < g.nz; ++k)
+      for (int64_t j = 0; j < g.ny; ++j) {
+        const double* src = q + c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + host_ng;
+        double* dst = stage.data() + dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + g.ox;
+        std::memcpy(dst, src, g.nx * sizeof(double));
+      }
+  }
+  HIP_TRY(hipMemcpyAsync(b->qn, stage.data(), b->state_elems * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  if (int rc = fill_ghosts(b, b->qn)) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  b->uploaded = true;
+  return FVM_OK;
+}
+
+int fvm_download(fvm_block* b, double* q, int host_ng) {
+  if (!b || !q) return fail(FVM_EINVAL, "null block or host array");
+  if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
+  fvm_ctx* ctx = b->ctx;
+  if (int rc = set_device(ctx)) return rc;
+  const BlockGeom& g = b->g;
+  std::vector<double> stage((size_t)b->state_elems);
+  HIP_TRY(hipMemcpyAsync(stage.data(), b->qn, b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const int gl = std::min(host_ng, g.ng);
+  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
+  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const int64_t hcs = hx * hy * hz;
+  const int gzl = b->dim == 3 ? gl : 0;
+  std::memset(q, 0, sizeof(double) * 5 * hcs);
+  for (int c = 0; c < 5; ++c) {
+    if (b->dim == 2 && c == 3) continue;   // rho*w == 0
+    const int dc = (b->dim == 2 && c == 4) ? 3 : c;
+    for (int64_t k = -gzl; k < g.nz + gzl; ++k)
+      for (int64_t j = -gl; j < g.ny + gl; ++j)
+        for (int64_t i = -gl; i < g.nx + gl; ++i) {
+          // faces only: at most one index outside the real range
+          const int outside = (i < 0 || i >= g.nx) + (j < 0 || j >= g.ny) + (k < 0 || k >= g.nz);
+          if (outside > 1) continue;
+          q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + (i + host_ng)] =
+              stage[dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + (i + g.ox)];
+        }
+  }
+  return FVM_OK;
+}
+
+int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
+  if (nsteps < 0) return fail(FVM_EINVAL, "nsteps must be >= 0");
+  const bool cfl_mode = !(dt > 0.0);
+  if (cfl_mode && !(cfl > 0.0)) return fail(FVM_EINVAL, "dt <= 0 needs cfl > 0");
+  for (fvm_block* b : ctx->blocks)
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
+  if (int rc = set_device(ctx)) return rc;
+  ctx->have_norms = false;
+  if (nsteps == 0) return FVM_OK;
+
+  // norm partial buffer: NC doubles per workgroup of every block
+  size_t need = 0;
+  for (fvm_block* b : ctx->blocks) need += (size_t)b->nblocks * b->nc;
+  if (int rc = ensure_norm_capacity(ctx, need)) return rc;
+
+  if (cfl_mode) {
+    HIP_TRY(hipMemsetAsync(ctx->d_smax, 0, sizeof(unsigned long long), ctx->stream));
+    for (fvm_block* b : ctx->blocks) {
+      if (b->nc == 5)
+        hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->qn, b->g, ctx->P.gamma, ctx->d_smax);
+      else
+        hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->qn, b->g, ctx->P.gamma, ctx->d_smax);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+
+  const int nstage = ctx->rk == RK4 ? 4 : 3;
+  size_t nev = 0;
+  if (ctx->timing) {
+    const size_t need_ev = 2 * (size_t)nsteps * nstage * ctx->blocks.size();
+    while (ctx->ev.size() < need_ev) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      ctx->ev.push_back(e);
+    }
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    if (cfl_mode) {
+      hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_smax, ctx->d_dt, cfl);
+      HIP_TRY(hipGetLastError());
+    }
+    for (int st = 1; st <= nstage; ++st) {
+      const bool last = st == nstage;
+      size_t part_off = 0;
+      for (fvm_block* b : ctx->blocks) {
+        StageArgs a{};
+        a.g = b->g;
+        a.P = ctx->P;
+        a.dt = cfl_mode ? 0.0 : dt;
+        a.dt_dev = cfl_mode ? ctx->d_dt : nullptr;
+        a.tiles_x = b->tiles_x; a.tiles_y = b->tiles_y; a.tiles_z = b->tiles_z;
+        a.chunk = b->chunk;
+        a.q0 = b->qn;
+        a.acc = nullptr;
+        if (ctx->rk == RK4) {
+          a.acc = b->acc;
+          switch (st) {
+            case 1: a.qin = b->qn; a.qout = b->qa; a.stage = ST_RK4_1; break;
+            case 2: a.qin = b->qa; a.qout = b->qb; a.stage = ST_RK4_2; break;
+            case 3: a.qin = b->qb; a.qout = b->qa; a.stage = ST_RK4_3; break;
+            default: a.qin = b->qa; a.qout = b->qn; a.stage = ST_RK4_4; break;
+          }
+        } else {
+          switch (st) {
+            case 1: a.qin = b->qn; a.qout = b->qa; a.stage = ST_SSP1; break;
+            case 2: a.qin = b->qa; a.qout = b->qb; a.stage = ST_SSP2; break;
+            default: a.qin = b->qb; a.qout = b->qn; a.stage = ST_SSP3; break;
+          }
+        }
+        a.norm_part = (last && s == nsteps - 1) ? ctx->d_norm + part_off : nullptr;
+        a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
+        part_off += (size_t)b->nblocks * b->nc;
+        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
+        if (int rc = launch_stage(ctx, b, a)) return rc;
+        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
+      }
+    }
+  }
+
+  // residual norms of the last stage (class_iteration.f90:210-248), summed
+  // over workgroup partials in a fixed order
+  ctx->h_norm.resize(need);
+  HIP_TRY(hipMemcpyAsync(ctx->h_norm.data(), ctx->d_norm, need * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  double dt_host = dt;
+  if (cfl_mode) HIP_TRY(hipMemcpyAsync(&dt_host, ctx->d_dt, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (dt_last) *dt_last = dt_host;
+  if (ctx->timing) {
+    for (size_t e = 0; e < nev; e += 2) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[e], ctx->ev[e + 1]));
+      ctx->stage_ms += ms;
+    }
+    ctx->stage_launches += (int64_t)(nev / 2);
+    for (fvm_block* b : ctx->blocks)
+      ctx->cell_updates += (double)b->g.nx * b->g.ny * b->g.nz * nstage * nsteps;
+  }
+  double res_abs[6] = {0, 0, 0, 0, 0, 0};
+  size_t off = 0;
+  for (fvm_block* b : ctx->blocks) {
+    double sums[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < b->nblocks; ++w)
+      for (int c = 0; c < b->nc; ++c) sums[c] += ctx->h_norm[off + (size_t)w * b->nc + c];
+    off += (size_t)b->nblocks * b->nc;
+    double res2[5] = {0, 0, 0, 0, 0};
+    const double npts = (double)b->g.nx * b->g.ny * b->g.nz;
+    for (int c = 0; c < b->nc; ++c) {
+      const int e = (b->nc == 4 && c == 3) ? 4 : c;   // 2-D: energy is equation 5
+      res2[e] = sums[c] / npts;
+    }
+    double res2t = 0.0;
+    for (int e = 0; e < 5; ++e) res2t += res2[e];
+    res2t = res2t / 5.0;
+    for (int e = 0; e < 5; ++e) res_abs[e] = std::sqrt(res_abs[e] * res_abs[e] + res2[e]);
+    res_abs[5] = std::sqrt(res_abs[5] * res_abs[5] + res2t);
+  }
+  int nan = 0;
+  for (int e = 0; e < 6; ++e) {
+    ctx->norms[e] = res_abs[e];
+    if (std::isnan(res_abs[e])) nan = 1;
+  }
+  if (res_abs[5] > 1.0e10) nan = 1;
+  if (cfl_mode && !(dt_host > 0.0 && std::isfinite(dt_host))) nan = 1;
+  ctx->nan_flag = nan;
+  ctx->have_norms = true;
+  if (cfl_mode && nan && !std::isfinite(dt_host)) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
+  return FVM_OK;
+}
+
+int fvm_norms(fvm_ctx* ctx, double out[6], int* nan_flag) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->have_norms) return fail(FVM_ESTATE, "no step has run");
+  if (out) for (int e = 0; e < 6; ++e) out[e] = ctx->norms[e];
+  if (nan_flag) *nan_flag = ctx->nan_flag;
+  return FVM_OK;
+}
+
+int fvm_timing_enable(fvm_ctx* ctx, int enable) {
+  if (int rc = check_ctx(ctx)) return rc;
+  ctx->timing = enable != 0;
+  ctx->stage_ms = 0.0;
+  ctx->stage_launches = 0;
+  ctx->cell_updates = 0.0;
+  return FVM_OK;
+}
+
+int fvm_timing_read(fvm_ctx* ctx, double* stage_ms, int64_t* launches, double* cell_updates) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (stage_ms) *stage_ms = ctx->stage_ms;
+  if (launches) *launches = ctx->stage_launches;
+  if (cell_updates) *cell_updates = ctx->cell_updates;
+  return FVM_OK;
+}
+
+int fvm_sync(fvm_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// fvm_numerics.hpp -- per-point FP64 numerics of the explicit residual path
+// (reconstruction, face fluxes, state conversion) as device functions.
+//
+// Every expression keeps the operation order of the FVM-3D-F90 scheme
+// routines and the file is compiled with -ffp-contract=off, so each result is
+// the same IEEE sequence as the reference (Fortran evaluates a*b*c as (a*b)*c,
+// x**2 as x*x and -a*b as -(a*b)).  Face normals are the unit axis vectors
+// (src/solve/class_space_discret.f90:458-461); products with the zero normal
+// components are dropped, which changes at most the sign of an exact zero in
+// a face flux and never the residual.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fvm {
+
+// ---- scheme codes, src/defs/type_defs_scheme.f90:11-36 -------------------
+enum : int {
+  RK_SSP3 = 101, RK4 = 1001,
+  RECON_SIMPLE = 200, RECON_MUSCL = 201, RECON_WENO = 202,
+  LIM_NONE = 250, LIM_MINMOD = 251, LIM_VANLEER = 252, LIM_VANALBADA = 253,
+  FLUX_ROE = 300, FLUX_VANLEER = 301, FLUX_AUSM = 303, FLUX_AUSMPLUS = 1303,
+};
+
+struct Params {
+  double gamma, kappa, entfix;
+  double g_gm1;   // gamma / (gamma - 1), evaluated once as the reference does per face
+  int limiter;
+  int weno_textbook;
+};
+
+// Fortran SIGN(1.0_dp, x)
+__device__ __forceinline__ double fsign1(double x) { return copysign(1.0, x); }
+// Fortran MIN(a, b) for finite reals
+__device__ __forceinline__ double fmin2(double a, double b) { return (b < a) ? b : a; }
+
+// ---- primitive <-> conserved, src/solve/class_flow_field.f90:353-396 ---------
+// NC = 4: 2-D Euler (rho, rho u, rho v, rho E), w == 0; NC = 5: 3-D.
+template <int NC>
+__device__ __forceinline__ void cons_to_prim(const double q[NC], double p[NC], double gamma) {
+  const double rho = q[0];
+  const double u = q[1] / rho, v = q[2] / rho;
+  p[0] = rho; p[1] = u; p[2] = v;
+  if constexpr (NC == 5) {
+    const double w = q[3] / rho;
+    p[3] = w;
+    p[4] = (gamma - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w));
+  } else {
+    p[3] = (gamma - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v));
+  }
+}
+
+// ---- MUSCL, src/scheme/scheme_reconstruct_MUSCL.f90:114-226 ----------------
+// Per-vertex form: both face values of vertex p2 share the limited slopes.
+//   L = value at the vertex's right face (left state of that face, wind -1)
+//   R = value at the vertex's left face  (right state of that face, wind +1)
+__device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3,
+                                             double kappa, int limiter,
+                                             double& L, double& R) {
+  const double D_m = p2 - p1;
+  const double D_p = p3 - p2;
+  double Db_m, Db_p;
+  if (limiter == LIM_MINMOD) {
+    // minmod(x, y, 1) is symmetric in x and y: one evaluation serves both.
+    Db_m = 0.5 * (fsign1(D_m) + fsign1(D_p)) * fmin2(fabs(D_m), fabs(1.0 * D_p));
+    Db_p = Db_m;
+  } else if (limiter == LIM_VANLEER) {
+    const double eps = 1.0e-6;
+    Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p / (fabs(D_m) + fabs(D_p) + eps);
+    Db_p = Db_m;
+  } else if (limiter == LIM_VANALBADA) {
+    const double eps = 1.0e-6;
+    Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) / (D_m * D_m + D_p * D_p + eps);
+    Db_p = Db_m;
+  } else {
+    Db_m = D_m;
+    Db_p = D_p;
+  }
+  L = p2 + 0.25 * ((1.0 - kappa) * Db_m + (1.0 + kappa) * Db_p);
+  R = p2 - 0.25 * ((1.0 - kappa) * Db_p + (1.0 + kappa) * Db_m);
+}
+
+// ---- WENO-JS, src/scheme/scheme_WENO.f90:87-121 ------------------------------
+// Value at the right face of u2.  textbook = 0 keeps the reference's
+// IS_1 = 13/12 (u1-2u2+u3)^2 + 1/4 (u1 + u3)^2 (scheme_WENO.f90:107-108).
+__device__ __forceinline__ double weno_js(double u0, double u1, double u2, double u3,
+                                          double u4, int textbook) {
+  const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
+  const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
+  const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+  const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
+  const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
+  const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
+  const double t0 = u0 - 2.0 * u1 + u2;
+  const double s0 = u0 - 4.0 * u1 + 3.0 * u2;
+  const double t1 = u1 - 2.0 * u2 + u3;
+  const double s1 = textbook ? (u1 - u3) : (u1 + u3);
+  const double t2 = u2 - 2.0 * u3 + u4;
+  const double s2 = 3.0 * u2 - 4.0 * u3 + u4;
+  const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
+  const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
+  const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
+  const double a0 = c_0 / (e0 * e0);
+  const double a1 = c_1 / (e1 * e1);
+  const double a2 = c_2 / (e2 * e2);
+  const double as = a0 + a1 + a2;
+  const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+  return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+// ---- Harten fix, src/scheme/scheme_FDS_Roe.f90:167-177 ----------------------
+__device__ __forceinline__ double entropy_correct(double lambda, double delta) {
+  return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) / delta;
+}
+
+// Face state as primitive components along the sweep direction D (0,1,2):
+// un = normal velocity, ut1/ut2 = the other two components in (u,v,w) order.
+// Fluxes are produced in (mass, mom_u, mom_v, mom_w, energy) order.
+
+// ---- Roe FDS, src/scheme/scheme_FDS_Roe.f90:25-138 ---------------------------
+template <int NC, int D>
+__device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
+                                         const Params& P, double F[NC]) {
+  const double gamma = P.gamma;
+  const double rho_l = L[0], rho_r = R[0];
+  const double u_l = L[1], v_l = L[2], w_l = (NC == 5) ? L[3] : 0.0;
+  const double u_r = R[1], v_r = R[2], w_r = (NC == 5) ? R[3] : 0.0;
+  const double p_l = L[NC - 1], p_r = R[NC - 1];
+  const double Vc_l = (D == 0) ? u_l : (D == 1) ? v_l : w_l;
+  const double Vc_r = (D == 0) ? u_r : (D == 1) ? v_r : w_r;
+  double ke_l, ke_r;
+  if constexpr (NC == 5) { ke_l = u_l * u_l + v_l * v_l + w_l * w_l; ke_r = u_r * u_r + v_r * v_r + w_r * w_r; }
+  else { ke_l = u_l * u_l + v_l * v_l; ke_r = u_r * u_r + v_r * v_r; }
+  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+  const double drho = rho_r - rho_l, du = u_r - u_l, dv = v_r - v_l, dw = w_r - w_l;
+  const double dpress = p_r - p_l, dVl = Vc_r - Vc_l;
+  const double rho_b = sqrt(rho_l * rho_r);
+  const double den = rho_l + rho_b;
+  const double u_b = (rho_l * u_l + rho_b * u_r) / den;
+  const double v_b = (rho_l * v_l + rho_b * v_r) / den;
+  const double w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) / den : 0.0;
+  const double H_b = (rho_l * H_l + rho_b * H_r) / den;
+  const double Vl_b = (D == 0) ? u_b : (D == 1) ? v_b : w_b;
+  double q2_b;
+  if constexpr (NC == 5) q2_b = u_b * u_b + v_b * v_b + w_b * w_b;
+  else q2_b = u_b * u_b + v_b * v_b;
+  const double c_b = sqrt((gamma - 1.0) * (H_b - 0.5 * q2_b));
+  const double delta = P.entfix * fabs(Vl_b + c_b);
+  const double lam1 = entropy_correct(fabs(Vl_b - c_b), delta);
+  const double lam2 = entropy_correct(fabs(Vl_b), delta);
+  const double lam5 = entropy_correct(fabs(Vl_b + c_b), delta);
+  const double d1 = lam1 * (dpress - rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+  const double d2 = lam2 * (drho - dpress / (c_b * c_b));
+  const double d5 = lam5 * (dpress + rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+  const double lr = lam2 * rho_b;
+  // mass
+  F[0] = 0.5 * ((rho_l * Vc_l + rho_r * Vc_r) - (d1 + d2 + d5));
+  // momentum components; the normal one carries -/+ c and the pressure term
+  const double ub[3] = {u_b, v_b, w_b};
+  const double dd[3] = {du, dv, dw};
+  const double vl[3] = {u_l, v_l, w_l};
+  const double vr[3] = {u_r, v_r, w_r};
+#pragma unroll
+  for (int m = 0; m < NC - 2; ++m) {
+    double e1, e2, e5, fl, fr;
+    if (m == D) {
+      e1 = d1 * (ub[m] - c_b);
+      e2 = d2 * ub[m] + lr * (dd[m] - dVl);
+      e5 = d5 * (ub[m] + c_b);
+      fl = rho_l * vl[m] * Vc_l + p_l;
+      fr = rho_r * vr[m] * Vc_r + p_r;
+    } else {
+      e1 = d1 * ub[m];
+      e2 = d2 * ub[m] + lr * dd[m];
+      e5 = d5 * ub[m];
+      fl = rho_l * vl[m] * Vc_l;
+      fr = rho_r * vr[m] * Vc_r;
+    }
+    F[1 + m] = 0.5 * ((fl + fr) - (e1 + e2 + e5));
+  }
+  // energy
+  double udu;
+  if constexpr (NC == 5) udu = u_b * du + v_b * dv + w_b * dw;
+  else udu = u_b * du + v_b * dv;
+  const double e1 = d1 * (H_b - c_b * Vl_b);
+  const double e2 = d2 * 0.5 * q2_b + lr * (udu - Vl_b * dVl);
+  const double e5 = d5 * (H_b + c_b * Vl_b);
+  F[NC - 1] = 0.5 * ((rho_l * H_l * Vc_l + rho_r * H_r * Vc_r) - (e1 + e2 + e5));
+}
+
+// ---- Van Leer FVS, src/scheme/scheme_FVS_VanLeer.f90:25-126 ------------------
+// The split keeps the subsonic momentum/energy factors in the supersonic
+// branch, exactly as the reference does.
+template <int NC, int D>
+__device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[NC],
+                                             const Params& P, double F[NC]) {
+  const double gamma = P.gamma;
+  const double rho_l = L[0], rho_r = R[0];
+  const double vl[3] = {L[1], L[2], (NC == 5) ? L[3] : 0.0};
+  const double vr[3] = {R[1], R[2], (NC == 5) ? R[3] : 0.0};
+  const double p_l = L[NC - 1], p_r = R[NC - 1];
+  const double Vc_l = vl[D], Vc_r = vr[D];
+  double q2_l, q2_r;
+  if constexpr (NC == 5) { q2_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; q2_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
+  else { q2_l = vl[0] * vl[0] + vl[1] * vl[1]; q2_r = vr[0] * vr[0] + vr[1] * vr[1]; }
+  const double c_l = sqrt(gamma * p_l / rho_l);
+  const double c_r = sqrt(gamma * p_r / rho_r);
+  double M_l = Vc_l / c_l, M_r = Vc_r / c_r;
+  M_l = (M_l >= 1.0) ? M_l : (M_l <= -1.0) ? 0.0 : 0.25 * (M_l + 1.0) * (M_l + 1.0);
+  M_r = (M_r >= 1.0) ? 0.0 : (M_r <= -1.0) ? M_r : -(0.25 * (M_r - 1.0) * (M_r - 1.0));
+  const double fp0 = rho_l * c_l * M_l;
+  const double fm0 = rho_r * c_r * M_r;
+  F[0] = fp0 + fm0;
+  const double gp = -Vc_l + 2.0 * c_l, gm = -Vc_r - 2.0 * c_r;
+#pragma unroll
+  for (int m = 0; m < NC - 2; ++m) {
+    double a, b;
+    if (m == D) { a = fp0 * (gp / gamma + vl[m]); b = fm0 * (gm / gamma + vr[m]); }
+    else { a = fp0 * (0.0 / gamma + vl[m]); b = fm0 * (0.0 / gamma + vr[m]); }
+    F[1 + m] = a + b;
+  }
+  const double tl = (gamma - 1.0) * Vc_l + 2.0 * c_l;
+  const double tr = (gamma - 1.0) * Vc_r - 2.0 * c_r;
+  const double den = 2.0 * (gamma * gamma - 1.0);
+  F[NC - 1] = fp0 * (tl * tl / den + (q2_l - Vc_l * Vc_l) / 2.0)
+            + fm0 * (tr * tr / den + (q2_r - Vc_r * Vc_r) / 2.0);
+}
+
+// ---- "AUSM" as the reference implements it, src/scheme/scheme_AUSM.f90 -------
+// beta = 0, alpha = 0 (:128-129,146-147), per-side c and M (:80-83),
+// Harten-type fix on |Mn| with delta = 0.1 (:160-171).
+template <int NC, int D>
+__device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC],
+                                          const Params& P, double F[NC]) {
+  const double gamma = P.gamma;
+  const double rho_l = L[0], rho_r = R[0];
+  const double vl[3] = {L[1], L[2], (NC == 5) ? L[3] : 0.0};
+  const double vr[3] = {R[1], R[2], (NC == 5) ? R[3] : 0.0};
+  const double p_l = L[NC - 1], p_r = R[NC - 1];
+  const double Vc_l = vl[D], Vc_r = vr[D];
+  double ke_l, ke_r;
+  if constexpr (NC == 5) { ke_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; ke_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
+  else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
+  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+  const double c_l = sqrt(gamma * p_l / rho_l);
+  const double c_r = sqrt(gamma * p_r / rho_r);
+  const double M_l = Vc_l / c_l, M_r = Vc_r / c_r;
+  // Mach split, beta = 0: the beta term adds an exact +0 for |M| < 1
+  const double mp = (fabs(M_l) >= 1.0) ? 0.5 * (M_l + 1.0 * fabs(M_l))
+                                       : 1.0 * (0.25 * ((M_l + 1.0) * (M_l + 1.0)));
+  const double mm = (fabs(M_r) >= 1.0) ? 0.5 * (M_r + -1.0 * fabs(M_r))
+                                       : -1.0 * (0.25 * ((M_r + -1.0) * (M_r + -1.0)));
+  const double Mn = mp + mm;
+  const double aMn = fabs(Mn);
+  const double delta = 0.1;
+  const double Mabs = (aMn > delta) ? aMn : 0.5 * (aMn * aMn + delta * delta) / delta;
+  // pressure split, alpha = 0: the alpha term adds an exact zero for |M| < 1
+  const double pp = ((fabs(M_l) >= 1.0) ? 0.5 * (1.0 + 1.0 * fsign1(M_l))
+                                        : 0.25 * ((M_l + 1.0) * (M_l + 1.0)) * (2.0 - 1.0 * M_l)) * p_l;
+  const double pm = ((fabs(M_r) >= 1.0) ? 0.5 * (1.0 + -1.0 * fsign1(M_r))
+                                        : 0.25 * ((M_r + -1.0) * (M_r + -1.0)) * (2.0 - -1.0 * M_r)) * p_r;
+  const double ph = pp + pm;
+  const double a = 0.5 * (Mn + Mabs), b = 0.5 * (Mn - Mabs);
+  const double phl = rho_l * c_l, phr = rho_r * c_r;
+  F[0] = a * phl + b * phr;
+#pragma unroll
+  for (int m = 0; m < NC - 2; ++m) {
+    const double f = a * (phl * vl[m]) + b * (phr * vr[m]);
+    F[1 + m] = (m == D) ? f + ph : f;
+  }
+  F[NC - 1] = a * (phl * H_l) + b * (phr * H_r);
+}
+
+// ---- AUSM+ (Liou 1996), no reference counterpart ------------------------------
+// J. Comput. Phys. 129 (1996) 364-382: beta = 1/8, alpha = 3/16,
+// c_1/2 = min(c~_L, c~_R), c~ = c*^2 / max(c*, |V|), c*^2 = 2(g-1)/(g+1) H.
+template <int NC, int D>
+__device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R[NC],
+                                              const Params& P, double F[NC]) {
+  const double gamma = P.gamma;
+  const double beta = 0.125, alpha = 0.1875;
+  const double rho_l = L[0], rho_r = R[0];
+  const double vl[3] = {L[1], L[2], (NC == 5) ? L[3] : 0.0};
+  const double vr[3] = {R[1], R[2], (NC == 5) ? R[3] : 0.0};
+  const double p_l = L[NC - 1], p_r = R[NC - 1];
+  const double Vc_l = vl[D], Vc_r = vr[D];
+  double ke_l, ke_r;
+  if constexpr (NC == 5) { ke_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; ke_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
+  else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
+  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+  const double cs2_l = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_l;
+  const double cs2_r = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_r;
+  const double cs_l = sqrt(cs2_l), cs_r = sqrt(cs2_r);
+  const double ct_l = cs2_l / fmax(cs_l, Vc_l);
+  const double ct_r = cs2_r / fmax(cs_r, -Vc_r);
+  const double c12 = fmin(ct_l, ct_r);
+  const double M_l = Vc_l / c12, M_r = Vc_r / c12;
+  double m4p, m4m, p5p, p5m;
+  if (fabs(M_l) >= 1.0) {
+    m4p = 0.5 * (M_l + fabs(M_l));
+    p5p = 0.5 * (1.0 + fsign1(M_l));
+  } else {
+    const double t = M_l * M_l - 1.0;
+    m4p = 0.25 * (M_l + 1.0) * (M_l + 1.0) + beta * t * t;
+    p5p = 0.25 * (M_l + 1.0) * (M_l + 1.0) * (2.0 - M_l) + alpha * M_l * t * t;
+  }
+  if (fabs(M_r) >= 1.0) {
+    m4m = 0.5 * (M_r - fabs(M_r));
+    p5m = 0.5 * (1.0 - fsign1(M_r));
+  } else {
+    const double t = M_r * M_r - 1.0;
+    m4m = -0.25 * (M_r - 1.0) * (M_r - 1.0) - beta * t * t;
+    p5m = 0.25 * (M_r - 1.0) * (M_r - 1.0) * (2.0 + M_r) - alpha * M_r * t * t;
+  }
+  const double m12 = m4p + m4m;
+  const double mpos = 0.5 * (m12 + fabs(m12)), mneg = 0.5 * (m12 - fabs(m12));
+  const double p12 = p5p * p_l + p5m * p_r;
+  F[0] = c12 * (mpos * rho_l * 1.0 + mneg * rho_r * 1.0);
+#pragma unroll
+  for (int m = 0; m < NC - 2; ++m) {
+    const double f = c12 * (mpos * rho_l * vl[m] + mneg * rho_r * vr[m]);
+    F[1 + m] = (m == D) ? f + p12 * 1.0 : f + p12 * 0.0;
+  }
+  F[NC - 1] = c12 * (mpos * rho_l * H_l + mneg * rho_r * H_r) + p12 * 0.0;
+}
+
+}  // namespace fvm

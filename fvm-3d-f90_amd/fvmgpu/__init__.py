@@ -1,0 +1,185 @@
+"""Python host side of the MI355X residual + RK path.
+
+A thin ctypes mirror of include/fvm_gpu.h.  The compute runs in
+lib/libfvmgpu.so (HIP, gfx950); there is no CPU fallback: importing this
+module on a machine without the built library raises, and so does creating a
+context without a GPU.
+
+Scheme codes are the reference's (src/defs/type_defs_scheme.f90:11-36).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import cases  # noqa: F401  (re-export)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfvmgpu.so")
+
+RK_SSP3, RK4 = 101, 1001
+SIMPLE, MUSCL, WENO = 200, 201, 202
+LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
+ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
+BC_EXTRAP, BC_PERIODIC = 0, 1
+
+# scheme names as the case file spells them (type_defs_scheme.f90:41-84)
+SCHEME_INDEX = {
+    "Runge-Kutta": RK_SSP3, "RK4": RK4,
+    "simple": SIMPLE, "MUSCL": MUSCL, "WENO": WENO,
+    "none_": LIM_NONE, "minmod_": LIM_MINMOD, "VanLeer_": LIM_VANLEER,
+    "vanAlbada_": LIM_VANALBADA,
+    "Roe": ROE, "VanLeer": VANLEER, "AUSM": AUSM, "AUSM+": AUSMPLUS,
+}
+
+EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
+           "fvm_scheme_set", "fvm_block_create", "fvm_block_destroy", "fvm_block_set_bc",
+           "fvm_upload", "fvm_download", "fvm_step", "fvm_norms", "fvm_sync",
+           "fvm_timing_enable", "fvm_timing_read")
+
+
+class FvmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libfvmgpu.so and declare its signatures (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FvmError(f"{path} is not built: run `make -C fvm-3d-f90_amd` "
+                       "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    c_void_pp = ctypes.POINTER(ctypes.c_void_p)
+    i64, ci, cd, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    sig = {
+        "fvm_last_error": ([], ctypes.c_char_p),
+        "fvm_version": ([], ctypes.c_char_p),
+        "fvm_ctx_create": ([ci, c_void_pp], ci),
+        "fvm_ctx_destroy": ([vp], ci),
+        "fvm_scheme_set": ([vp, ci, ci, ci, ci, cd, cd, cd, ci], ci),
+        "fvm_block_create": ([vp, i64, i64, i64, cd, cd, cd, c_void_pp], ci),
+        "fvm_block_destroy": ([vp], ci),
+        "fvm_block_set_bc": ([vp, ci, ci], ci),
+        "fvm_upload": ([vp, vp, ci], ci),
+        "fvm_download": ([vp, vp, ci], ci),
+        "fvm_step": ([vp, ci, cd, cd, ctypes.POINTER(cd)], ci),
+        "fvm_norms": ([vp, ctypes.POINTER(cd), ctypes.POINTER(ci)], ci),
+        "fvm_sync": ([vp], ci),
+        "fvm_timing_enable": ([vp, ci], ci),
+        "fvm_timing_read": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FvmError(f"fvm error {rc}: {_lib.fvm_last_error().decode()}")
+
+
+class Block:
+    """One structured block: ni x nj x nk real vertices (nk == 1: 2-D)."""
+
+    def __init__(self, ctx, shape, spacing):
+        self.ctx = ctx
+        self.ni, self.nj, self.nk = (int(s) for s in shape)
+        self.dim = 2 if self.nk == 1 else 3
+        h = ctypes.c_void_p()
+        _check(_lib.fvm_block_create(ctx._h, self.ni, self.nj, self.nk, float(spacing[0]),
+                                     float(spacing[1]), float(spacing[2]), ctypes.byref(h)))
+        self._h = h
+
+    def set_bc(self, face, kind):
+        _check(_lib.fvm_block_set_bc(self._h, face, kind))
+
+    def set_periodic(self, dims=(0, 1, 2)):
+        for d in dims:
+            if d < self.dim:
+                self.set_bc(2 * d, BC_PERIODIC)
+
+    def host_shape(self, ng):
+        nz = self.nk + 2 * ng if self.dim == 3 else 1
+        return (5, nz, self.nj + 2 * ng, self.ni + 2 * ng)
+
+    def upload(self, q, ng=0):
+        """q: (5, nk(+2ng), nj+2ng, ni+2ng) float64 C-order, i.e. the memory
+        image of the Fortran array q(i, j, k, eq)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != self.host_shape(ng):
+            raise FvmError(f"upload shape {q.shape} != {self.host_shape(ng)}")
+        _check(_lib.fvm_upload(self._h, q.ctypes.data, ng))
+
+    def download(self, ng=0):
+        q = np.zeros(self.host_shape(ng))
+        _check(_lib.fvm_download(self._h, q.ctypes.data, ng))
+        return q
+
+
+class Context:
+    """A device context: one stream and the blocks created on it."""
+
+    def __init__(self, device=0):
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.fvm_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.blocks = []
+
+    def close(self):
+        if self._h:
+            _lib.fvm_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scheme(self, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=RK_SSP3, kappa=0.33333,
+               entropy_fix=0.2, gamma=1.4, ref_compat=1):
+        _check(_lib.fvm_scheme_set(self._h, recon, limiter, flux, rk, kappa, entropy_fix, gamma,
+                                   ref_compat))
+
+    def block(self, shape, spacing):
+        b = Block(self, shape, spacing)
+        self.blocks.append(b)
+        return b
+
+    def step(self, nsteps=1, dt=0.0, cfl=0.0):
+        """Advance every block; dt > 0 fixed, else CFL.  Returns the last dt."""
+        out = ctypes.c_double(0.0)
+        _check(_lib.fvm_step(self._h, int(nsteps), float(dt), float(cfl), ctypes.byref(out)))
+        return out.value
+
+    def norms(self):
+        out = (ctypes.c_double * 6)()
+        flag = ctypes.c_int(0)
+        _check(_lib.fvm_norms(self._h, out, ctypes.byref(flag)))
+        return np.array(out[:]), bool(flag.value)
+
+    def sync(self):
+        _check(_lib.fvm_sync(self._h))
+
+    def timing(self, enable=True):
+        _check(_lib.fvm_timing_enable(self._h, int(enable)))
+
+    def timing_read(self):
+        ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        _check(_lib.fvm_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cu)))
+        return ms.value, n.value, cu.value

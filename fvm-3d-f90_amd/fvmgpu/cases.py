@@ -1,0 +1,136 @@
+"""Initial states of the benchmark cases (host side, numpy).
+
+All generators return primitive variables p of shape (nk, nj, ni, 5) in the
+order (rho, u, v, w, p) on the real vertices, and `to_conserved` gives the
+(5, nk, nj, ni) conserved array in the memory image of the Fortran
+q(i, j, k, eq) that fvm_upload expects (host_ng = 0).
+"""
+import numpy as np
+
+
+def to_conserved(p, gamma=1.4):
+    """p -> q, src/solve/class_flow_field.f90:353-377 (same operation order)."""
+    rho, u, v, w, pr = (p[..., m] for m in range(5))
+    q = np.empty((5,) + p.shape[:-1])
+    q[0] = rho
+    q[1] = rho * u
+    q[2] = rho * v
+    q[3] = rho * w
+    q[4] = pr / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return q
+
+
+def to_primitive(q, gamma=1.4):
+    """q -> p, src/solve/class_flow_field.f90:383-396."""
+    rho = q[0]
+    u, v, w = q[1] / rho, q[2] / rho, q[3] / rho
+    p = np.empty(q.shape[1:] + (5,))
+    p[..., 0] = rho
+    p[..., 1] = u
+    p[..., 2] = v
+    p[..., 3] = w
+    p[..., 4] = (gamma - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w))
+    return p
+
+
+def nodal_coords(n, length):
+    """Vertices x = dx * (i - 1), dx = L / (N - 1) (src/base/method_gridgen.f90:120-131)."""
+    dx = length / float(n - 1)
+    return dx, dx * np.arange(n, dtype=np.float64)
+
+
+# Four-quadrant states of the reference's hard-coded case, in the order the
+# regions are written (src/solve/class_flow_field.f90:207-243): region 1 is
+# the upper-right quadrant x, y in [0.8, 1].
+REF_QUADRANTS = (
+    ((0.8, 1.0), (0.8, 1.0), (1.5, 0.0, 0.0, 0.0, 1.5)),
+    ((0.0, 0.8), (0.8, 1.0), (0.5323, 1.206, 0.0, 0.0, 0.3)),
+    ((0.0, 0.8), (0.0, 0.8), (0.138, 1.206, 1.206, 0.0, 0.029)),
+    ((0.8, 1.0), (0.0, 0.8), (0.5323, 0.0, 1.206, 0.0, 0.3)),
+)
+
+
+def riemann2d(n, seed=0, mid=0.8):
+    """2-D four-quadrant Riemann problem on the unit square, n x n vertices.
+
+    seed 0: the reference's states (Lax-Liu configuration 3).  Regions are
+    applied in order with inclusive bounds (src/base/defs_block_data.f90:
+    166-201), so later regions win on the x = 0.8 / y = 0.8 lines.
+    seed >= 1: each quadrant draws rho in U[0.1, 2], p in U[0.02, 1.5],
+    u, v in U[-1.3, 1.3] from numpy's PCG64 stream of that seed.
+    Returns (p, dx)."""
+    dx, x = nodal_coords(n, 1.0)
+    if seed == 0:
+        quads = REF_QUADRANTS
+        xm = ym = mid
+    else:
+        rng = np.random.default_rng(seed)
+        xm = ym = mid
+        quads = []
+        for (xr, yr) in (((xm, 1.0), (ym, 1.0)), ((0.0, xm), (ym, 1.0)),
+                         ((0.0, xm), (0.0, ym)), ((xm, 1.0), (0.0, ym))):
+            rho = rng.uniform(0.1, 2.0)
+            pr = rng.uniform(0.02, 1.5)
+            u, v = rng.uniform(-1.3, 1.3, 2)
+            quads.append((xr, yr, (rho, u, v, 0.0, pr)))
+    p = np.zeros((1, n, n, 5))
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    for (xa, xb), (ya, yb), state in quads:
+        m = (X >= xa) & (X <= xb) & (Y >= ya) & (Y <= yb)
+        p[0][m] = state
+    return p, dx
+
+
+def isentropic_vortex(n, seed=0, length=10.0, beta=5.0, gamma=1.4, u_inf=(1.0, 1.0, 0.0),
+                      center=None):
+    """Yee's isentropic vortex extruded along z on a periodic [0, L]^3 box of
+    n^3 vertices, x = i h with h = L / n (vertex n is vertex 0).  Free stream
+    rho = p = 1.  Returns (p, h, centre)."""
+    h = length / n
+    x = h * np.arange(n, dtype=np.float64)
+    if center is None:
+        if seed == 0:
+            center = (0.5 * length, 0.5 * length)
+        else:
+            rng = np.random.default_rng(seed)
+            center = tuple(rng.uniform(0.3 * length, 0.7 * length, 2))
+    p = np.empty((n, n, n, 5))
+    prim2 = vortex_2d(x, x, center, length, beta, gamma, u_inf)
+    p[:] = prim2[None]
+    return p, h, center
+
+
+def vortex_2d(x, y, center, length, beta, gamma, u_inf, t=0.0):
+    """Exact vortex state at time t (translated with the free stream, wrapped
+    periodically), shape (ny, nx, 5)."""
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    xc = (center[0] + u_inf[0] * t) % length
+    yc = (center[1] + u_inf[1] * t) % length
+    dxv = (X - xc + 0.5 * length) % length - 0.5 * length
+    dyv = (Y - yc + 0.5 * length) % length - 0.5 * length
+    r2 = dxv * dxv + dyv * dyv
+    e = np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (gamma - 1.0) * beta * beta / (8.0 * gamma * np.pi * np.pi) * e * e
+    rho = T ** (1.0 / (gamma - 1.0))
+    out = np.empty(X.shape + (5,))
+    out[..., 0] = rho
+    out[..., 1] = u_inf[0] - beta / (2.0 * np.pi) * e * dyv
+    out[..., 2] = u_inf[1] + beta / (2.0 * np.pi) * e * dxv
+    out[..., 3] = u_inf[2]
+    out[..., 4] = rho ** gamma
+    return out
+
+
+def smooth_3d(shape, seed=0, amp=0.2):
+    """Smooth positive random field for parity tests: (nk, nj, ni, 5)."""
+    nk, nj, ni = shape
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.linspace(0.0, 1.0, m) for m in (nk, nj, ni)), indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    p = np.empty((nk, nj, ni, 5))
+    p[..., 0] = 1.0 + amp * np.sin(2 * np.pi * x + ph[0]) * np.cos(2 * np.pi * y + ph[1]) + 0.1 * z
+    p[..., 1] = 0.3 * np.sin(2 * np.pi * y + ph[2])
+    p[..., 2] = 0.2 * np.cos(2 * np.pi * z + ph[3])
+    p[..., 3] = 0.25 * np.sin(2 * np.pi * x + ph[4])
+    p[..., 4] = 1.0 + amp * np.cos(2 * np.pi * (x + y + z) + ph[5])
+    return p

@@ -1,0 +1,140 @@
+/*
+ * fvm_gpu.h -- C ABI of the MI355X explicit residual + Runge-Kutta path.
+ *
+ * Drop-in for the solver's RK-stage call of FVM-3D-F90: the Fortran driver
+ * (or any C/Python host) keeps case input, mesh and scheme selection, uploads
+ * the conserved state once, and calls fvm_step / fvm_stage where the
+ * reference calls ClassTimeDiscret%Runge_Kutta_sub
+ * (src/solve/class_time_discret.f90:159-196, called from solve_sub :146 in
+ * the time loop src/process/solver_sub.f90:196-206).  State stays in HBM
+ * between calls.
+ *
+ * Conventions
+ *   - Every call returns 0 on success and a negative FVM_E* code on failure;
+ *     fvm_last_error() gives the text (the reference's ClassError flags plus
+ *     mpi%stop(msg), src/core/global_error.f90:82-146,
+ *     src/core/parallel_mpi.f90:111-133, become return codes; nothing
+ *     aborts the process).
+ *   - Host arrays are Fortran column-major q(i, j, k, eq) with eq = rho,
+ *     rho*u, rho*v, rho*w, rho*E (the equation order of
+ *     src/process/solver_sub.f90:131-136) and host_ng ghost layers on each
+ *     side of every swept direction (the reference allocates 3,
+ *     src/base/defs_ghost_cell.f90:42).  A 2-D block (nk == 1) has no k
+ *     ghosts on the host; its rho*w must be identically zero.
+ *   - Scheme codes are the reference's (src/defs/type_defs_scheme.f90:11-36);
+ *     codes >= 1000 are additions with no reference counterpart.
+ *   - One host thread per context.  Contexts are independent.
+ */
+#ifndef FVM_GPU_H
+#define FVM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- scheme codes ------------------------------------------------------ */
+#define FVM_RK_SSP3        101   /* "Runge-Kutta": 3-stage Shu-Osher SSP-RK3  */
+#define FVM_RK4            1001  /* classical RK4 (no reference counterpart)   */
+#define FVM_SIMPLE         200
+#define FVM_MUSCL          201
+#define FVM_WENO           202
+#define FVM_LIM_NONE       250
+#define FVM_LIM_MINMOD     251
+#define FVM_LIM_VANLEER    252
+#define FVM_LIM_VANALBADA  253
+#define FVM_ROE            300
+#define FVM_VANLEER        301
+#define FVM_AUSM           303
+#define FVM_AUSMPLUS       1303  /* Liou's AUSM+ (no reference counterpart)    */
+
+/* ---- boundary kinds (fvm_block_set_bc) --------------------------------- */
+#define FVM_BC_EXTRAP      0     /* zeroth-order ghost copy, the reference's only BC */
+#define FVM_BC_PERIODIC    1     /* nodal periodic: vertex n is vertex 0       */
+
+/* ---- error codes ------------------------------------------------------- */
+#define FVM_OK             0
+#define FVM_EINVAL        -1     /* bad argument                              */
+#define FVM_EHIP          -2     /* HIP runtime error                         */
+#define FVM_ENOMEM        -3     /* device allocation failed                  */
+#define FVM_ESTATE        -4     /* call out of order (e.g. no scheme set)     */
+#define FVM_ENAN          -5     /* NaN in the CFL wave speed                 */
+
+typedef struct fvm_ctx fvm_ctx;
+typedef struct fvm_block fvm_block;
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *fvm_last_error(void);
+
+/* Library version string. */
+const char *fvm_version(void);
+
+/* Context on HIP device `device`: owns one stream and every block created on
+ * it.  Replaces the solver singletons of src/solve/global_class.f90. */
+int fvm_ctx_create(int device, fvm_ctx **out);
+int fvm_ctx_destroy(fvm_ctx *ctx);
+
+/* Scheme selection, ClassSpaceDiscret%input / ClassTimeDiscret%input
+ * (src/solve/class_space_discret.f90:72-94,
+ *  src/solve/class_time_discret.f90:66-82):
+ *   recon   FVM_SIMPLE | FVM_MUSCL | FVM_WENO
+ *   limiter FVM_LIM_*  (MUSCL only)
+ *   flux    FVM_ROE | FVM_VANLEER | FVM_AUSM | FVM_AUSMPLUS
+ *   rk      FVM_RK_SSP3 | FVM_RK4
+ *   kappa, entropy_fix as read from the case file (a missing key reads 0.0)
+ *   ref_compat 1: WENO keeps the reference's IS_1 and unmirrored right state
+ *              (src/scheme/scheme_WENO.f90:54-56,107-108); 0: textbook WENO5-JS */
+int fvm_scheme_set(fvm_ctx *ctx, int recon, int limiter, int flux, int rk,
+                   double kappa, double entropy_fix, double gamma, int ref_compat);
+
+/* One structured block of ni x nj x nk real vertices (nk == 1: 2-D, swept
+ * along i and j only), vertex spacing dx, dy, dz (the reference's uniform
+ * cell_size, src/base/method_gridgen.f90:120-131).  Replaces
+ * ClassBlockData%allocate_memory (src/base/defs_block_data.f90:111-163). */
+int fvm_block_create(fvm_ctx *ctx, int64_t ni, int64_t nj, int64_t nk,
+                     double dx, double dy, double dz, fvm_block **out);
+int fvm_block_destroy(fvm_block *blk);
+
+/* Boundary kind of one block face: face 0..5 = i-, i+, j-, j+, k-, k+.
+ * Default FVM_BC_EXTRAP (src/base/defs_block_data.f90:226-256). */
+int fvm_block_set_bc(fvm_block *blk, int face, int kind);
+
+/* Host <-> device copies of the conserved state (host layout above).  Upload
+ * also fills the device ghost layers (ClassFlowField%initial's ghost update,
+ * src/solve/class_flow_field.f90:249-253). */
+int fvm_upload(fvm_block *blk, const double *q, int host_ng);
+int fvm_download(fvm_block *blk, double *q, int host_ng);
+
+/* nsteps full time steps on every block of the context
+ * (ClassTimeDiscret%solve, src/solve/class_time_discret.f90:125-156).
+ *   dt > 0         fixed time step (the reference's "fixed" timestep_set)
+ *   dt <= 0        global CFL step dt = cfl / max sum_d (|u_d|+c)/dr_d,
+ *                  recomputed from the state at the start of every step
+ *                  (no reference counterpart: "CFL" is unimplemented there)
+ * dt_last (nullable) receives the last step's dt.  Residual norms of the
+ * last stage of the last step are kept for fvm_norms. */
+int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
+
+/* Residual norms of the last stage of the last fvm_step call, as
+ * ClassIteration%compute_residual computes them
+ * (src/solve/class_iteration.f90:210-248): out[0..4] per equation,
+ * out[5] total; nan_flag = 1 if any is NaN or above 1e10
+ * (converged_fun's abort tests, :183-191). */
+int fvm_norms(fvm_ctx *ctx, double out[6], int *nan_flag);
+
+/* Stage-kernel timing with HIP events on the context's stream (no
+ * reference counterpart; the reference only wall-clocks the whole solve,
+ * src/solve/class_output.f90:266-334).  Enabling resets the counters;
+ * fvm_timing_read returns the summed stage-kernel milliseconds, the number of
+ * stage launches and the cell updates they performed. */
+int fvm_timing_enable(fvm_ctx *ctx, int enable);
+int fvm_timing_read(fvm_ctx *ctx, double *stage_ms, int64_t *launches, double *cell_updates);
+
+/* Wait for all work of the context. */
+int fvm_sync(fvm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM_GPU_H */
