@@ -37,7 +37,22 @@ def parse():
     ap.add_argument("--cfl", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (scale reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy kernel of known size: 2^27 doubles (1 GiB) read and written
+    once per launch, 5 launches."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": x.numel() * 8, "launches": 5}))
 
 
 def cpu_baseline(p, h, args):
@@ -58,6 +73,9 @@ def cpu_baseline(p, h, args):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)

@@ -418,6 +418,23 @@ void orc_runge_kutta(int m, double dt, int64_t n, double *u0, double *um,
     }
 }
 
+/* Classical RK4 with an accumulator; no reference counterpart (the
+ * reference fixes total_RK_stage = 3, src/solve/class_time_discret.f90:29). */
+void orc_rk4_stage(int s, double dt, int64_t n, const double *u0, double *um,
+                   double *acc, const double *rm)
+{
+    const double dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
+    for (int64_t x = 0; x < n; ++x) {
+        double r = rm[x];
+        switch (s) {
+        case 1: acc[x] = u0[x] + dt6 * r; um[x] = u0[x] + dt2 * r; break;
+        case 2: acc[x] = acc[x] + dt3 * r; um[x] = u0[x] + dt2 * r; break;
+        case 3: acc[x] = acc[x] + dt3 * r; um[x] = u0[x] + dt * r; break;
+        default: um[x] = acc[x] + dt6 * r; break;
+        }
+    }
+}
+
 /* ======================================================================== */
 /*  Whole-block solver                                                       */
 /* ======================================================================== */
@@ -677,22 +694,12 @@ int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl
         double dt = dt_fix;
         if (dt_fix <= 0.0) dt = orc_cfl_dt(n, c->dim, q0, c->gamma, c->dr, cfl);
         if (dt_out) dt_out[step] = dt;
-        const double dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
         for (int s = 1; s <= nstage; ++s) {
             memset(rhs, 0, sizeof(double) * NEQ * nreal);
             inviscid_residual(&b, c, rhs);
             gather_real(&b, qm);
             if (c->rk == ORC_RK4) {
-                /* classical RK4 with an accumulator; no reference counterpart */
-                for (int64_t x = 0; x < NEQ * nreal; ++x) {
-                    double r = rhs[x];
-                    switch (s) {
-                    case 1: acc[x] = q0[x] + dt6 * r; qm[x] = q0[x] + dt2 * r; break;
-                    case 2: acc[x] = acc[x] + dt3 * r; qm[x] = q0[x] + dt2 * r; break;
-                    case 3: acc[x] = acc[x] + dt3 * r; qm[x] = q0[x] + dt * r; break;
-                    default: qm[x] = acc[x] + dt6 * r; break;
-                    }
-                }
+                orc_rk4_stage(s, dt, NEQ * nreal, q0, qm, acc, rhs);
             } else {
                 orc_runge_kutta(s, dt, NEQ * nreal, q0, qm, rhs);
             }

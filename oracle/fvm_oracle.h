@@ -46,6 +46,11 @@ void orc_flux_ausmplus(int64_t nface, int neq, double gamma, const double *norm,
 void orc_runge_kutta(int m, double dt, int64_t n, double *u0, double *um,
                      const double *rm);
 
+/* Classical RK4 stage s = 1..4 with an accumulator (no reference
+ * counterpart); the combine orc_solve applies for rk = ORC_RK4. */
+void orc_rk4_stage(int s, double dt, int64_t n, const double *u0, double *um,
+                   double *acc, const double *rm);
+
 /* ---- whole-block solver ------------------------------------------------ */
 typedef struct orc_config {
     int dim;              /* 2: sweep i,j (k planes independent); 3: i,j,k   */

@@ -59,6 +59,8 @@ def oracle():
         lib.orc_solve.restype = ci
         lib.orc_cfl_dt.argtypes = [P, ci, P, cd, P, cd]
         lib.orc_cfl_dt.restype = cd
+        lib.orc_rk4_stage.argtypes = [ci, cd, i64, P, P, P, P]
+        lib.orc_rk4_stage.restype = None
         _oracle = lib
     return _oracle
 
@@ -170,3 +172,20 @@ def ref_solve(p_real, idim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, kappa
     lib.ref_solve(ni, nj, nk, idim, dr[0], dr[1], dr[2], recon, limiter, flux, kappa, entfix,
                   gamma, dt, nsteps, p_from_q, _ptr(p_real), _ptr(q), _ptr(rhs), _ptr(res))
     return {"q": q, "rhs": rhs, "res": res[:nsteps]}
+
+
+def rk4_stage(s, dt, u0, um, acc, rm):
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    um = np.array(um, dtype=np.float64)
+    acc = np.array(acc, dtype=np.float64)
+    rm = np.ascontiguousarray(rm, dtype=np.float64)
+    oracle().orc_rk4_stage(s, dt, u0.size, _ptr(u0), _ptr(um), _ptr(acc), _ptr(rm))
+    return um, acc
+
+
+def cfl_dt(q_real, dim, dr, gamma=1.4, cfl=1.0):
+    """q_real: (5, nk, nj, ni)."""
+    q_real = np.ascontiguousarray(q_real, dtype=np.float64)
+    n = np.array(q_real.shape[1:][::-1], dtype=np.int64)
+    dr = np.ascontiguousarray(dr, dtype=np.float64)
+    return oracle().orc_cfl_dt(_ptr(n), dim, _ptr(q_real), gamma, _ptr(dr), cfl)
