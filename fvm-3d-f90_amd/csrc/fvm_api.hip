@@ -7,14 +7,17 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/fvm_gpu.h"
 #include "fvm_launch.hpp"
+#include "fvm_stage.hpp"
 
 using namespace fvm;
+using namespace fvm::exact;
 
 namespace {
 
@@ -53,7 +56,7 @@ struct fvm_block {
   size_t state_elems = 0; // nc * cs
   bool uploaded = false;
   // launch shape
-  int ty = 4, chunk = 16;
+  int tile = 0, chunk = 16;
   int tiles_x = 0, tiles_y = 0, tiles_z = 0, nblocks = 0;
 };
 
@@ -61,6 +64,7 @@ struct fvm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool scheme_set = false;
+  bool fast = false;       // FVM_NUMERICS_FAST: FMA/reciprocal kernel family
   int recon = 0, flux = 0, rk = 0;
   Params P{};
   std::vector<fvm_block*> blocks;
@@ -161,23 +165,37 @@ int set_device(fvm_ctx* ctx) {
   return FVM_OK;
 }
 
-// Launch shape of a block: 3-D tiles of 62 x ty columns marching over
-// `chunk` planes; 2-D rows of 62 columns marching over `chunk` rows.
-void plan_launch(fvm_block* b) {
+// Launch shape of a block.  3-D: tile (TX x TY columns, kTile3d) marching
+// over `chunk` planes -- TX is 64 unless 32 wastes fewer columns, the chunk
+// is the longest (<= 32 planes) that still gives >= 512 workgroups.  2-D: rows
+// of 62 columns marching over `chunk` rows.  FVM_TILE / FVM_CHUNK override.
+void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   const BlockGeom& g = b->g;
-  b->tiles_x = (g.nx + 61) / 62;
+  const char* et = getenv("FVM_TILE");
+  const char* ec = getenv("FVM_CHUNK");
+  char tbuf[16], cbuf[16];
+  if (force_tile >= 0) { snprintf(tbuf, sizeof tbuf, "%d", force_tile); et = tbuf; }
+  if (force_chunk > 0) { snprintf(cbuf, sizeof cbuf, "%d", force_chunk); ec = cbuf; }
   if (b->dim == 2) {
+    b->tiles_x = (g.nx + 61) / 62;
     b->tiles_y = 1;
-    // keep enough workgroups to fill the device on small grids
     int chunk = 16;
     while (chunk > 2 && (int64_t)b->tiles_x * ((g.ny + chunk - 1) / chunk) < 2048) chunk /= 2;
+    if (ec) chunk = std::max(1, atoi(ec));
     b->chunk = chunk;
     b->tiles_z = (g.ny + chunk - 1) / chunk;
   } else {
-    b->tiles_y = (g.ny + b->ty - 1) / b->ty;
+    const int w64 = (g.nx + 63) / 64 * 64 - g.nx, w32 = (g.nx + 31) / 32 * 32 - g.nx;
+    int tile = w32 <= w64 ? 1 : 0;
+    if (et) tile = std::min(std::max(atoi(et), 0), kNumTiles3d - 1);
+    b->tile = tile;
+    const int tx = kTile3d[tile][0], ty = kTile3d[tile][1];
+    b->tiles_x = (g.nx + tx - 1) / tx;
+    b->tiles_y = (g.ny + ty - 1) / ty;
     int chunk = 32;
-    while (chunk > 4 && (int64_t)b->tiles_x * b->tiles_y * ((g.nz + chunk - 1) / chunk) < 2048)
+    while (chunk > 8 && (int64_t)b->tiles_x * b->tiles_y * ((g.nz + chunk - 1) / chunk) < 512)
       chunk /= 2;
+    if (ec) chunk = std::max(1, atoi(ec));
     b->chunk = chunk;
     b->tiles_z = (g.nz + chunk - 1) / chunk;
   }
@@ -204,21 +222,43 @@ int fill_ghosts(fvm_block* b, double* q) {
   return FVM_OK;
 }
 
-int launch_stage(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
-  hipError_t e = hipErrorInvalidValue;
+template <class Fam>
+hipError_t launch_family(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
   if (b->dim == 2) {
     switch (ctx->recon) {
-      case RECON_SIMPLE: e = launch_stage2d_simple(ctx->flux, a, b->nblocks, ctx->stream); break;
-      case RECON_MUSCL: e = launch_stage2d_muscl(ctx->flux, a, b->nblocks, ctx->stream); break;
-      case RECON_WENO: e = launch_stage2d_weno(ctx->flux, a, b->nblocks, ctx->stream); break;
+      case RECON_SIMPLE: return Fam::s2(ctx->flux, a, b->nblocks, ctx->stream);
+      case RECON_MUSCL: return Fam::m2(ctx->flux, a, b->nblocks, ctx->stream);
+      case RECON_WENO: return Fam::w2(ctx->flux, a, b->nblocks, ctx->stream);
     }
   } else {
     switch (ctx->recon) {
-      case RECON_SIMPLE: e = launch_stage3d_simple(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
-      case RECON_MUSCL: e = launch_stage3d_muscl(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
-      case RECON_WENO: e = launch_stage3d_weno(ctx->flux, b->ty, a, b->nblocks, ctx->stream); break;
+      case RECON_SIMPLE: return Fam::s3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
+      case RECON_MUSCL: return Fam::m3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
+      case RECON_WENO: return Fam::w3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
     }
   }
+  return hipErrorInvalidValue;
+}
+
+struct ExactFam {
+  static constexpr auto s2 = fvm::exact::launch_stage2d_simple;
+  static constexpr auto m2 = fvm::exact::launch_stage2d_muscl;
+  static constexpr auto w2 = fvm::exact::launch_stage2d_weno;
+  static constexpr auto s3 = fvm::exact::launch_stage3d_simple;
+  static constexpr auto m3 = fvm::exact::launch_stage3d_muscl;
+  static constexpr auto w3 = fvm::exact::launch_stage3d_weno;
+};
+struct FastFam {
+  static constexpr auto s2 = fvm::fast::launch_stage2d_simple;
+  static constexpr auto m2 = fvm::fast::launch_stage2d_muscl;
+  static constexpr auto w2 = fvm::fast::launch_stage2d_weno;
+  static constexpr auto s3 = fvm::fast::launch_stage3d_simple;
+  static constexpr auto m3 = fvm::fast::launch_stage3d_muscl;
+  static constexpr auto w3 = fvm::fast::launch_stage3d_weno;
+};
+
+int launch_stage(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
+  const hipError_t e = ctx->fast ? launch_family<FastFam>(ctx, b, a) : launch_family<ExactFam>(ctx, b, a);
   if (e != hipSuccess) return fail(FVM_EHIP, "stage launch: %s", hipGetErrorString(e));
   return FVM_OK;
 }
@@ -293,6 +333,14 @@ int fvm_scheme_set(fvm_ctx* ctx, int recon, int limiter, int flux, int rk, doubl
   return FVM_OK;
 }
 
+int fvm_set_numerics(fvm_ctx* ctx, int mode) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (mode != FVM_NUMERICS_EXACT && mode != FVM_NUMERICS_FAST)
+    return fail(FVM_EINVAL, "unknown numerics mode %d", mode);
+  ctx->fast = mode == FVM_NUMERICS_FAST;
+  return FVM_OK;
+}
+
 int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
                      double dz, fvm_block** out) {
   if (int rc = check_ctx(ctx)) return rc;
@@ -318,6 +366,7 @@ int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx
   const int64_t planes = dim == 3 ? nk + 2 * kGhost : 1;
   g.cs = (g.pxy * planes + 63) / 64 * 64;   // 512-byte aligned components
   g.dx = dx; g.dy = dy; g.dz = dim == 3 ? dz : 1.0;
+  g.rdx = 1.0 / g.dx; g.rdy = 1.0 / g.dy; g.rdz = 1.0 / g.dz;
   for (int f = 0; f < 6; ++f) g.bc[f] = BC_EXTRAP;
   g.ng = kGhost;
   b->state_elems = (size_t)b->nc * g.cs;
@@ -331,7 +380,6 @@ int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx
     }
     HIP_TRY(hipMemsetAsync(*p, 0, bytes, ctx->stream));
   }
-  b->ty = 4;
   plan_launch(b);
   ctx->blocks.push_back(b);
   *out = b;
@@ -579,6 +627,60 @@ int fvm_norms(fvm_ctx* ctx, double out[6], int* nan_flag) {
   if (!ctx->have_norms) return fail(FVM_ESTATE, "no step has run");
   if (out) for (int e = 0; e < 6; ++e) out[e] = ctx->norms[e];
   if (nan_flag) *nan_flag = ctx->nan_flag;
+  return FVM_OK;
+}
+
+// Pick each block's launch shape by timing one stage launch per candidate
+// (3-D: every compiled tile x chunk in {4, 8, 16, 32}; 2-D: chunk in
+// {2, 4, 8, 16}).  The probe is the scheme's first stage, which reads the
+// current state and writes only the stage buffers, so the state is untouched.
+int fvm_autotune(fvm_ctx* ctx, int reps) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (reps < 1) reps = 1;
+  if (int rc = set_device(ctx)) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  for (fvm_block* b : ctx->blocks) {
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
+    const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
+    float best = 1e30f;
+    int best_tile = b->tile, best_chunk = b->chunk;
+    const int ntiles = b->dim == 3 ? kNumTiles3d : 1;
+    for (int tile = 0; tile < ntiles; ++tile)
+      for (int ci = 0; ci < 4; ++ci) {
+        const int chunk = b->dim == 3 ? chunks3[ci] : chunks2[ci];
+        plan_launch(b, b->dim == 3 ? tile : -1, chunk);
+        StageArgs a{};
+        a.g = b->g; a.P = ctx->P; a.dt = 1.0e-6; a.dt_dev = nullptr;
+        a.tiles_x = b->tiles_x; a.tiles_y = b->tiles_y; a.tiles_z = b->tiles_z;
+        a.chunk = b->chunk;
+        a.qin = b->qn; a.q0 = b->qn; a.qout = b->qa;
+        a.acc = ctx->rk == RK4 ? b->acc : nullptr;
+        a.stage = ctx->rk == RK4 ? ST_RK4_1 : ST_SSP1;
+        if (int rc = launch_stage(ctx, b, a)) return rc;   // warm
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+          if (int rc = launch_stage(ctx, b, a)) return rc;
+        HIP_TRY(hipEventRecord(e1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_tile = b->tile; best_chunk = b->chunk; }
+      }
+    plan_launch(b, b->dim == 3 ? best_tile : -1, best_chunk);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return FVM_OK;
+}
+
+int fvm_block_launch_shape(fvm_block* b, int* tile_x, int* tile_y, int* chunk) {
+  if (!b) return fail(FVM_EINVAL, "null block");
+  if (tile_x) *tile_x = b->dim == 3 ? kTile3d[b->tile][0] : 62;
+  if (tile_y) *tile_y = b->dim == 3 ? kTile3d[b->tile][1] : 1;
+  if (chunk) *chunk = b->chunk;
   return FVM_OK;
 }
 

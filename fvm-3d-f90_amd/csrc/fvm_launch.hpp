@@ -3,18 +3,31 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "fvm_stage.hpp"
+#include "fvm_types.hpp"
 
 namespace fvm {
 
-// 3-D tile heights the library is built with (rows of 62 columns).
-constexpr int kTileRows[] = {2, 4};
+// 3-D tile shapes the library is built with: id -> (TX, TY)
+constexpr int kNumTiles3d = 3;
+constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}};
 
+namespace exact {
 hipError_t launch_stage2d_simple(int flux, const StageArgs& a, int nblocks, hipStream_t s);
 hipError_t launch_stage2d_muscl(int flux, const StageArgs& a, int nblocks, hipStream_t s);
 hipError_t launch_stage2d_weno(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_simple(int flux, int ty, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_muscl(int flux, int ty, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_weno(int flux, int ty, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_simple(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_muscl(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_weno(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+
+}  // namespace exact
+namespace fast {
+hipError_t launch_stage2d_simple(int flux, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage2d_muscl(int flux, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage2d_weno(int flux, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_simple(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_muscl(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage3d_weno(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+
+}  // namespace fast
 
 }  // namespace fvm

@@ -1,32 +1,47 @@
 // fvm_numerics.hpp -- per-point FP64 numerics of the explicit residual path
 // (reconstruction, face fluxes, state conversion) as device functions.
 //
-// Every expression keeps the operation order of the FVM-3D-F90 scheme
-// routines and the file is compiled with -ffp-contract=off, so each result is
-// the same IEEE sequence as the reference (Fortran evaluates a*b*c as (a*b)*c,
-// x**2 as x*x and -a*b as -(a*b)).  Face normals are the unit axis vectors
+// Two families are compiled from this file.  exact (-ffp-contract=off): every
+// expression keeps the operation order of the FVM-3D-F90 scheme routines, so
+// each result is the same IEEE sequence as the reference (Fortran evaluates
+// a*b*c as (a*b)*c, x**2 as x*x and -a*b as -(a*b)).  fast (FVM_FAST=1,
+// -ffp-contract=fast): the same formulas with FMA contraction and
+// reciprocal multiplies in place of repeated divisions; results differ from
+// the reference at round-off and are held to a relative tolerance.  Face normals are the unit axis vectors
 // (src/solve/class_space_discret.f90:458-461); products with the zero normal
 // components are dropped, which changes at most the sign of an exact zero in
 // a face flux and never the residual.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fvm_types.hpp"
+
+#ifndef FVM_FAST
+#define FVM_FAST 0
+#endif
+
 namespace fvm {
+#if FVM_FAST
+namespace fast {
+#else
+namespace exact {
+#endif
+constexpr bool kFast = FVM_FAST != 0;
 
-// ---- scheme codes, src/defs/type_defs_scheme.f90:11-36 -------------------
-enum : int {
-  RK_SSP3 = 101, RK4 = 1001,
-  RECON_SIMPLE = 200, RECON_MUSCL = 201, RECON_WENO = 202,
-  LIM_NONE = 250, LIM_MINMOD = 251, LIM_VANLEER = 252, LIM_VANALBADA = 253,
-  FLUX_ROE = 300, FLUX_VANLEER = 301, FLUX_AUSM = 303, FLUX_AUSMPLUS = 1303,
-};
-
-struct Params {
-  double gamma, kappa, entfix;
-  double g_gm1;   // gamma / (gamma - 1), evaluated once as the reference does per face
-  int limiter;
-  int weno_textbook;
-};
+// Reciprocal.  exact: IEEE division.  fast: the hardware reciprocal estimate
+// refined by two Newton-Raphson steps (error within one ulp for the positive
+// normal operands it sees here: densities, sound speeds, sums of them).
+__device__ __forceinline__ double frcp(double x) {
+  if constexpr (kFast) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  } else {
+    return 1.0 / x;
+  }
+}
 
 // Fortran SIGN(1.0_dp, x)
 __device__ __forceinline__ double fsign1(double x) { return copysign(1.0, x); }
@@ -38,7 +53,23 @@ __device__ __forceinline__ double fmin2(double a, double b) { return (b < a) ? b
 template <int NC>
 __device__ __forceinline__ void cons_to_prim(const double q[NC], double p[NC], double gamma) {
   const double rho = q[0];
-  const double u = q[1] / rho, v = q[2] / rho;
+  double u, v;
+  if constexpr (kFast) {
+    const double rr = frcp(rho);
+    u = q[1] * rr;
+    v = q[2] * rr;
+    if constexpr (NC == 5) {
+      const double w = q[3] * rr;
+      p[0] = rho; p[1] = u; p[2] = v; p[3] = w;
+      p[4] = (gamma - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w));
+    } else {
+      p[0] = rho; p[1] = u; p[2] = v;
+      p[3] = (gamma - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v));
+    }
+    return;
+  }
+  u = q[1] / rho;
+  v = q[2] / rho;
   p[0] = rho; p[1] = u; p[2] = v;
   if constexpr (NC == 5) {
     const double w = q[3] / rho;
@@ -65,11 +96,13 @@ __device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3,
     Db_p = Db_m;
   } else if (limiter == LIM_VANLEER) {
     const double eps = 1.0e-6;
-    Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p / (fabs(D_m) + fabs(D_p) + eps);
+    if constexpr (kFast) Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p * frcp(fabs(D_m) + fabs(D_p) + eps);
+    else Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p / (fabs(D_m) + fabs(D_p) + eps);
     Db_p = Db_m;
   } else if (limiter == LIM_VANALBADA) {
     const double eps = 1.0e-6;
-    Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) / (D_m * D_m + D_p * D_p + eps);
+    if constexpr (kFast) Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) * frcp(D_m * D_m + D_p * D_p + eps);
+    else Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) / (D_m * D_m + D_p * D_p + eps);
     Db_p = Db_m;
   } else {
     Db_m = D_m;
@@ -99,6 +132,12 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
   const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
   const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
   const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
+  if constexpr (kFast) {
+    // alpha_k ~ c_k / e_k^2: scale all three by e0^2 e1^2 e2^2, one division
+    const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+    const double b0 = c_0 * (E1 * E2), b1 = c_1 * (E0 * E2), b2 = c_2 * (E0 * E1);
+    return (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
+  }
   const double a0 = c_0 / (e0 * e0);
   const double a1 = c_1 / (e1 * e1);
   const double a2 = c_2 / (e2 * e2);
@@ -109,6 +148,7 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
 
 // ---- Harten fix, src/scheme/scheme_FDS_Roe.f90:167-177 ----------------------
 __device__ __forceinline__ double entropy_correct(double lambda, double delta) {
+  if constexpr (kFast) return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) * frcp(delta);
   return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) / delta;
 }
 
@@ -130,16 +170,27 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   double ke_l, ke_r;
   if constexpr (NC == 5) { ke_l = u_l * u_l + v_l * v_l + w_l * w_l; ke_r = u_r * u_r + v_r * v_r + w_r * w_r; }
   else { ke_l = u_l * u_l + v_l * v_l; ke_r = u_r * u_r + v_r * v_r; }
-  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+  const double H_l = kFast ? P.g_gm1 * p_l * frcp(rho_l) + 0.5 * ke_l
+                           : P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+  const double H_r = kFast ? P.g_gm1 * p_r * frcp(rho_r) + 0.5 * ke_r
+                           : P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
   const double drho = rho_r - rho_l, du = u_r - u_l, dv = v_r - v_l, dw = w_r - w_l;
   const double dpress = p_r - p_l, dVl = Vc_r - Vc_l;
   const double rho_b = sqrt(rho_l * rho_r);
   const double den = rho_l + rho_b;
-  const double u_b = (rho_l * u_l + rho_b * u_r) / den;
-  const double v_b = (rho_l * v_l + rho_b * v_r) / den;
-  const double w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) / den : 0.0;
-  const double H_b = (rho_l * H_l + rho_b * H_r) / den;
+  double u_b, v_b, w_b, H_b;
+  if constexpr (kFast) {
+    const double rden = frcp(den);
+    u_b = (rho_l * u_l + rho_b * u_r) * rden;
+    v_b = (rho_l * v_l + rho_b * v_r) * rden;
+    w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) * rden : 0.0;
+    H_b = (rho_l * H_l + rho_b * H_r) * rden;
+  } else {
+    u_b = (rho_l * u_l + rho_b * u_r) / den;
+    v_b = (rho_l * v_l + rho_b * v_r) / den;
+    w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) / den : 0.0;
+    H_b = (rho_l * H_l + rho_b * H_r) / den;
+  }
   const double Vl_b = (D == 0) ? u_b : (D == 1) ? v_b : w_b;
   double q2_b;
   if constexpr (NC == 5) q2_b = u_b * u_b + v_b * v_b + w_b * w_b;
@@ -149,9 +200,17 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   const double lam1 = entropy_correct(fabs(Vl_b - c_b), delta);
   const double lam2 = entropy_correct(fabs(Vl_b), delta);
   const double lam5 = entropy_correct(fabs(Vl_b + c_b), delta);
-  const double d1 = lam1 * (dpress - rho_b * c_b * dVl) / (2.0 * c_b * c_b);
-  const double d2 = lam2 * (drho - dpress / (c_b * c_b));
-  const double d5 = lam5 * (dpress + rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+  double d1, d2, d5;
+  if constexpr (kFast) {
+    const double ic2 = frcp(c_b * c_b), hic2 = 0.5 * ic2;
+    d1 = lam1 * (dpress - rho_b * c_b * dVl) * hic2;
+    d2 = lam2 * (drho - dpress * ic2);
+    d5 = lam5 * (dpress + rho_b * c_b * dVl) * hic2;
+  } else {
+    d1 = lam1 * (dpress - rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+    d2 = lam2 * (drho - dpress / (c_b * c_b));
+    d5 = lam5 * (dpress + rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+  }
   const double lr = lam2 * rho_b;
   // mass
   F[0] = 0.5 * ((rho_l * Vc_l + rho_r * Vc_r) - (d1 + d2 + d5));
@@ -203,25 +262,34 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
   double q2_l, q2_r;
   if constexpr (NC == 5) { q2_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; q2_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { q2_l = vl[0] * vl[0] + vl[1] * vl[1]; q2_r = vr[0] * vr[0] + vr[1] * vr[1]; }
-  const double c_l = sqrt(gamma * p_l / rho_l);
-  const double c_r = sqrt(gamma * p_r / rho_r);
-  double M_l = Vc_l / c_l, M_r = Vc_r / c_r;
+  const double c_l = kFast ? sqrt(gamma * p_l * frcp(rho_l)) : sqrt(gamma * p_l / rho_l);
+  const double c_r = kFast ? sqrt(gamma * p_r * frcp(rho_r)) : sqrt(gamma * p_r / rho_r);
+  double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
+  double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
   M_l = (M_l >= 1.0) ? M_l : (M_l <= -1.0) ? 0.0 : 0.25 * (M_l + 1.0) * (M_l + 1.0);
   M_r = (M_r >= 1.0) ? 0.0 : (M_r <= -1.0) ? M_r : -(0.25 * (M_r - 1.0) * (M_r - 1.0));
   const double fp0 = rho_l * c_l * M_l;
   const double fm0 = rho_r * c_r * M_r;
   F[0] = fp0 + fm0;
   const double gp = -Vc_l + 2.0 * c_l, gm = -Vc_r - 2.0 * c_r;
+  const double rg = frcp(gamma);
 #pragma unroll
   for (int m = 0; m < NC - 2; ++m) {
     double a, b;
-    if (m == D) { a = fp0 * (gp / gamma + vl[m]); b = fm0 * (gm / gamma + vr[m]); }
+    if (m == D && kFast) { a = fp0 * (gp * rg + vl[m]); b = fm0 * (gm * rg + vr[m]); }
+    else if (m == D) { a = fp0 * (gp / gamma + vl[m]); b = fm0 * (gm / gamma + vr[m]); }
     else { a = fp0 * (0.0 / gamma + vl[m]); b = fm0 * (0.0 / gamma + vr[m]); }
     F[1 + m] = a + b;
   }
   const double tl = (gamma - 1.0) * Vc_l + 2.0 * c_l;
   const double tr = (gamma - 1.0) * Vc_r - 2.0 * c_r;
   const double den = 2.0 * (gamma * gamma - 1.0);
+  if constexpr (kFast) {
+    const double rden = frcp(den);
+    F[NC - 1] = fp0 * (tl * tl * rden + (q2_l - Vc_l * Vc_l) * 0.5)
+              + fm0 * (tr * tr * rden + (q2_r - Vc_r * Vc_r) * 0.5);
+    return;
+  }
   F[NC - 1] = fp0 * (tl * tl / den + (q2_l - Vc_l * Vc_l) / 2.0)
             + fm0 * (tr * tr / den + (q2_r - Vc_r * Vc_r) / 2.0);
 }
@@ -241,11 +309,13 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   double ke_l, ke_r;
   if constexpr (NC == 5) { ke_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; ke_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
-  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-  const double c_l = sqrt(gamma * p_l / rho_l);
-  const double c_r = sqrt(gamma * p_r / rho_r);
-  const double M_l = Vc_l / c_l, M_r = Vc_r / c_r;
+  const double ir_l = kFast ? frcp(rho_l) : 0.0, ir_r = kFast ? frcp(rho_r) : 0.0;
+  const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+  const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+  const double c_l = kFast ? sqrt(gamma * p_l * ir_l) : sqrt(gamma * p_l / rho_l);
+  const double c_r = kFast ? sqrt(gamma * p_r * ir_r) : sqrt(gamma * p_r / rho_r);
+  const double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
+  const double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
   // Mach split, beta = 0: the beta term adds an exact +0 for |M| < 1
   const double mp = (fabs(M_l) >= 1.0) ? 0.5 * (M_l + 1.0 * fabs(M_l))
                                        : 1.0 * (0.25 * ((M_l + 1.0) * (M_l + 1.0)));
@@ -254,7 +324,9 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   const double Mn = mp + mm;
   const double aMn = fabs(Mn);
   const double delta = 0.1;
-  const double Mabs = (aMn > delta) ? aMn : 0.5 * (aMn * aMn + delta * delta) / delta;
+  const double Mabs = (aMn > delta) ? aMn
+                    : (kFast ? 5.0 * (aMn * aMn + delta * delta)
+                             : 0.5 * (aMn * aMn + delta * delta) / delta);
   // pressure split, alpha = 0: the alpha term adds an exact zero for |M| < 1
   const double pp = ((fabs(M_l) >= 1.0) ? 0.5 * (1.0 + 1.0 * fsign1(M_l))
                                         : 0.25 * ((M_l + 1.0) * (M_l + 1.0)) * (2.0 - 1.0 * M_l)) * p_l;
@@ -326,4 +398,5 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
   F[NC - 1] = c12 * (mpos * rho_l * H_l + mneg * rho_r * H_r) + p12 * 0.0;
 }
 
+}  // namespace exact / fast
 }  // namespace fvm

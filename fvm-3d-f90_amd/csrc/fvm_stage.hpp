@@ -25,45 +25,11 @@
 #include "fvm_numerics.hpp"
 
 namespace fvm {
-
-// ---------------------------------------------------------------- geometry
-struct BlockGeom {
-  int nx, ny, nz;        // real vertices (nz = 1 in 2-D)
-  int ox, oy, oz;        // array offset of real vertex 0 along i, j, k
-  int64_t px, pxy, cs;   // row pitch, plane pitch, component stride (elements)
-  double dx, dy, dz;
-  int bc[6];             // i-, i+, j-, j+, k-, k+ : BC_EXTRAP / BC_PERIODIC
-  int ng;                // ghost layers written per side
-};
-
-enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1 };
-
-// RK stage kinds: the combine each launch applies
-enum : int {
-  ST_SSP1 = 0,   // out = q0 + dt r
-  ST_SSP2 = 1,   // out = (3/4 q0 + 1/4 qin) + (1/4 dt) r
-  ST_SSP3 = 2,   // out = (1/3 q0 + 2/3 qin) + (2/3 dt) r
-  ST_RK4_1 = 3,  // acc = q0 + dt/6 r ; out = q0 + dt/2 r
-  ST_RK4_2 = 4,  // acc = acc + dt/3 r ; out = q0 + dt/2 r
-  ST_RK4_3 = 5,  // acc = acc + dt/3 r ; out = q0 + dt r
-  ST_RK4_4 = 6,  // out = acc + dt/6 r
-};
-
-struct StageArgs {
-  const double* qin;     // stencil input, ghosts valid
-  const double* q0;      // u^n, read at the cell only (may alias qout)
-  double* qout;          // stage output (interior + ghost scatter)
-  double* acc;           // RK4 accumulator (cell only)
-  const double* dt_dev;  // if non-null, dt is read here (CFL mode)
-  double dt;
-  BlockGeom g;
-  Params P;
-  int stage;
-  int tiles_x, tiles_y, tiles_z;  // tile grid; blockIdx.x is remapped over it
-  int chunk;                      // planes (3-D) or rows (2-D) per workgroup
-  double* norm_part;              // per-workgroup sums of rhs^2 (NC each), or null
-  unsigned long long* smax;       // CFL wave-speed max of the output, or null
-};
+#if FVM_FAST
+namespace fast {
+#else
+namespace exact {
+#endif
 
 __device__ __forceinline__ int64_t gidx(const BlockGeom& g, int i, int j, int k) {
   return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
@@ -379,8 +345,14 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
       double out[NC];
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        double r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
-        r = r - (Fn[m] - Fc[m]) / g.dy;
+        double r;
+        if constexpr (kFast) {
+          r = -(Fx[m][t] - Fx[m][t - 1]) * g.rdx;
+          r = r - (Fn[m] - Fc[m]) * g.rdy;
+        } else {
+          r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
+          r = r - (Fn[m] - Fc[m]) / g.dy;
+        }
         const double q0 = a.q0[ci + m * g.cs];
         double* accp = a.acc ? a.acc + ci + m * g.cs : nullptr;
         out[m] = rk_apply(a.stage, kc, q0, a.qin[ci + m * g.cs], accp, r);
@@ -408,182 +380,252 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 }
 
 // ============================================================================
-// 3-D stage: tile TX = 62 by TY rows; NT = 64 * TY threads; march along k.
-// Thread (tx, ty) owns column (i0 - 1 + tx, j0 + ty).
+// 3-D stage: tile of TX (32 or 64) x TY columns, NT = TX * TY threads, every
+// thread owns the column (i0 + tx, j0 + ty) and marches along k.
+//
+// Per plane k:
+//   A  convert the prefetched raw values: plane k (owned vertex + i/j halo)
+//      into the LDS plane buffer, V(k+1+RV) into the owner's k-queue; issue
+//      the loads for plane k+1 and V(k+2+RV) so they fly during the compute
+//   B  vertex reconstructions: i-direction at columns i0-1..i0+TX of the TY
+//      rows, j-direction at rows j0-1..j0+TY of the TX columns (flat task
+//      list: each thread's own two vertices first, then the halo ring)
+//   C  face fluxes: TX+1 i-faces per row, TY+1 j-faces per column (own faces
+//      first, then the far edge), and the owner's k-face k+1 from its queue
+//   D  owner update: residual, RK combine, store with ghost images
 // ============================================================================
-template <class RC, int FLUX, int TY>
-__global__ __launch_bounds__(64 * TY) void stage3d_kernel(StageArgs a) {
-  constexpr int NC = 5, TX = 62, NT = 64 * TY;
+template <class RC, int FLUX, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void stage3d_kernel(StageArgs a) {
+  constexpr int NC = 5, NT = TX * TY;
   constexpr int RV = RC::RV, H = RC::H;
-  constexpr int NW = RV + (RV > 1 ? RV : 1) + 1;
-  constexpr int QOFF = (RV > 1 ? RV : 1);
-  constexpr int VW = 64 + 2 * RV;            // Vs row width: columns i0-1-RV .. i0+TX+RV
-  constexpr int VR = TY + 2 * H;             // Vs rows: j0-H .. j0+TY+H-1
+  constexpr int NW = RV + (RV > 1 ? RV : 1) + 1;   // k-queue V(c - max(RV,1) .. c + RV)
+  constexpr int QOFF = (RV > 1 ? RV : 1);          // queue index of the centre c
+  constexpr int VW = TX + 2 * H, VR = TY + 2 * H, VSN = VW * VR;
   constexpr int NLR = RC::SAME ? 1 : 2;
-  // x-reconstructions [TY][64], y-reconstructions [TY+2][TX]; the face fluxes
-  // reuse the plane buffer once both reconstructions are done.
-  constexpr int LRX = TY * 64, LRY = (TY + 2) * TX;
-  constexpr int FXN = TY * (TX + 1), FYN = (TY + 1) * TX;
-  constexpr int VSN = VR * VW;
+  constexpr int XRW = TX + 2, LRX = TY * XRW;      // i-recon columns i0-1 .. i0+TX
+  constexpr int LRY = (TY + 2) * TX;               // j-recon rows j0-1 .. j0+TY
+  constexpr int FXW = TX + 1, FXN = TY * FXW, FYN = (TY + 1) * TX;
   constexpr int AREA = VSN > FXN + FYN ? VSN : FXN + FYN;
+  constexpr int NXH = 2 * H * TY, NYH = 2 * H * TX, NHALO = NXH + NYH;
+  constexpr int NHL = (NHALO + NT - 1) / NT;       // halo loads per thread
+  constexpr int RT = 2 * NT + 2 * TY + 2 * TX;     // reconstruction tasks
+  constexpr int FT = 2 * NT + TY + TX;             // face tasks
   __shared__ double Vs[NC * AREA];
   __shared__ double LRx[NLR][NC * LRX];
   __shared__ double LRy[NLR][NC * LRY];
   __shared__ double red[NT / 64];
-  double* Fx = Vs;                 // [NC][FXN], valid between the face and update phases
+  double* Fx = Vs;                 // [NC][FXN] after phase B
   double* Fy = Vs + NC * FXN;      // [NC][FYN]
 
   const BlockGeom& g = a.g;
   const Params& P = a.P;
   const int t = threadIdx.x;
-  const int tx = t & 63, ty = t >> 6;
+  const int tx = t % TX, ty = t / TX;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = bid % a.tiles_x;
   const int tj = (bid / a.tiles_x) % a.tiles_y;
   const int tk = bid / (a.tiles_x * a.tiles_y);
   const int i0 = ti * TX, j0 = tj * TY;
   const int k0 = tk * a.chunk, k1 = min(k0 + a.chunk, g.nz);
-  const int c = i0 - 1 + tx, r = j0 + ty;
-  const int cl = clampi(c, -H, g.nx + H - 1), rl = clampi(r, -H, g.ny + H - 1);
-  const bool owner = tx >= 1 && tx <= TX && c < g.nx && r < g.ny;
+  const int c = i0 + tx, r = j0 + ty;
+  const bool owner = c < g.nx && r < g.ny;
+  const int ci_ = clampi(c, -H, g.nx + H - 1), rj_ = clampi(r, -H, g.ny + H - 1);
   const double dt = a.dt_dev ? *a.dt_dev : a.dt;
   const RKCoef kc = rk_coef(a.stage, dt);
 
-  double Vq[NW][NC];
-  auto loadV = [&](int i, int j, int k, double v[NC]) {
-    double raw[NC];
-    load_raw<RC, NC>(a.qin, g.cs,
-                     gidx(g, clampi(i, -H, g.nx + H - 1), clampi(j, -H, g.ny + H - 1),
-                          clampi(k, -H, g.nz + H - 1)), raw);
-    to_recon_vars<RC, NC>(raw, v, P.gamma);
+  auto gaddr = [&](int i, int j, int k) {
+    return gidx(g, clampi(i, -H, g.nx + H - 1), clampi(j, -H, g.ny + H - 1),
+                clampi(k, -H, g.nz + H - 1));
   };
+  // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
+  auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
+    if (e < NXH) {
+      const int row = e / (2 * H), s = e % (2 * H);
+      hi = s < H ? i0 - H + s : i0 + TX + (s - H);
+      hj = j0 + row;
+      slot = (row + H) * VW + (s < H ? s : TX + H + (s - H));
+    } else {
+      const int e2 = e - NXH;
+      const int col = e2 % TX, s = e2 / TX;
+      hi = i0 + col;
+      hj = s < H ? j0 - H + s : j0 + TY + (s - H);
+      slot = (s < H ? s : TY + H + (s - H)) * VW + col + H;
+    }
+  };
+
+  // ---- k-queue warm-up: vertices k0-1 and k0, face k0
+  double Vq[NW][NC];
 #pragma unroll
-  for (int w = 0; w < NW; ++w) loadV(cl, rl, k0 - 1 - QOFF + w, Vq[w]);
+  for (int w = 0; w < NW; ++w) {
+    double raw[NC];
+    load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 - 1 - QOFF + w), raw);
+    to_recon_vars<RC, NC>(raw, Vq[w], P.gamma);
+  }
   double Lc[NC], Fc[NC], Lt[NC], Rt[NC];
   RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
 #pragma unroll
   for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+  {
+    double raw[NC];
 #pragma unroll
-  for (int w = 0; w < NW - 1; ++w)
+    for (int w = 0; w < NW - 1; ++w)
 #pragma unroll
-    for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-  loadV(cl, rl, k0 + RV, Vq[NW - 1]);
+      for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+    load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 + RV), raw);
+    to_recon_vars<RC, NC>(raw, Vq[NW - 1], P.gamma);
+  }
   RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
   face_flux<FLUX, NC, 2>(Lc, Rt, P, Fc);
 #pragma unroll
   for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+
+  // ---- prefetch for plane k0
+  double oraw[NC];
+  load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 + 1 + RV), oraw);
+  double hraw[NHL][NC];
+  int hslot[NHL];
+#pragma unroll
+  for (int h = 0; h < NHL; ++h) {
+    const int e = t + h * NT;
+    hslot[h] = -1;
+    if (e < NHALO) {
+      int hi, hj;
+      halo_pos(e, hi, hj, hslot[h]);
+      load_raw<RC, NC>(a.qin, g.cs, gaddr(hi, hj, k0), hraw[h]);
+    }
+  }
 
   double nrm[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
   double smax = 0.0;
 
-  // extra plane loads: x far halo (2*RV columns x TY rows), y halo (2*H rows x TX columns)
-  constexpr int NXH = 2 * RV * TY, NYH = 2 * H * TX;
-
   for (int k = k0; k < k1; ++k) {
+    // ---- A: queue shift (centre k+1), plane k into LDS, next loads
 #pragma unroll
     for (int w = 0; w < NW - 1; ++w)
 #pragma unroll
       for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-    loadV(cl, rl, k + 1 + RV, Vq[NW - 1]);
-    // plane k into LDS
+    to_recon_vars<RC, NC>(oraw, Vq[NW - 1], P.gamma);
 #pragma unroll
-    for (int m = 0; m < NC; ++m) Vs[m * VSN + (ty + H) * VW + tx + RV] = Vq[QOFF - 1][m];
-    for (int e = t; e < NXH + NYH; e += NT) {
-      int hi, hj, slot;
-      if (e < NXH) {
-        constexpr int XW = 2 * RV > 0 ? 2 * RV : 1;
-        const int row = e / XW, s = e % XW;
-        hi = s < RV ? i0 - 1 - RV + s : i0 + TX + 1 + (s - RV);
-        hj = j0 + row;
-        slot = (row + H) * VW + (s < RV ? s : 64 + RV + (s - RV));
+    for (int m = 0; m < NC; ++m) Vs[m * VSN + (ty + H) * VW + tx + H] = Vq[QOFF - 1][m];
+#pragma unroll
+    for (int h = 0; h < NHL; ++h) {
+      if (hslot[h] >= 0) {
+        double v[NC];
+        to_recon_vars<RC, NC>(hraw[h], v, P.gamma);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vs[m * VSN + hslot[h]] = v[m];
+      }
+    }
+    if (k + 1 < k1) {
+      load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k + 2 + RV), oraw);
+#pragma unroll
+      for (int h = 0; h < NHL; ++h) {
+        const int e = t + h * NT;
+        if (e < NHALO) {
+          int hi, hj, sl;
+          halo_pos(e, hi, hj, sl);
+          load_raw<RC, NC>(a.qin, g.cs, gaddr(hi, hj, k + 1), hraw[h]);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- B: vertex reconstructions
+    for (int e = t; e < RT; e += NT) {
+      int src, stride, dst;
+      bool xdir;
+      if (e < NT) {                       // own vertex, i-direction
+        const int x = e % TX, y = e / TX;
+        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+      } else if (e < 2 * NT) {            // own vertex, j-direction
+        const int e1 = e - NT, x = e1 % TX, y = e1 / TX;
+        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
+      } else if (e < 2 * NT + 2 * TY) {   // i-direction at columns i0-1 and i0+TX
+        const int e2 = e - 2 * NT, y = e2 >> 1, side = e2 & 1;
+        const int x = side ? TX : -1;
+        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+      } else {                            // j-direction at rows j0-1 and j0+TY
+        const int e3 = e - 2 * NT - 2 * TY, x = e3 % TX, side = e3 / TX;
+        const int y = side ? TY : -1;
+        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
+      }
+      double W[2 * RV + 1][NC], L[NC], R[NC];
+#pragma unroll
+      for (int s = 0; s < 2 * RV + 1; ++s)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + src + (s - RV) * stride];
+      RC::template vertex<NC>(W, P, L, R);
+      if (xdir) {
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          LRx[0][m * LRX + dst] = L[m];
+          if constexpr (NLR == 2) LRx[NLR - 1][m * LRX + dst] = R[m];
+        }
       } else {
-        const int e2 = e - NXH;
-        const int col = e2 % TX, s = e2 / TX;
-        hi = i0 + col;
-        hj = s < H ? j0 - H + s : j0 + TY + (s - H);
-        slot = (s < H ? s : TY + H + (s - H)) * VW + col + 1 + RV;
-      }
-      double v[NC];
-      loadV(hi, hj, k, v);
 #pragma unroll
-      for (int m = 0; m < NC; ++m) Vs[m * VSN + slot] = v[m];
-    }
-    __syncthreads();
-    // reconstructions: x at the owned vertex (row ty); y at rows j0-1 .. j0+TY
-    // of columns i0 .. i0+TX-1
-    {
-      double W[2 * RV + 1][NC], L[NC], R[NC];
-#pragma unroll
-      for (int s = 0; s < 2 * RV + 1; ++s)
-#pragma unroll
-        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + (ty + H) * VW + tx + s];
-      RC::template vertex<NC>(W, P, L, R);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        LRx[0][m * LRX + ty * 64 + tx] = L[m];
-        if constexpr (NLR == 2) LRx[NLR - 1][m * LRX + ty * 64 + tx] = R[m];
-      }
-    }
-    for (int e = t; e < LRY; e += NT) {
-      const int y = e / TX, col = e % TX;
-      double W[2 * RV + 1][NC], L[NC], R[NC];
-#pragma unroll
-      for (int s = 0; s < 2 * RV + 1; ++s)
-#pragma unroll
-        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + (y - 1 + H - RV + s) * VW + col + 1 + RV];
-      RC::template vertex<NC>(W, P, L, R);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        LRy[0][m * LRY + e] = L[m];
-        if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + e] = R[m];
+        for (int m = 0; m < NC; ++m) {
+          LRy[0][m * LRY + dst] = L[m];
+          if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + dst] = R[m];
+        }
       }
     }
     __syncthreads();
-    // x-faces f = i0 + s, s = 0..TX: F(L(f-1), R(f))
-    for (int e = t; e < FXN; e += NT) {
-      const int row = e / (TX + 1), s = e % (TX + 1);
+
+    // ---- C: face fluxes (into the plane buffer, dead after B)
+    for (int e = t; e < FT; e += NT) {
       double L[NC], R[NC], F[NC];
+      if (e < NT || (e >= 2 * NT && e < 2 * NT + TY)) {   // i-face s of row y
+        int y, s;
+        if (e < NT) { s = e % TX; y = e / TX; } else { s = TX; y = e - 2 * NT; }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        L[m] = LRx[0][m * LRX + row * 64 + s];
-        R[m] = LRx[NLR - 1][m * LRX + row * 64 + s + 1];
+        for (int m = 0; m < NC; ++m) {
+          L[m] = LRx[0][m * LRX + y * XRW + s];
+          R[m] = LRx[NLR - 1][m * LRX + y * XRW + s + 1];
+        }
+        face_flux<FLUX, NC, 0>(L, R, P, F);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fx[m * FXN + y * FXW + s] = F[m];
+      } else {                                            // j-face s of column x
+        int x, s;
+        if (e < 2 * NT) { const int e1 = e - NT; x = e1 % TX; s = e1 / TX; }
+        else { x = e - 2 * NT - TY; s = TY; }
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          L[m] = LRy[0][m * LRY + s * TX + x];
+          R[m] = LRy[NLR - 1][m * LRY + (s + 1) * TX + x];
+        }
+        face_flux<FLUX, NC, 1>(L, R, P, F);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fy[m * FYN + s * TX + x] = F[m];
       }
-      face_flux<FLUX, NC, 0>(L, R, P, F);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Fx[m * FXN + e] = F[m];
     }
-    // y-faces fj = j0 + y, y = 0..TY: F(L(row y-1), R(row y)) = recon rows (y, y+1)
-    for (int e = t; e < FYN; e += NT) {
-      const int y = e / TX, col = e % TX;
-      double L[NC], R[NC], F[NC];
-#pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        L[m] = LRy[0][m * LRY + y * TX + col];
-        R[m] = LRy[NLR - 1][m * LRY + (y + 1) * TX + col];
-      }
-      face_flux<FLUX, NC, 1>(L, R, P, F);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Fy[m * FYN + e] = F[m];
-    }
-    // z-face k + 1 of the owned column
+    // k-face k+1 of the owned column
     double Fn[NC];
     RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
     face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
     __syncthreads();
-    if (owner && k < g.nz) {
-      const int64_t ci = gidx(g, c, r, k);
-      const int col = tx - 1;
+
+    // ---- D: update
+    if (owner) {
+      const int64_t cidx = gidx(g, c, r, k);
       double out[NC];
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        double rr = 0.0 - (Fx[m * FXN + ty * (TX + 1) + col + 1] - Fx[m * FXN + ty * (TX + 1) + col]) / g.dx;
-        rr = rr - (Fy[m * FYN + (ty + 1) * TX + col] - Fy[m * FYN + ty * TX + col]) / g.dy;
-        rr = rr - (Fn[m] - Fc[m]) / g.dz;
-        const double q0 = a.q0[ci + m * g.cs];
-        double* accp = a.acc ? a.acc + ci + m * g.cs : nullptr;
-        out[m] = rk_apply(a.stage, kc, q0, a.qin[ci + m * g.cs], accp, rr);
+        const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
+        const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
+        double rr;
+        if constexpr (kFast) {
+          rr = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
+        } else {
+          rr = 0.0 - fxd / g.dx;
+          rr = rr - fyd / g.dy;
+          rr = rr - (Fn[m] - Fc[m]) / g.dz;
+        }
+        const double q0 = a.q0[cidx + m * g.cs];
+        double* accp = a.acc ? a.acc + cidx + m * g.cs : nullptr;
+        out[m] = rk_apply(a.stage, kc, q0, a.qin[cidx + m * g.cs], accp, rr);
         nrm[m] += rr * rr;
       }
       store_with_ghosts<NC, 3>(g, a.qout, c, r, k, out);
@@ -607,4 +649,5 @@ __global__ __launch_bounds__(64 * TY) void stage3d_kernel(StageArgs a) {
   }
 }
 
+}  // namespace exact / fast
 }  // namespace fvm

@@ -1,0 +1,234 @@
+!=============================================================================80
+!
+!> Fortran driver of the MI355X residual + RK path.
+!
+!  Reads an FVM-3D-F90 case file (namelist groups General, Geometry,
+!  Inviscid_Flux, Time_Stepping, ... as in case/Riemann-2D.nml), builds the
+!  uniform nodal grid (src/base/method_gridgen.f90:84-136) and the four-
+!  quadrant initial state (src/solve/class_flow_field.f90:195-256), then runs
+!  the time loop of solve_process (src/process/solver_sub.f90:177-217) with
+!  the RK stages on the GPU through the fvm_gpu module.  Residuals are
+!  printed as the reference's terminal table (src/solve/class_output.f90:
+!  414-463); NaN or |R| > 1e10 stops the run (class_iteration.f90:183-191).
+!
+!  Usage:  fvm_driver --input=<case.nml>
+!  Extra group (optional, not in the reference):
+!    &GPU  device = 0, numerics = "exact" | "fast", cfl = 0.5,
+!          max_step = 0 (0: no limit), print_every = 1,
+!          output_file = "" (raw dump of the final q with 3 ghost layers) /
+!
+!=============================================================================80
+program fvm_driver
+
+  use iso_c_binding
+  use fvm_gpu
+  implicit none
+
+  integer, parameter :: dp = c_double, NG = 3
+
+!...Case file (namelist names as the reference spells them)
+  character(len=64) :: name = "", dimension = "", equation = "", gasModel = ""
+  logical :: dimensionless = .true.
+  character(len=64) :: type = "", geoType = "", gridType = ""
+  real(dp) :: geoSize = 1.0_dp
+  integer :: gridSize(3) = [0, 0, 0]
+  logical :: writeGrid = .false.
+  character(len=64) :: reconstruct = "", limiter = "", flux_split = ""
+  real(dp) :: kappa = 0.0_dp, entropy_fix = 0.0_dp
+  character(len=64) :: time_scheme = "", timestep_set = ""
+  real(dp) :: fix_timestep = 0.0_dp, total_time = 0.0_dp, output_time = 0.0_dp
+  real(dp) :: absolute_residual = 0.0_dp, relative_residual = 0.0_dp
+  character(len=64) :: format = ""
+  logical :: Density = .false., VelocityX = .false., VelocityY = .false., Pressure = .false.
+  integer :: device = 0, max_step = 0, print_every = 1
+  character(len=16) :: numerics = "exact"
+  real(dp) :: cfl = 0.5_dp
+  character(len=256) :: output_file = ""
+
+  namelist /General/ name, dimension, dimensionless, equation, gasModel
+  namelist /Geometry/ type, geoType, geoSize, gridType, gridSize, writeGrid
+  namelist /Inviscid_Flux/ reconstruct, kappa, limiter, flux_split, entropy_fix
+  ! the empty groups of the case file (Viscous_Flux, Boundary_Condition,
+  ! Initial_Condition) carry nothing and are skipped by the namelist reads
+  namelist /Time_Stepping/ time_scheme, timestep_set, fix_timestep, total_time, &
+                           output_time, absolute_residual, relative_residual
+  namelist /Output/ format, Density, VelocityX, VelocityY, Pressure
+  namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file
+
+  character(len=512) :: arg, casefile
+  integer :: unit, ios, i, j, k, ni, nj, nk, step, nan_flag, eq
+  real(dp) :: dx, dy, dz, gamma, time, dt, dt_used, res(6), res0(6)
+  real(dp), allocatable, target :: q(:,:,:,:)
+  real(dp), allocatable :: x(:), y(:), p(:,:,:,:)
+  type(c_ptr) :: ctx, blk
+  integer(c_int) :: rc, recon_c, lim_c, flux_c, rk_c, mode
+  logical :: converged
+
+!...--input=<file> (src/base/method_argparse.f90:175-197)
+  casefile = ""
+  do i = 1, command_argument_count()
+    call get_command_argument(i, arg)
+    if (arg(1:8) == "--input=") casefile = arg(9:)
+  end do
+  if (len_trim(casefile) == 0) then
+    write(*, '(a)') "usage: fvm_driver --input=<case.nml>"
+    error stop 2
+  end if
+
+  open(newunit=unit, file=trim(casefile), status="old", action="read", iostat=ios)
+  if (ios /= 0) then
+    write(*, '(a, a)') "cannot open case file ", trim(casefile)
+    error stop 2
+  end if
+  read(unit, nml=General, iostat=ios);          call need(ios, "General")
+  rewind(unit); read(unit, nml=Geometry, iostat=ios);      call need(ios, "Geometry")
+  rewind(unit); read(unit, nml=Inviscid_Flux, iostat=ios); call need(ios, "Inviscid_Flux")
+  rewind(unit); read(unit, nml=Time_Stepping, iostat=ios); call need(ios, "Time_Stepping")
+  rewind(unit); read(unit, nml=GPU, iostat=ios)            ! optional group
+  close(unit)
+
+!...Grid: N vertices per direction on [0, geoSize], dx = L / (N - 1)
+  gamma = 1.4_dp
+  ni = gridSize(1); nj = gridSize(2)
+  if (trim(dimension) == "2D") then
+    nk = 1
+  else
+    nk = gridSize(3)
+  end if
+  if (ni < 2 .or. nj < 2 .or. nk < 1) then
+    write(*, '(a)') "invalid gridSize"
+    error stop 2
+  end if
+  dx = geoSize / real(ni - 1, dp)
+  dy = geoSize / real(nj - 1, dp)
+  dz = 1.0_dp
+  if (nk > 1) dz = geoSize / real(nk - 1, dp)
+  allocate(x(ni), y(nj))
+  do i = 1, ni
+    x(i) = dx * real(i - 1, dp)
+  end do
+  do j = 1, nj
+    y(j) = dy * real(j - 1, dp)
+  end do
+
+!...Initial state: four quadrants, regions applied in order with inclusive
+!   bounds so later regions win on the x = 0.8 / y = 0.8 lines
+  allocate(p(5, ni, nj, nk))
+  p = 0.0_dp
+  call set_region(0.8_dp, 1.0_dp, 0.8_dp, 1.0_dp, [1.5_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.5_dp])
+  call set_region(0.0_dp, 0.8_dp, 0.8_dp, 1.0_dp, [0.5323_dp, 1.206_dp, 0.0_dp, 0.0_dp, 0.3_dp])
+  call set_region(0.0_dp, 0.8_dp, 0.0_dp, 0.8_dp, [0.138_dp, 1.206_dp, 1.206_dp, 0.0_dp, 0.029_dp])
+  call set_region(0.8_dp, 1.0_dp, 0.0_dp, 0.8_dp, [0.5323_dp, 0.0_dp, 1.206_dp, 0.0_dp, 0.3_dp])
+  if (nk == 1) then
+    allocate(q(ni + 2*NG, nj + 2*NG, 1, 5))
+  else
+    allocate(q(ni + 2*NG, nj + 2*NG, nk + 2*NG, 5))
+  end if
+  q = 0.0_dp
+  do k = 1, nk
+    do j = 1, nj
+      do i = 1, ni
+        call p_to_q(p(:, i, j, k), q(i + NG, j + NG, k + merge(0, NG, nk == 1), :))
+      end do
+    end do
+  end do
+
+!...Scheme selection (codes of src/defs/type_defs_scheme.f90)
+  recon_c = scheme_index(reconstruct)
+  lim_c = scheme_index(limiter)
+  flux_c = scheme_index(flux_split)
+  rk_c = scheme_index(time_scheme)
+  if (recon_c < 0 .or. flux_c < 0 .or. rk_c < 0) then
+    write(*, '(a)') "invalid scheme in case file"
+    error stop 2
+  end if
+  mode = FVM_NUMERICS_EXACT
+  if (trim(numerics) == "fast") mode = FVM_NUMERICS_FAST
+
+  call fvm_check(fvm_ctx_create(int(device, c_int), ctx), "ctx_create")
+  call fvm_check(fvm_scheme_set(ctx, recon_c, lim_c, flux_c, rk_c, kappa, entropy_fix, &
+                                gamma, 1_c_int), "scheme_set")
+  call fvm_check(fvm_set_numerics(ctx, mode), "set_numerics")
+  call fvm_check(fvm_block_create(ctx, int(ni, c_int64_t), int(nj, c_int64_t), &
+                                  int(nk, c_int64_t), dx, dy, dz, blk), "block_create")
+  call fvm_check(fvm_upload(blk, c_loc(q), int(NG, c_int)), "upload")
+
+!...Time loop (src/process/solver_sub.f90:196-206)
+  write(*, '(a, a, a, i0, a, i0, a, i0)') "case ", trim(name), ": ", ni, " x ", nj, " x ", nk
+  write(*, '(a8, a14, 5a14)') "Step", "Time", "R_rho", "R_momx", "R_momy", "R_energy", "R_total"
+  time = 0.0_dp
+  step = 0
+  converged = .false.
+  do while (.not. converged)
+    step = step + 1
+    if (trim(timestep_set) == "CFL") then
+      dt = -1.0_dp
+    else
+      dt = fix_timestep
+    end if
+    rc = fvm_step(ctx, 1_c_int, dt, cfl, dt_used)
+    call fvm_check(rc, "step")
+    time = time + dt_used
+    call fvm_check(fvm_norms(ctx, res, nan_flag), "norms")
+    if (step == 1) res0 = res
+    if (nan_flag /= 0) then
+      write(*, '(a, es12.4)') "Error: ABSRES ", res(6)
+      error stop 3
+    end if
+    if (mod(step, max(print_every, 1)) == 0) &
+      write(*, '(i8, es14.6, 5es14.6)') step, time, (res(eq) / res0(eq), eq = 1, 3), &
+        res(5) / res0(5), res(6) / res0(6)
+    if (max_step > 0 .and. step >= max_step) converged = .true.
+    if (time > total_time - 1.0e-10_dp) converged = .true.
+    if (res(6) < absolute_residual) converged = .true.
+  end do
+  write(*, '(a, i0, a, es14.6)') "finished after ", step, " steps at t = ", time
+
+  call fvm_check(fvm_download(blk, c_loc(q), int(NG, c_int)), "download")
+  if (len_trim(output_file) > 0) then
+    open(newunit=unit, file=trim(output_file), access="stream", form="unformatted", &
+         status="replace", action="write")
+    write(unit) int(ni, c_int32_t), int(nj, c_int32_t), int(nk, c_int32_t), &
+                int(NG, c_int32_t), step, time
+    write(unit) q
+    close(unit)
+  end if
+  call fvm_check(fvm_ctx_destroy(ctx), "ctx_destroy")
+
+contains
+
+  subroutine need(ios, group)
+    integer, intent(in) :: ios
+    character(len=*), intent(in) :: group
+    if (ios /= 0) then
+      write(*, '(a, a, a, i0)') "case file: cannot read group &", group, ", iostat ", ios
+      error stop 2
+    end if
+  end subroutine need
+
+  !> init_value_sub (src/base/defs_block_data.f90:166-201) for one region
+  subroutine set_region(x0, x1, y0, y1, state)
+    real(dp), intent(in) :: x0, x1, y0, y1, state(5)
+    integer :: ii, jj, kk
+    do kk = 1, nk
+      do jj = 1, nj
+        do ii = 1, ni
+          if (x(ii) >= x0 .and. x(ii) <= x1 .and. y(jj) >= y0 .and. y(jj) <= y1) &
+            p(:, ii, jj, kk) = state
+        end do
+      end do
+    end do
+  end subroutine set_region
+
+  !> convert_p_to_q_blk_sub (src/solve/class_flow_field.f90:353-377)
+  subroutine p_to_q(pv, qv)
+    real(dp), intent(in) :: pv(5)
+    real(dp), intent(out) :: qv(5)
+    qv(1) = pv(1)
+    qv(2) = pv(1) * pv(2)
+    qv(3) = pv(1) * pv(3)
+    qv(4) = pv(1) * pv(4)
+    qv(5) = pv(5) / (gamma - 1.0_dp) + 0.5_dp * pv(1) * (pv(2)**2 + pv(3)**2 + pv(4)**2)
+  end subroutine p_to_q
+
+end program fvm_driver

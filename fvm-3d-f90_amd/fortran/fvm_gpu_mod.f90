@@ -1,0 +1,189 @@
+!=============================================================================80
+!
+!> ISO_C_BINDING interface to the MI355X residual + RK library
+!> (include/fvm_gpu.h).  A Fortran solver calls fvm_step where FVM-3D-F90
+!> calls ClassTimeDiscret%Runge_Kutta_sub (src/solve/class_time_discret.f90:
+!> 159-196); the conserved state stays on the device between calls.
+!
+!  Host arrays keep the reference's layout: q(i, j, k, eq), column-major,
+!  eq = rho, rho*u, rho*v, rho*w, rho*E, with ng ghost layers per swept side
+!  (src/base/defs_block_data.f90:111-163).
+!
+!=============================================================================80
+module fvm_gpu
+
+  use iso_c_binding
+  implicit none
+  private
+
+!...Scheme codes, src/defs/type_defs_scheme.f90:11-36 (>= 1000: additions)
+  integer(c_int), parameter, public :: FVM_RK_SSP3 = 101, FVM_RK4 = 1001
+  integer(c_int), parameter, public :: FVM_SIMPLE = 200, FVM_MUSCL = 201, FVM_WENO = 202
+  integer(c_int), parameter, public :: FVM_LIM_NONE = 250, FVM_LIM_MINMOD = 251
+  integer(c_int), parameter, public :: FVM_LIM_VANLEER = 252, FVM_LIM_VANALBADA = 253
+  integer(c_int), parameter, public :: FVM_ROE = 300, FVM_VANLEER = 301, FVM_AUSM = 303
+  integer(c_int), parameter, public :: FVM_AUSMPLUS = 1303
+  integer(c_int), parameter, public :: FVM_BC_EXTRAP = 0, FVM_BC_PERIODIC = 1
+  integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
+
+  public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
+  public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc
+  public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
+  public :: fvm_check, fvm_error_text, scheme_index
+
+  interface
+    function fvm_last_error() bind(C, name="fvm_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr) :: msg
+    end function fvm_last_error
+
+    function fvm_ctx_create(device, ctx) bind(C, name="fvm_ctx_create") result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function fvm_ctx_create
+
+    function fvm_ctx_destroy(ctx) bind(C, name="fvm_ctx_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function fvm_ctx_destroy
+
+    function fvm_scheme_set(ctx, recon, limiter, flux, rk, kappa, entropy_fix, gamma, &
+                            ref_compat) bind(C, name="fvm_scheme_set") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: recon, limiter, flux, rk, ref_compat
+      real(c_double), value :: kappa, entropy_fix, gamma
+      integer(c_int) :: rc
+    end function fvm_scheme_set
+
+    function fvm_set_numerics(ctx, mode) bind(C, name="fvm_set_numerics") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      integer(c_int) :: rc
+    end function fvm_set_numerics
+
+    function fvm_block_create(ctx, ni, nj, nk, dx, dy, dz, blk) &
+        bind(C, name="fvm_block_create") result(rc)
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ni, nj, nk
+      real(c_double), value :: dx, dy, dz
+      type(c_ptr), intent(out) :: blk
+      integer(c_int) :: rc
+    end function fvm_block_create
+
+    function fvm_block_destroy(blk) bind(C, name="fvm_block_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: blk
+      integer(c_int) :: rc
+    end function fvm_block_destroy
+
+    function fvm_block_set_bc(blk, face, kind) bind(C, name="fvm_block_set_bc") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: blk
+      integer(c_int), value :: face, kind
+      integer(c_int) :: rc
+    end function fvm_block_set_bc
+
+    function fvm_upload(blk, q, host_ng) bind(C, name="fvm_upload") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: blk
+      type(c_ptr), value :: q
+      integer(c_int), value :: host_ng
+      integer(c_int) :: rc
+    end function fvm_upload
+
+    function fvm_download(blk, q, host_ng) bind(C, name="fvm_download") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: blk
+      type(c_ptr), value :: q
+      integer(c_int), value :: host_ng
+      integer(c_int) :: rc
+    end function fvm_download
+
+    function fvm_step(ctx, nsteps, dt, cfl, dt_last) bind(C, name="fvm_step") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      real(c_double), value :: dt, cfl
+      real(c_double), intent(out) :: dt_last
+      integer(c_int) :: rc
+    end function fvm_step
+
+    function fvm_norms(ctx, res, nan_flag) bind(C, name="fvm_norms") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: res(6)
+      integer(c_int), intent(out) :: nan_flag
+      integer(c_int) :: rc
+    end function fvm_norms
+
+    function fvm_sync(ctx) bind(C, name="fvm_sync") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function fvm_sync
+
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function c_strlen
+  end interface
+
+contains
+
+  !> Text of the library's last error on this thread.
+  function fvm_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: i, n
+    p = fvm_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function fvm_error_text
+
+  !> Stop with the library's message on a nonzero return, as the reference
+  !> stops on ClassError flags (src/core/parallel_mpi.f90:111-133).
+  subroutine fvm_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= 0) then
+      write(*, '(a, a, a, i0, a, a)') "fvm_gpu error in ", where, " (", rc, "): ", &
+        fvm_error_text()
+      error stop 1
+    end if
+  end subroutine fvm_check
+
+  !> Case-file spelling -> scheme code (src/defs/type_defs_scheme.f90:41-84);
+  !> -1 (the reference's NULL) for an unknown name.
+  integer(c_int) function scheme_index(name) result(code)
+    character(len=*), intent(in) :: name
+    select case (trim(name))
+      case ("Runge-Kutta");   code = FVM_RK_SSP3
+      case ("RK4");           code = FVM_RK4
+      case ("simple");        code = FVM_SIMPLE
+      case ("MUSCL");         code = FVM_MUSCL
+      case ("WENO");          code = FVM_WENO
+      case ("none_");         code = FVM_LIM_NONE
+      case ("minmod_");       code = FVM_LIM_MINMOD
+      case ("VanLeer_");      code = FVM_LIM_VANLEER
+      case ("vanAlbada_");    code = FVM_LIM_VANALBADA
+      case ("Roe");           code = FVM_ROE
+      case ("VanLeer");       code = FVM_VANLEER
+      case ("AUSM");          code = FVM_AUSM
+      case ("AUSM+");         code = FVM_AUSMPLUS
+      case default;           code = -1
+    end select
+  end function scheme_index
+
+end module fvm_gpu

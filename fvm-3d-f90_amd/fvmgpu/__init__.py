@@ -22,6 +22,7 @@ SIMPLE, MUSCL, WENO = 200, 201, 202
 LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
+NUMERICS_EXACT, NUMERICS_FAST = 0, 1
 
 # scheme names as the case file spells them (type_defs_scheme.f90:41-84)
 SCHEME_INDEX = {
@@ -35,7 +36,8 @@ SCHEME_INDEX = {
 EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_scheme_set", "fvm_block_create", "fvm_block_destroy", "fvm_block_set_bc",
            "fvm_upload", "fvm_download", "fvm_step", "fvm_norms", "fvm_sync",
-           "fvm_timing_enable", "fvm_timing_read")
+           "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
+           "fvm_block_launch_shape")
 
 
 class FvmError(RuntimeError):
@@ -71,6 +73,10 @@ def load_library(path=LIB_PATH):
         "fvm_norms": ([vp, ctypes.POINTER(cd), ctypes.POINTER(ci)], ci),
         "fvm_sync": ([vp], ci),
         "fvm_timing_enable": ([vp, ci], ci),
+        "fvm_set_numerics": ([vp, ci], ci),
+        "fvm_autotune": ([vp, ci], ci),
+        "fvm_block_launch_shape": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                    ctypes.POINTER(ci)], ci),
         "fvm_timing_read": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
     }
     for name, (args, res) in sig.items():
@@ -118,6 +124,12 @@ class Block:
             raise FvmError(f"upload shape {q.shape} != {self.host_shape(ng)}")
         _check(_lib.fvm_upload(self._h, q.ctypes.data, ng))
 
+    def launch_shape(self):
+        tx, ty, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.fvm_block_launch_shape(self._h, ctypes.byref(tx), ctypes.byref(ty),
+                                           ctypes.byref(ch)))
+        return tx.value, ty.value, ch.value
+
     def download(self, ng=0):
         q = np.zeros(self.host_shape(ng))
         _check(_lib.fvm_download(self._h, q.ctypes.data, ng))
@@ -156,10 +168,19 @@ class Context:
         _check(_lib.fvm_scheme_set(self._h, recon, limiter, flux, rk, kappa, entropy_fix, gamma,
                                    ref_compat))
 
+    def numerics(self, mode):
+        """NUMERICS_EXACT (bit-identical to the reference arithmetic) or
+        NUMERICS_FAST (FMA contraction + reciprocals, tolerance-checked)."""
+        _check(_lib.fvm_set_numerics(self._h, mode))
+
     def block(self, shape, spacing):
         b = Block(self, shape, spacing)
         self.blocks.append(b)
         return b
+
+    def autotune(self, reps=3):
+        """Pick each block's tile shape and k-chunk by timing stage launches."""
+        _check(_lib.fvm_autotune(self._h, int(reps)))
 
     def step(self, nsteps=1, dt=0.0, cfl=0.0):
         """Advance every block; dt > 0 fixed, else CFL.  Returns the last dt."""

@@ -88,6 +88,16 @@ int fvm_ctx_destroy(fvm_ctx *ctx);
 int fvm_scheme_set(fvm_ctx *ctx, int recon, int limiter, int flux, int rk,
                    double kappa, double entropy_fix, double gamma, int ref_compat);
 
+/* Numerics of the stage kernels (no reference counterpart):
+ *   FVM_NUMERICS_EXACT (default): the reference's IEEE operation order, no
+ *     FMA contraction; results are bit-identical to the CPU path.
+ *   FVM_NUMERICS_FAST: the same formulas with FMA contraction and
+ *     reciprocal multiplies; relative L-inf difference from the exact path
+ *     <= 1e-12 after one step and <= 1e-9 after 1000 steps (tested). */
+#define FVM_NUMERICS_EXACT 0
+#define FVM_NUMERICS_FAST  1
+int fvm_set_numerics(fvm_ctx *ctx, int mode);
+
 /* One structured block of ni x nj x nk real vertices (nk == 1: 2-D, swept
  * along i and j only), vertex spacing dx, dy, dz (the reference's uniform
  * cell_size, src/base/method_gridgen.f90:120-131).  Replaces
@@ -122,6 +132,13 @@ int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
  * out[5] total; nan_flag = 1 if any is NaN or above 1e10
  * (converged_fun's abort tests, :183-191). */
 int fvm_norms(fvm_ctx *ctx, double out[6], int *nan_flag);
+
+/* Autotuner (no reference counterpart): times `reps` launches of the first
+ * RK stage for every compiled tile shape and k-chunk of each block and keeps
+ * the fastest.  The probe writes only stage buffers; the state is unchanged.
+ * Call after fvm_upload.  fvm_block_launch_shape reports the choice. */
+int fvm_autotune(fvm_ctx *ctx, int reps);
+int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk);
 
 /* Stage-kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only wall-clocks the whole solve,

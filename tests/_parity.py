@@ -7,9 +7,10 @@ import fvmgpu
 from fvmgpu import cases
 
 
-def run_gpu(ctx, p, spacing, nsteps, dt=0.0, cfl=0.0, bc=None, **scheme):
+def run_gpu(ctx, p, spacing, nsteps, dt=0.0, cfl=0.0, bc=None, numerics=0, **scheme):
     nk, nj, ni, _ = p.shape
     ctx.scheme(**scheme)
+    ctx.numerics(numerics)
     blk = ctx.block((ni, nj, nk), spacing)
     if bc is not None:
         for f, kind in enumerate(bc):

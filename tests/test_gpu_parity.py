@@ -46,3 +46,37 @@ def test_smooth3d_bitwise(recon, lim, flux):
     print("rel linf", rel_linf(qg, qo))
     assert np.array_equal(qg, qo)
     np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("tile", ["0", "1", "2"])
+@pytest.mark.parametrize("chunk", ["3", "8", "32"])
+def test_tile_shapes_bitwise(tile, chunk, monkeypatch):
+    """Every compiled tile shape and k-chunk gives the same bits (partial
+    tiles in i, j and k included)."""
+    monkeypatch.setenv("FVM_TILE", tile)
+    monkeypatch.setenv("FVM_CHUNK", chunk)
+    p = cases.smooth_3d((19, 11, 67), seed=9)
+    sp = (1 / 66, 1 / 10, 1 / 18)
+    print(f"case tiles tile={tile} chunk={chunk} 67x11x19", flush=True)
+    with fvmgpu.Context() as ctx:
+        qg, _, _, _ = run_gpu(ctx, p, sp, 2, dt=1e-3, bc=(1, 1, 0, 0, 1, 1), recon=O.MUSCL,
+                              limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+    qo, _ = run_oracle(p, sp, 2, dt=1e-3, bc=(1, 1, 0, 0, 1, 1), recon=O.MUSCL,
+                       limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+    assert np.array_equal(qg, qo)
+
+
+@pytest.mark.parametrize("recon,lim,flux", [(O.MUSCL, O.LIM_VANALBADA, O.ROE),
+                                            (O.WENO, O.LIM_NONE, O.AUSM)])
+def test_rk4_periodic_cfl_bitwise(recon, lim, flux):
+    """RK4 + periodic ghosts + the device CFL step, against the oracle."""
+    p, h, _ = cases.isentropic_vortex(24, seed=1)
+    print(f"case vortex24 rk4 cfl recon={recon} flux={flux}", flush=True)
+    with fvmgpu.Context() as ctx:
+        qg, ng, _, dt_last = run_gpu(ctx, p, (h, h, h), 5, cfl=0.8, bc=(1,) * 6, recon=recon,
+                                     limiter=lim, flux=flux, rk=O.RK4)
+    qo, r = run_oracle(p, (h, h, h), 5, cfl=0.8, bc=(1,) * 6, recon=recon, limiter=lim,
+                       flux=flux, rk=O.RK4)
+    assert dt_last == r["dt"][-1]
+    assert np.array_equal(qg, qo)
+    np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12)
