@@ -394,8 +394,12 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 //      first, then the far edge), and the owner's k-face k+1 from its queue
 //   D  owner update: residual, RK combine, store with ghost images
 // ============================================================================
+#ifndef FVM_WPE
+#define FVM_WPE 2
+#endif
 template <class RC, int FLUX, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void stage3d_kernel(StageArgs a) {
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FVM_WPE)))
+void stage3d_kernel(StageArgs a) {
   constexpr int NC = 5, NT = TX * TY;
   constexpr int RV = RC::RV, H = RC::H;
   constexpr int NW = RV + (RV > 1 ? RV : 1) + 1;   // k-queue V(c - max(RV,1) .. c + RV)

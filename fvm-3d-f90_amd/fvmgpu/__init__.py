@@ -15,7 +15,8 @@ import numpy as np
 from . import cases  # noqa: F401  (re-export)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfvmgpu.so")
+LIB_PATH = os.environ.get("FVM_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                     "libfvmgpu.so")
 
 RK_SSP3, RK4 = 101, 1001
 SIMPLE, MUSCL, WENO = 200, 201, 202
