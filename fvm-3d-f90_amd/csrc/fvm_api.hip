@@ -1,6 +1,7 @@
 // fvm_api.hip -- C ABI of the MI355X residual + RK path (include/fvm_gpu.h):
-// contexts, blocks, host transfers, step orchestration and the small
-// auxiliary kernels (ghost fill, CFL reduction, dt update).
+// contexts, blocks and their connections, host transfers, step orchestration,
+// the autotuner and the small auxiliary kernels (ghost fill, CFL reduction,
+// dt update).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,28 +50,34 @@ struct fvm_block {
   fvm_ctx* ctx = nullptr;
   int dim = 3, nc = 5;
   BlockGeom g{};
-  double* qn = nullptr;   // u^n, then u^{n+1}
-  double* qa = nullptr;   // stage buffers
-  double* qb = nullptr;
-  double* acc = nullptr;  // RK4 accumulator
-  size_t state_elems = 0; // nc * cs
+  double* q[4] = {nullptr, nullptr, nullptr, nullptr};   // u^n, stage a, stage b, RK4 acc
+  size_t state_elems = 0;                                // nc * cs
   bool uploaded = false;
+  int nbr[6] = {-1, -1, -1, -1, -1, -1};                // neighbour block index (context order)
+  int nbr_shared[6] = {0, 0, 0, 0, 0, 0};
   // launch shape
   int tile = 0, chunk = 16;
-  int tiles_x = 0, tiles_y = 0, tiles_z = 0, nblocks = 0;
+  int tiles_x = 0, tiles_y = 0, tiles_z = 0, ntiles = 0, tile_base = 0;
 };
 
 struct fvm_ctx {
   int device = 0;
+  int num_cu = 256;
   hipStream_t stream = nullptr;
   bool scheme_set = false;
   bool fast = false;       // FVM_NUMERICS_FAST: FMA/reciprocal kernel family
+  int launch_mode = FVM_LAUNCH_BATCHED;
   int recon = 0, flux = 0, rk = 0;
   Params P{};
   std::vector<fvm_block*> blocks;
+  BlockDesc* d_desc = nullptr;           // descriptor table on the device
+  size_t desc_cap = 0;
+  bool desc_dirty = true;                // table must be rebuilt before a launch
+  bool ghosts_dirty = true;              // u^n ghost layers must be refilled
+  int total_tiles = 0;
   double* d_dt = nullptr;                 // CFL dt of the current step
   unsigned long long* d_smax = nullptr;   // wave-speed max of the current state
-  double* d_norm = nullptr;               // per-workgroup partial sums of rhs^2
+  double* d_norm = nullptr;               // per-tile partial sums of rhs^2
   size_t norm_cap = 0;                    // doubles
   std::vector<double> h_norm;
   double norms[6] = {0, 0, 0, 0, 0, 0};
@@ -89,13 +96,20 @@ struct fvm_ctx {
 // ============================================================================
 namespace {
 
-// Ghost layers of one face region: extrapolation copies the boundary vertex
-// (src/base/defs_block_data.f90:226-256), periodic the vertex n periods away.
-__global__ void ghost_fill_kernel(double* __restrict__ q, BlockGeom g, int nc, int d, int side) {
+// Ghost layers of one face region of block `bi`, state array `buf`:
+// extrapolation copies the boundary vertex (src/base/defs_block_data.f90:
+// 226-256), periodic the vertex n periods away, a connected face the donor
+// vertex of the neighbour block.
+__global__ void ghost_fill_kernel(const BlockDesc* __restrict__ tab, int bi, int buf, int nc, int d,
+                                  int side) {
+  const BlockDesc& B = tab[bi];
+  const BlockGeom& g = B.g;
+  double* q = B.q[buf];
   const int n[3] = {g.nx, g.ny, g.nz};
   const int a_dim = d == 0 ? 1 : 0, b_dim = d == 2 ? 1 : 2;
   const int na = n[a_dim], nb = n[b_dim];
   const int64_t total = (int64_t)na * nb * g.ng;
+  const int kind = g.bc[2 * d + side];
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int aa = (int)(e % na);
@@ -104,16 +118,28 @@ __global__ void ghost_fill_kernel(double* __restrict__ q, BlockGeom g, int nc, i
     int x[3], s[3];
     x[a_dim] = aa; x[b_dim] = bb;
     s[a_dim] = aa; s[b_dim] = bb;
-    const bool periodic = g.bc[2 * d + side] == BC_PERIODIC;
-    if (side == 0) {
-      x[d] = -l;
-      s[d] = periodic ? n[d] - l : 0;
+    x[d] = side == 0 ? -l : n[d] - 1 + l;
+    const double* src = q;
+    int64_t cs = g.cs;
+    int64_t si;
+    if (kind == BC_CONN) {
+      const int nbk = B.nbr[2 * d + side];
+      const BlockGeom& h = tab[nbk].g;
+      const int hn[3] = {h.nx, h.ny, h.nz};
+      const int sh = B.nbr_shared[2 * d + side];
+      // low face: ghost -l is the neighbour's vertex n-1-l (shared) or n-l;
+      // high face: ghost n-1+l is the neighbour's vertex l (shared) or l-1
+      s[d] = side == 0 ? (sh ? hn[d] - 1 - l : hn[d] - l) : (sh ? l : l - 1);
+      src = tab[nbk].q[buf];
+      cs = h.cs;
+      si = (int64_t)(s[2] + h.oz) * h.pxy + (int64_t)(s[1] + h.oy) * h.px + (s[0] + h.ox);
     } else {
-      x[d] = n[d] - 1 + l;
-      s[d] = periodic ? l - 1 : n[d] - 1;
+      if (kind == BC_PERIODIC) s[d] = side == 0 ? n[d] - l : l - 1;
+      else s[d] = side == 0 ? 0 : n[d] - 1;
+      si = gidx(g, s[0], s[1], s[2]);
     }
-    const int64_t di = gidx(g, x[0], x[1], x[2]), si = gidx(g, s[0], s[1], s[2]);
-    for (int c = 0; c < nc; ++c) q[di + c * g.cs] = q[si + c * g.cs];
+    const int64_t di = gidx(g, x[0], x[1], x[2]);
+    for (int c = 0; c < nc; ++c) q[di + c * g.cs] = src[si + c * cs];
   }
 }
 
@@ -165,29 +191,36 @@ int set_device(fvm_ctx* ctx) {
   return FVM_OK;
 }
 
+int block_index(fvm_ctx* ctx, fvm_block* b) {
+  for (size_t i = 0; i < ctx->blocks.size(); ++i)
+    if (ctx->blocks[i] == b) return (int)i;
+  return -1;
+}
+
 // Launch shape of a block.  3-D: tile (TX x TY columns, kTile3d) marching
-// over `chunk` planes -- TX is 64 unless 32 wastes fewer columns, the chunk
-// is the longest (<= 32 planes) that still gives >= 512 workgroups.  2-D: rows
-// of 62 columns marching over `chunk` rows.  FVM_TILE / FVM_CHUNK override.
+// over `chunk` planes -- 32 x 8 unless 64 wastes fewer columns, the chunk is
+// the longest (<= 32 planes) that still gives >= 512 tiles.  2-D: rows of 62
+// columns marching over `chunk` rows.  FVM_TILE / FVM_CHUNK override.
 void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   const BlockGeom& g = b->g;
   const char* et = getenv("FVM_TILE");
   const char* ec = getenv("FVM_CHUNK");
-  char tbuf[16], cbuf[16];
-  if (force_tile >= 0) { snprintf(tbuf, sizeof tbuf, "%d", force_tile); et = tbuf; }
-  if (force_chunk > 0) { snprintf(cbuf, sizeof cbuf, "%d", force_chunk); ec = cbuf; }
+  int tile_env = et ? std::min(std::max(atoi(et), 0), kNumTiles3d - 1) : -1;
+  int chunk_env = ec ? std::max(1, atoi(ec)) : -1;
+  if (force_tile >= 0) tile_env = force_tile;
+  if (force_chunk > 0) chunk_env = force_chunk;
   if (b->dim == 2) {
     b->tiles_x = (g.nx + 61) / 62;
     b->tiles_y = 1;
     int chunk = 16;
     while (chunk > 2 && (int64_t)b->tiles_x * ((g.ny + chunk - 1) / chunk) < 2048) chunk /= 2;
-    if (ec) chunk = std::max(1, atoi(ec));
+    if (chunk_env > 0) chunk = chunk_env;
     b->chunk = chunk;
     b->tiles_z = (g.ny + chunk - 1) / chunk;
   } else {
     const int w64 = (g.nx + 63) / 64 * 64 - g.nx, w32 = (g.nx + 31) / 32 * 32 - g.nx;
     int tile = w32 <= w64 ? 1 : 0;
-    if (et) tile = std::min(std::max(atoi(et), 0), kNumTiles3d - 1);
+    if (tile_env >= 0) tile = tile_env;
     b->tile = tile;
     const int tx = kTile3d[tile][0], ty = kTile3d[tile][1];
     b->tiles_x = (g.nx + tx - 1) / tx;
@@ -195,11 +228,47 @@ void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
     int chunk = 32;
     while (chunk > 8 && (int64_t)b->tiles_x * b->tiles_y * ((g.nz + chunk - 1) / chunk) < 512)
       chunk /= 2;
-    if (ec) chunk = std::max(1, atoi(ec));
+    if (chunk_env > 0) chunk = chunk_env;
     b->chunk = chunk;
     b->tiles_z = (g.nz + chunk - 1) / chunk;
   }
-  b->nblocks = b->tiles_x * b->tiles_y * b->tiles_z;
+  b->ntiles = b->tiles_x * b->tiles_y * b->tiles_z;
+  if (b->ctx) b->ctx->desc_dirty = true;
+}
+
+// Upload the descriptor table (blocks in context order, tiles numbered
+// block after block).
+int sync_desc(fvm_ctx* ctx) {
+  if (!ctx->desc_dirty) return FVM_OK;
+  std::vector<BlockDesc> h(ctx->blocks.size());
+  int base = 0;
+  for (size_t i = 0; i < ctx->blocks.size(); ++i) {
+    fvm_block* b = ctx->blocks[i];
+    BlockDesc& d = h[i];
+    for (int k = 0; k < 4; ++k) d.q[k] = b->q[k];
+    d.g = b->g;
+    for (int f = 0; f < 6; ++f) { d.nbr[f] = b->nbr[f]; d.nbr_shared[f] = b->nbr_shared[f]; }
+    d.tiles_x = b->tiles_x; d.tiles_y = b->tiles_y; d.tiles_z = b->tiles_z; d.chunk = b->chunk;
+    d.tile_base = base;
+    b->tile_base = base;
+    base += b->ntiles;
+  }
+  ctx->total_tiles = base;
+  if (h.size() > ctx->desc_cap) {
+    if (ctx->d_desc) HIP_TRY(hipFree(ctx->d_desc));
+    ctx->d_desc = nullptr;
+    const size_t cap = std::max<size_t>(h.size(), 16);
+    if (hipMalloc(&ctx->d_desc, cap * sizeof(BlockDesc)) != hipSuccess)
+      return fail(FVM_ENOMEM, "cannot allocate the block table");
+    ctx->desc_cap = cap;
+  }
+  if (!h.empty())
+    HIP_TRY(hipMemcpyAsync(ctx->d_desc, h.data(), h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                           ctx->stream));
+  // the staging vector is released at return: wait for the copy
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->desc_dirty = false;
+  return FVM_OK;
 }
 
 int ensure_norm_capacity(fvm_ctx* ctx, size_t doubles) {
@@ -212,29 +281,34 @@ int ensure_norm_capacity(fvm_ctx* ctx, size_t doubles) {
   return FVM_OK;
 }
 
-int fill_ghosts(fvm_block* b, double* q) {
-  for (int d = 0; d < b->dim; ++d)
-    for (int side = 0; side < 2; ++side) {
-      hipLaunchKernelGGL(ghost_fill_kernel, dim3(256), dim3(256), 0, b->ctx->stream, q, b->g,
-                         b->nc, d, side);
-      HIP_TRY(hipGetLastError());
-    }
+// Refill the ghost layers of state array `buf` of every block.
+int fill_ghosts_all(fvm_ctx* ctx, int buf) {
+  if (int rc = sync_desc(ctx)) return rc;
+  for (size_t i = 0; i < ctx->blocks.size(); ++i) {
+    fvm_block* b = ctx->blocks[i];
+    for (int d = 0; d < b->dim; ++d)
+      for (int side = 0; side < 2; ++side) {
+        hipLaunchKernelGGL(ghost_fill_kernel, dim3(256), dim3(256), 0, ctx->stream, ctx->d_desc, (int)i,
+                           buf, b->nc, d, side);
+        HIP_TRY(hipGetLastError());
+      }
+  }
   return FVM_OK;
 }
 
 template <class Fam>
-hipError_t launch_family(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
-  if (b->dim == 2) {
+hipError_t launch_family(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
+  if (dim == 2) {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s2(ctx->flux, a, b->nblocks, ctx->stream);
-      case RECON_MUSCL: return Fam::m2(ctx->flux, a, b->nblocks, ctx->stream);
-      case RECON_WENO: return Fam::w2(ctx->flux, a, b->nblocks, ctx->stream);
+      case RECON_SIMPLE: return Fam::s2(ctx->flux, a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m2(ctx->flux, a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w2(ctx->flux, a, grid, ctx->stream);
     }
   } else {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
-      case RECON_MUSCL: return Fam::m3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
-      case RECON_WENO: return Fam::w3(ctx->flux, b->tile, a, b->nblocks, ctx->stream);
+      case RECON_SIMPLE: return Fam::s3(ctx->flux, tile, a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m3(ctx->flux, tile, a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w3(ctx->flux, tile, a, grid, ctx->stream);
     }
   }
   return hipErrorInvalidValue;
@@ -257,9 +331,45 @@ struct FastFam {
   static constexpr auto w3 = fvm::fast::launch_stage3d_weno;
 };
 
-int launch_stage(fvm_ctx* ctx, fvm_block* b, const StageArgs& a) {
-  const hipError_t e = ctx->fast ? launch_family<FastFam>(ctx, b, a) : launch_family<ExactFam>(ctx, b, a);
+int launch_one(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
+  if (grid <= 0 || a.ntiles <= 0) return FVM_OK;
+  const hipError_t e = ctx->fast ? launch_family<FastFam>(ctx, dim, tile, a, grid)
+                                 : launch_family<ExactFam>(ctx, dim, tile, a, grid);
   if (e != hipSuccess) return fail(FVM_EHIP, "stage launch: %s", hipGetErrorString(e));
+  return FVM_OK;
+}
+
+// Can all blocks go in one launch (same kernel instantiation)?
+bool batchable(fvm_ctx* ctx) {
+  for (fvm_block* b : ctx->blocks)
+    if (b->dim != ctx->blocks[0]->dim || (b->dim == 3 && b->tile != ctx->blocks[0]->tile))
+      return false;
+  return true;
+}
+
+// Grid of a launch over `ntiles` tiles: one workgroup per tile, or in
+// persistent mode at most two per CU (a multiple of 8, so each workgroup
+// stays on one XCD for every tile it loops over).
+int grid_for(fvm_ctx* ctx, int ntiles) {
+  if (ctx->launch_mode != FVM_LAUNCH_PERSISTENT) return ntiles;
+  return std::min(ntiles, 2 * ctx->num_cu);
+}
+
+// One RK stage over every block: a single launch over the whole tile list
+// (batched or persistent), or one launch per block.
+int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
+  a.blocks = ctx->d_desc;
+  a.nblocks = (int)ctx->blocks.size();
+  if (ctx->launch_mode != FVM_LAUNCH_PER_BLOCK && batchable(ctx)) {
+    a.tile0 = 0;
+    a.ntiles = ctx->total_tiles;
+    return launch_one(ctx, ctx->blocks[0]->dim, ctx->blocks[0]->tile, a, grid_for(ctx, a.ntiles));
+  }
+  for (fvm_block* b : ctx->blocks) {
+    a.tile0 = b->tile_base;
+    a.ntiles = b->ntiles;
+    if (int rc = launch_one(ctx, b->dim, b->tile, a, grid_for(ctx, b->ntiles))) return rc;
+  }
   return FVM_OK;
 }
 
@@ -272,7 +382,7 @@ extern "C" {
 
 const char* fvm_last_error(void) { return g_err.c_str(); }
 
-const char* fvm_version(void) { return "fvm-mi355x 0.1.0 (gfx950, fp64)"; }
+const char* fvm_version(void) { return "fvm-mi355x 0.2.0 (gfx950, fp64)"; }
 
 int fvm_ctx_create(int device, fvm_ctx** out) {
   if (!out) return fail(FVM_EINVAL, "null output pointer");
@@ -282,6 +392,9 @@ int fvm_ctx_create(int device, fvm_ctx** out) {
   fvm_ctx* ctx = new fvm_ctx();
   ctx->device = device;
   HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cu = prop.multiProcessorCount;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc(&ctx->d_dt, sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_smax, sizeof(unsigned long long)));
@@ -296,10 +409,11 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (fvm_block* b : ctx->blocks) {
-    (void)hipFree(b->qn); (void)hipFree(b->qa); (void)hipFree(b->qb); (void)hipFree(b->acc);
+    for (double* p : b->q) (void)hipFree(p);
     delete b;
   }
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  (void)hipFree(ctx->d_desc);
   (void)hipFree(ctx->d_dt);
   (void)hipFree(ctx->d_smax);
   (void)hipFree(ctx->d_norm);
@@ -341,18 +455,29 @@ int fvm_set_numerics(fvm_ctx* ctx, int mode) {
   return FVM_OK;
 }
 
+int fvm_set_launch_mode(fvm_ctx* ctx, int mode) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (mode != FVM_LAUNCH_PER_BLOCK && mode != FVM_LAUNCH_BATCHED && mode != FVM_LAUNCH_PERSISTENT)
+    return fail(FVM_EINVAL, "unknown launch mode %d", mode);
+  ctx->launch_mode = mode;
+  return FVM_OK;
+}
+
 int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
                      double dz, fvm_block** out) {
   if (int rc = check_ctx(ctx)) return rc;
   if (!out) return fail(FVM_EINVAL, "null output pointer");
-  if (ni < 2 || nj < 2 || nk < 1) return fail(FVM_EINVAL, "block needs ni, nj >= 2 and nk >= 1 (got %lld %lld %lld)",
-                                              (long long)ni, (long long)nj, (long long)nk);
+  if (ni < 4 || nj < 4 || nk < 1)
+    return fail(FVM_EINVAL, "block needs ni, nj >= 4 and nk >= 1 (got %lld %lld %lld)",
+                (long long)ni, (long long)nj, (long long)nk);
   if (ni > (1 << 24) || nj > (1 << 24) || nk > (1 << 24))
     return fail(FVM_EINVAL, "block extent too large");
   const int dim = nk == 1 ? 2 : 3;
-  if (dim == 3 && nk < 2) return fail(FVM_EINVAL, "3-D block needs nk >= 2");
+  if (dim == 3 && nk < 4) return fail(FVM_EINVAL, "3-D block needs nk >= 4");
   if (!(dx > 0) || !(dy > 0) || (dim == 3 && !(dz > 0)))
     return fail(FVM_EINVAL, "vertex spacing must be positive");
+  if (!ctx->blocks.empty() && ctx->blocks[0]->dim != dim)
+    return fail(FVM_EINVAL, "all blocks of a context must have the same dimension");
   if (int rc = set_device(ctx)) return rc;
   fvm_block* b = new fvm_block();
   b->ctx = ctx;
@@ -371,17 +496,18 @@ int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx
   g.ng = kGhost;
   b->state_elems = (size_t)b->nc * g.cs;
   const size_t bytes = b->state_elems * sizeof(double);
-  double** bufs[4] = {&b->qn, &b->qa, &b->qb, &b->acc};
-  for (double** p : bufs) {
-    if (hipMalloc(p, bytes) != hipSuccess) {
-      for (double** r : bufs) if (*r) (void)hipFree(*r);
+  for (int k = 0; k < 4; ++k) {
+    if (hipMalloc(&b->q[k], bytes) != hipSuccess) {
+      for (double* p : b->q) if (p) (void)hipFree(p);
       delete b;
       return fail(FVM_ENOMEM, "cannot allocate %zu bytes of block state", bytes);
     }
-    HIP_TRY(hipMemsetAsync(*p, 0, bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(b->q[k], 0, bytes, ctx->stream));
   }
   plan_launch(b);
   ctx->blocks.push_back(b);
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
   *out = b;
   return FVM_OK;
 }
@@ -391,9 +517,18 @@ int fvm_block_destroy(fvm_block* b) {
   fvm_ctx* ctx = b->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(b->qn); (void)hipFree(b->qa); (void)hipFree(b->qb); (void)hipFree(b->acc);
+  const int idx = block_index(ctx, b);
+  // drop connections to this block and renumber the rest
+  for (fvm_block* o : ctx->blocks)
+    for (int f = 0; f < 6; ++f) {
+      if (o->nbr[f] == idx) { o->nbr[f] = -1; o->g.bc[f] = BC_EXTRAP; }
+      else if (o->nbr[f] > idx) --o->nbr[f];
+    }
+  for (double* p : b->q) (void)hipFree(p);
   ctx->blocks.erase(std::remove(ctx->blocks.begin(), ctx->blocks.end(), b), ctx->blocks.end());
   delete b;
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
   return FVM_OK;
 }
 
@@ -402,16 +537,40 @@ int fvm_block_set_bc(fvm_block* b, int face, int kind) {
   if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
   if (kind != FVM_BC_EXTRAP && kind != FVM_BC_PERIODIC) return fail(FVM_EINVAL, "unknown BC kind %d", kind);
   if (b->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
+  const int d = face / 2, other = 2 * d + (1 - face % 2);
+  if (kind == FVM_BC_PERIODIC && (b->g.bc[face] == BC_CONN || b->g.bc[other] == BC_CONN))
+    return fail(FVM_EINVAL, "face %d is connected to another block", face);
+  if (b->g.bc[face] == BC_CONN) return fail(FVM_EINVAL, "face %d is connected to another block", face);
   b->g.bc[face] = kind;
-  // periodic pairs must agree
-  const int d = face / 2;
-  const int other = b->g.bc[2 * d + (1 - face % 2)];
-  if (kind == FVM_BC_PERIODIC && other != FVM_BC_PERIODIC) b->g.bc[2 * d + (1 - face % 2)] = FVM_BC_PERIODIC;
-  if (kind == FVM_BC_EXTRAP && other == FVM_BC_PERIODIC) b->g.bc[2 * d + (1 - face % 2)] = FVM_BC_EXTRAP;
-  if (b->uploaded) {
-    if (int rc = set_device(b->ctx)) return rc;
-    return fill_ghosts(b, b->qn);
+  if (kind == FVM_BC_PERIODIC) b->g.bc[other] = kind;              // periodic on both faces
+  else if (b->g.bc[other] == BC_PERIODIC) b->g.bc[other] = BC_EXTRAP;
+  b->ctx->desc_dirty = true;
+  b->ctx->ghosts_dirty = true;
+  return FVM_OK;
+}
+
+int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
+  if (!a || !b) return fail(FVM_EINVAL, "null block");
+  if (a->ctx != b->ctx) return fail(FVM_EINVAL, "blocks belong to different contexts");
+  if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
+  if (a->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
+  const int d = face / 2, bface = face ^ 1;
+  const int na[3] = {a->g.nx, a->g.ny, a->g.nz}, nb[3] = {b->g.nx, b->g.ny, b->g.nz};
+  for (int e = 0; e < 3; ++e)
+    if (e != d && na[e] != nb[e])
+      return fail(FVM_EINVAL, "faces do not conform: extent %d is %d vs %d", e, na[e], nb[e]);
+  if (na[d] < 2 * kGhost || nb[d] < 2 * kGhost)
+    return fail(FVM_EINVAL, "connected blocks need >= %d vertices across the face", 2 * kGhost);
+  if (a == b) {
+    if (!shared) return fvm_block_set_bc(a, face, FVM_BC_PERIODIC);
+    return fail(FVM_EINVAL, "a block cannot share a vertex plane with itself");
   }
+  fvm_ctx* ctx = a->ctx;
+  const int ia = block_index(ctx, a), ib = block_index(ctx, b);
+  a->g.bc[face] = BC_CONN; a->nbr[face] = ib; a->nbr_shared[face] = shared ? 1 : 0;
+  b->g.bc[bface] = BC_CONN; b->nbr[bface] = ia; b->nbr_shared[bface] = shared ? 1 : 0;
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
   return FVM_OK;
 }
 
@@ -431,12 +590,11 @@ int fvm_upload(fvm_block* b, const double* q, int host_ng) {
     int dc = c;
     if (b->dim == 2) {
       if (c == 3) {
-        for (int64_t k = 0; k < g.nz; ++k)
-          for (int64_t j = 0; j < g.ny; ++j)
-            for (int64_t i = 0; i < g.nx; ++i) {
-              const double v = q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + i + host_ng];
-              if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
-            }
+        for (int64_t j = 0; j < g.ny; ++j)
+          for (int64_t i = 0; i < g.nx; ++i) {
+            const double v = q[c * hcs + (j + host_ng) * hx + i + host_ng];
+            if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
+          }
         continue;
       }
       if (c == 4) dc = 3;
@@ -448,11 +606,11 @@ int fvm_upload(fvm_block* b, const double* q, int host_ng) {
         std::memcpy(dst, src, g.nx * sizeof(double));
       }
   }
-  HIP_TRY(hipMemcpyAsync(b->qn, stage.data(), b->state_elems * sizeof(double), hipMemcpyHostToDevice,
+  HIP_TRY(hipMemcpyAsync(b->q[0], stage.data(), b->state_elems * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
-  if (int rc = fill_ghosts(b, b->qn)) return rc;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   b->uploaded = true;
+  ctx->ghosts_dirty = true;
   return FVM_OK;
 }
 
@@ -461,9 +619,17 @@ int fvm_download(fvm_block* b, double* q, int host_ng) {
   if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
   fvm_ctx* ctx = b->ctx;
   if (int rc = set_device(ctx)) return rc;
+  if (ctx->ghosts_dirty && b->uploaded) {
+    bool all = true;
+    for (fvm_block* o : ctx->blocks) all = all && o->uploaded;
+    if (all) {
+      if (int rc = fill_ghosts_all(ctx, 0)) return rc;
+      ctx->ghosts_dirty = false;
+    }
+  }
   const BlockGeom& g = b->g;
   std::vector<double> stage((size_t)b->state_elems);
-  HIP_TRY(hipMemcpyAsync(stage.data(), b->qn, b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
+  HIP_TRY(hipMemcpyAsync(stage.data(), b->q[0], b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   const int gl = std::min(host_ng, g.ng);
@@ -501,19 +667,24 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   if (int rc = set_device(ctx)) return rc;
   ctx->have_norms = false;
   if (nsteps == 0) return FVM_OK;
+  if (int rc = sync_desc(ctx)) return rc;
+  if (ctx->ghosts_dirty) {
+    if (int rc = fill_ghosts_all(ctx, 0)) return rc;
+    ctx->ghosts_dirty = false;
+  }
 
-  // norm partial buffer: NC doubles per workgroup of every block
-  size_t need = 0;
-  for (fvm_block* b : ctx->blocks) need += (size_t)b->nblocks * b->nc;
+  // norm partials: NC doubles per tile
+  const int nc = ctx->blocks[0]->nc;
+  const size_t need = (size_t)ctx->total_tiles * nc;
   if (int rc = ensure_norm_capacity(ctx, need)) return rc;
 
   if (cfl_mode) {
     HIP_TRY(hipMemsetAsync(ctx->d_smax, 0, sizeof(unsigned long long), ctx->stream));
     for (fvm_block* b : ctx->blocks) {
       if (b->nc == 5)
-        hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->qn, b->g, ctx->P.gamma, ctx->d_smax);
+        hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g, ctx->P.gamma, ctx->d_smax);
       else
-        hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->qn, b->g, ctx->P.gamma, ctx->d_smax);
+        hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g, ctx->P.gamma, ctx->d_smax);
       HIP_TRY(hipGetLastError());
     }
   }
@@ -521,7 +692,7 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   size_t nev = 0;
   if (ctx->timing) {
-    const size_t need_ev = 2 * (size_t)nsteps * nstage * ctx->blocks.size();
+    const size_t need_ev = 2 * (size_t)nsteps * nstage;
     while (ctx->ev.size() < need_ev) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -535,44 +706,38 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     }
     for (int st = 1; st <= nstage; ++st) {
       const bool last = st == nstage;
-      size_t part_off = 0;
-      for (fvm_block* b : ctx->blocks) {
-        StageArgs a{};
-        a.g = b->g;
-        a.P = ctx->P;
-        a.dt = cfl_mode ? 0.0 : dt;
-        a.dt_dev = cfl_mode ? ctx->d_dt : nullptr;
-        a.tiles_x = b->tiles_x; a.tiles_y = b->tiles_y; a.tiles_z = b->tiles_z;
-        a.chunk = b->chunk;
-        a.q0 = b->qn;
-        a.acc = nullptr;
-        if (ctx->rk == RK4) {
-          a.acc = b->acc;
-          switch (st) {
-            case 1: a.qin = b->qn; a.qout = b->qa; a.stage = ST_RK4_1; break;
-            case 2: a.qin = b->qa; a.qout = b->qb; a.stage = ST_RK4_2; break;
-            case 3: a.qin = b->qb; a.qout = b->qa; a.stage = ST_RK4_3; break;
-            default: a.qin = b->qa; a.qout = b->qn; a.stage = ST_RK4_4; break;
-          }
-        } else {
-          switch (st) {
-            case 1: a.qin = b->qn; a.qout = b->qa; a.stage = ST_SSP1; break;
-            case 2: a.qin = b->qa; a.qout = b->qb; a.stage = ST_SSP2; break;
-            default: a.qin = b->qb; a.qout = b->qn; a.stage = ST_SSP3; break;
-          }
+      StageArgs a{};
+      a.P = ctx->P;
+      a.dt = cfl_mode ? 0.0 : dt;
+      a.dt_dev = cfl_mode ? ctx->d_dt : nullptr;
+      a.q0_buf = 0;
+      a.use_acc = 0;
+      if (ctx->rk == RK4) {
+        a.use_acc = 1;
+        switch (st) {
+          case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_RK4_1; break;
+          case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_RK4_2; break;
+          case 3: a.in_buf = 2; a.out_buf = 1; a.stage = ST_RK4_3; break;
+          default: a.in_buf = 1; a.out_buf = 0; a.stage = ST_RK4_4; break;
         }
-        a.norm_part = (last && s == nsteps - 1) ? ctx->d_norm + part_off : nullptr;
-        a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
-        part_off += (size_t)b->nblocks * b->nc;
-        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
-        if (int rc = launch_stage(ctx, b, a)) return rc;
-        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
+      } else {
+        switch (st) {
+          case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_SSP1; break;
+          case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_SSP2; break;
+          default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
+        }
       }
+      a.norm_part = (last && s == nsteps - 1) ? ctx->d_norm : nullptr;
+      a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
+      if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
+      if (int rc = launch_stage_all(ctx, a)) return rc;
+      if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
     }
   }
 
-  // residual norms of the last stage (class_iteration.f90:210-248), summed
-  // over workgroup partials in a fixed order
+  // residual norms of the last stage (class_iteration.f90:210-248): per-tile
+  // partials summed in tile order per block; blocks combined as the
+  // reference accumulates them, R = sqrt(R_prev^2 + R_blk^2)
   ctx->h_norm.resize(need);
   HIP_TRY(hipMemcpyAsync(ctx->h_norm.data(), ctx->d_norm, need * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
@@ -591,16 +756,14 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
       ctx->cell_updates += (double)b->g.nx * b->g.ny * b->g.nz * nstage * nsteps;
   }
   double res_abs[6] = {0, 0, 0, 0, 0, 0};
-  size_t off = 0;
   for (fvm_block* b : ctx->blocks) {
     double sums[5] = {0, 0, 0, 0, 0};
-    for (int w = 0; w < b->nblocks; ++w)
-      for (int c = 0; c < b->nc; ++c) sums[c] += ctx->h_norm[off + (size_t)w * b->nc + c];
-    off += (size_t)b->nblocks * b->nc;
+    for (int w = 0; w < b->ntiles; ++w)
+      for (int c = 0; c < nc; ++c) sums[c] += ctx->h_norm[(size_t)(b->tile_base + w) * nc + c];
     double res2[5] = {0, 0, 0, 0, 0};
     const double npts = (double)b->g.nx * b->g.ny * b->g.nz;
-    for (int c = 0; c < b->nc; ++c) {
-      const int e = (b->nc == 4 && c == 3) ? 4 : c;   // 2-D: energy is equation 5
+    for (int c = 0; c < nc; ++c) {
+      const int e = (nc == 4 && c == 3) ? 4 : c;   // 2-D: energy is equation 5
       res2[e] = sums[c] / npts;
     }
     double res2t = 0.0;
@@ -630,47 +793,64 @@ int fvm_norms(fvm_ctx* ctx, double out[6], int* nan_flag) {
   return FVM_OK;
 }
 
-// Pick each block's launch shape by timing one stage launch per candidate
-// (3-D: every compiled tile x chunk in {4, 8, 16, 32}; 2-D: chunk in
-// {2, 4, 8, 16}).  The probe is the scheme's first stage, which reads the
-// current state and writes only the stage buffers, so the state is untouched.
+// Pick the launch shape by timing launches of the first RK stage per
+// candidate (3-D: every compiled tile x chunk in {4, 8, 16, 32}; 2-D: chunk in
+// {2, 4, 8, 16}).  Batched modes choose one shape for all blocks (they share a
+// launch); per-block mode tunes each block alone.  The probe reads u^n and
+// writes only stage buffers, so the state is untouched.
 int fvm_autotune(fvm_ctx* ctx, int reps) {
   if (int rc = check_ctx(ctx)) return rc;
   if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
+  for (fvm_block* b : ctx->blocks)
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
   if (reps < 1) reps = 1;
   if (int rc = set_device(ctx)) return rc;
+  if (int rc = sync_desc(ctx)) return rc;
+  if (ctx->ghosts_dirty) {
+    if (int rc = fill_ghosts_all(ctx, 0)) return rc;
+    ctx->ghosts_dirty = false;
+  }
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  for (fvm_block* b : ctx->blocks) {
-    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
-    const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
+  const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
+  const int dim = ctx->blocks[0]->dim;
+  const bool together = ctx->launch_mode != FVM_LAUNCH_PER_BLOCK;
+  const size_t groups = together ? 1 : ctx->blocks.size();
+  for (size_t gi = 0; gi < groups; ++gi) {
+    std::vector<fvm_block*> sel;
+    if (together) sel = ctx->blocks; else sel.push_back(ctx->blocks[gi]);
     float best = 1e30f;
-    int best_tile = b->tile, best_chunk = b->chunk;
-    const int ntiles = b->dim == 3 ? kNumTiles3d : 1;
+    int best_tile = sel[0]->tile, best_chunk = sel[0]->chunk;
+    const int ntiles = dim == 3 ? kNumTiles3d : 1;
     for (int tile = 0; tile < ntiles; ++tile)
       for (int ci = 0; ci < 4; ++ci) {
-        const int chunk = b->dim == 3 ? chunks3[ci] : chunks2[ci];
-        plan_launch(b, b->dim == 3 ? tile : -1, chunk);
+        const int chunk = dim == 3 ? chunks3[ci] : chunks2[ci];
+        for (fvm_block* b : sel) plan_launch(b, dim == 3 ? tile : -1, chunk);
+        if (int rc = sync_desc(ctx)) return rc;
         StageArgs a{};
-        a.g = b->g; a.P = ctx->P; a.dt = 1.0e-6; a.dt_dev = nullptr;
-        a.tiles_x = b->tiles_x; a.tiles_y = b->tiles_y; a.tiles_z = b->tiles_z;
-        a.chunk = b->chunk;
-        a.qin = b->qn; a.q0 = b->qn; a.qout = b->qa;
-        a.acc = ctx->rk == RK4 ? b->acc : nullptr;
+        a.P = ctx->P; a.dt = 1.0e-6; a.dt_dev = nullptr;
+        a.in_buf = 0; a.q0_buf = 0; a.out_buf = 1;
+        a.use_acc = ctx->rk == RK4 ? 1 : 0;
         a.stage = ctx->rk == RK4 ? ST_RK4_1 : ST_SSP1;
-        if (int rc = launch_stage(ctx, b, a)) return rc;   // warm
+        a.blocks = ctx->d_desc; a.nblocks = (int)ctx->blocks.size();
+        a.tile0 = together ? 0 : sel[0]->tile_base;
+        a.ntiles = together ? ctx->total_tiles : sel[0]->ntiles;
+        const int grid = grid_for(ctx, a.ntiles);
+        if (int rc = launch_one(ctx, dim, sel[0]->tile, a, grid)) return rc;   // warm
         HIP_TRY(hipEventRecord(e0, ctx->stream));
         for (int r = 0; r < reps; ++r)
-          if (int rc = launch_stage(ctx, b, a)) return rc;
+          if (int rc = launch_one(ctx, dim, sel[0]->tile, a, grid)) return rc;
         HIP_TRY(hipEventRecord(e1, ctx->stream));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_tile = b->tile; best_chunk = b->chunk; }
+        if (ms < best) { best = ms; best_tile = sel[0]->tile; best_chunk = sel[0]->chunk; }
       }
-    plan_launch(b, b->dim == 3 ? best_tile : -1, best_chunk);
+    for (fvm_block* b : sel) plan_launch(b, dim == 3 ? best_tile : -1, best_chunk);
   }
+  if (int rc = sync_desc(ctx)) return rc;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return FVM_OK;

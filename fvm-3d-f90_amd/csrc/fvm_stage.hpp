@@ -164,10 +164,24 @@ __device__ __forceinline__ double wave_speed(const double q[NC], const BlockGeom
   return s;
 }
 
-// Write one output vertex and its ghost images along every swept direction.
+// Index of (i, j, k) in a block whose geometry is read from the descriptor
+// table (neighbour writes).
+__device__ __forceinline__ int64_t gidx_of(const BlockDesc* __restrict__ tab, int blk, int i,
+                                           int j, int k) {
+  const BlockGeom& g = tab[blk].g;
+  return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
+}
+
+// Write one output vertex and its ghost images along every swept direction:
+// its own ghosts (extrapolation: copies of the boundary vertex; periodic: the
+// vertex n periods away) and, across connected faces, the neighbour block's
+// ghosts it donates to (shared interface vertex: vertex n-1-l of this block is
+// ghost -l of the next; abutting blocks: vertex n-l is ghost -l).
 template <int NC, int NDIR>
-__device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, double* __restrict__ out,
-                                                  int i, int j, int k, const double v[NC]) {
+__device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, const BlockDesc* __restrict__ tab,
+                                                  const BlockDesc& B, int out_buf,
+                                                  double* __restrict__ out, int i, int j, int k,
+                                                  const double v[NC]) {
   const int64_t base = gidx(g, i, j, k);
 #pragma unroll
   for (int c = 0; c < NC; ++c) out[base + c * g.cs] = v[c];
@@ -176,27 +190,58 @@ __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, double* __
 #pragma unroll
   for (int d = 0; d < NDIR; ++d) {
     const int64_t stride = d == 0 ? 1 : (d == 1 ? g.px : g.pxy);
+    const int blo = g.bc[2 * d], bhi = g.bc[2 * d + 1];
     for (int l = 1; l <= g.ng; ++l) {
-      // low side
+      // own ghosts, low side
       int tgt = -1000000;
-      if (g.bc[2 * d] == BC_PERIODIC) { if (x[d] == n[d] - l) tgt = -l; }
-      else if (x[d] == 0) tgt = -l;
+      if (blo == BC_PERIODIC) { if (x[d] == n[d] - l) tgt = -l; }
+      else if (blo == BC_EXTRAP && x[d] == 0) tgt = -l;
       if (tgt != -1000000) {
         const int64_t o = base + (int64_t)(tgt - x[d]) * stride;
 #pragma unroll
         for (int c = 0; c < NC; ++c) out[o + c * g.cs] = v[c];
       }
-      // high side
+      // own ghosts, high side
       tgt = -1000000;
-      if (g.bc[2 * d + 1] == BC_PERIODIC) { if (x[d] == l - 1) tgt = n[d] - 1 + l; }
-      else if (x[d] == n[d] - 1) tgt = n[d] - 1 + l;
+      if (bhi == BC_PERIODIC) { if (x[d] == l - 1) tgt = n[d] - 1 + l; }
+      else if (bhi == BC_EXTRAP && x[d] == n[d] - 1) tgt = n[d] - 1 + l;
       if (tgt != -1000000) {
         const int64_t o = base + (int64_t)(tgt - x[d]) * stride;
 #pragma unroll
         for (int c = 0; c < NC; ++c) out[o + c * g.cs] = v[c];
+      }
+      // neighbour across the low face: its ghost n_nb - 1 + l
+      if (blo == BC_CONN && x[d] == (B.nbr_shared[2 * d] ? l : l - 1)) {
+        const int nb = B.nbr[2 * d];
+        int y[3] = {x[0], x[1], x[2]};
+        const int nn[3] = {tab[nb].g.nx, tab[nb].g.ny, tab[nb].g.nz};
+        y[d] = nn[d] - 1 + l;
+        double* dst = tab[nb].q[out_buf];
+        const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
+        const int64_t cs = tab[nb].g.cs;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
+      }
+      // neighbour across the high face: its ghost -l
+      if (bhi == BC_CONN && x[d] == (B.nbr_shared[2 * d + 1] ? n[d] - 1 - l : n[d] - l)) {
+        const int nb = B.nbr[2 * d + 1];
+        int y[3] = {x[0], x[1], x[2]};
+        y[d] = -l;
+        double* dst = tab[nb].q[out_buf];
+        const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
+        const int64_t cs = tab[nb].g.cs;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
       }
     }
   }
+}
+
+// Block of a tile in the launch's tile list (uniform per workgroup).
+__device__ __forceinline__ int block_of_tile(const BlockDesc* __restrict__ tab, int nblocks, int tile) {
+  int b = 0;
+  while (b + 1 < nblocks && tab[b + 1].tile_base <= tile) ++b;
+  return b;
 }
 
 // Workgroup reduction helpers (deterministic order).
@@ -235,7 +280,7 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
 // 2-D stage: tile of TX = 62 columns, one wave, marching along j.
 // Thread t owns column c = i0 - 1 + t (t = 0..63): it keeps the j-queue of
 // its column, reconstructs its vertex along i, and for t in 1..62 updates
-// cell c.
+// cell c.  A workgroup loops over tiles of the launch's tile list.
 // ============================================================================
 template <class RC, int FLUX>
 __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
@@ -249,128 +294,138 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
   __shared__ double Fx[NC][NT];
   __shared__ double red[NT / 64];
 
-  const BlockGeom& g = a.g;
   const Params& P = a.P;
   const int t = threadIdx.x;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int ti = bid % a.tiles_x, tj = bid / a.tiles_x;
-  const int i0 = ti * TX;
-  const int j0 = tj * a.chunk;
-  const int j1 = min(j0 + a.chunk, g.ny);
-  const int c = i0 - 1 + t;                       // owned column
-  const int cl = clampi(c, -H, g.nx + H - 1);
-  const bool owner = (t >= 1) && (t <= TX) && (c < g.nx);
   const double dt = a.dt_dev ? *a.dt_dev : a.dt;
   const RKCoef kc = rk_coef(a.stage, dt);
-
-  // j-queue of column c: Vq[w] = V(c, jc - QOFF + w), jc = vertex being reconstructed
-  double Vq[NW][NC];
-  auto loadV = [&](int j, double v[NC]) {
-    double raw[NC];
-    load_raw<RC, NC>(a.qin, g.cs, gidx(g, cl, clampi(j, -H, g.ny + H - 1), 0), raw);
-    to_recon_vars<RC, NC>(raw, v, P.gamma);
-  };
-#pragma unroll
-  for (int w = 0; w < NW; ++w) loadV(j0 - 1 - QOFF + w, Vq[w]);
-  double Lc[NC], Fc[NC], Lt[NC], Rt[NC];
-  // vertex j0-1: carry its L
-  RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
-  // vertex j0: face j0 = F(L(j0-1), R(j0))
-#pragma unroll
-  for (int w = 0; w < NW - 1; ++w)
-#pragma unroll
-    for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-  loadV(j0 + RV, Vq[NW - 1]);
-  RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-  face_flux<FLUX, NC, 1>(Lc, Rt, P, Fc);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
-
-  double nrm[NC];
-#pragma unroll
-  for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
   double smax = 0.0;
 
-  for (int j = j0; j < j1; ++j) {
-    // shift: centre becomes j + 1; Vq[QOFF - 1] is V(j)
+  for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
+    const int tile = a.tile0 + xcd_remap(it, a.ntiles);
+    const int bi = block_of_tile(a.blocks, a.nblocks, tile);
+    const BlockDesc& B = a.blocks[bi];
+    const BlockGeom g = B.g;
+    const double* __restrict__ qin = B.q[a.in_buf];
+    const double* __restrict__ q0p = B.q[a.q0_buf];
+    double* __restrict__ qout = B.q[a.out_buf];
+    double* __restrict__ acc = a.use_acc ? B.q[3] : nullptr;
+    const int lt = tile - B.tile_base;
+    const int ti = lt % B.tiles_x, tj = lt / B.tiles_x;
+    const int i0 = ti * TX;
+    const int j0 = tj * B.chunk;
+    const int j1 = min(j0 + B.chunk, g.ny);
+    const int c = i0 - 1 + t;                       // owned column
+    const int cl = clampi(c, -H, g.nx + H - 1);
+    const bool owner = (t >= 1) && (t <= TX) && (c < g.nx);
+
+    // j-queue of column c: Vq[w] = V(c, jc - QOFF + w), jc = vertex being reconstructed
+    double Vq[NW][NC];
+    auto loadV = [&](int j, double v[NC]) {
+      double raw[NC];
+      load_raw<RC, NC>(qin, g.cs, gidx(g, cl, clampi(j, -H, g.ny + H - 1), 0), raw);
+      to_recon_vars<RC, NC>(raw, v, P.gamma);
+    };
+#pragma unroll
+    for (int w = 0; w < NW; ++w) loadV(j0 - 1 - QOFF + w, Vq[w]);
+    double Lc[NC], Fc[NC], Lt[NC], Rt[NC];
+    // vertex j0-1: carry its L
+    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+    // vertex j0: face j0 = F(L(j0-1), R(j0))
 #pragma unroll
     for (int w = 0; w < NW - 1; ++w)
 #pragma unroll
       for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-    loadV(j + 1 + RV, Vq[NW - 1]);
-    // row j into LDS: owned columns plus RV far-halo columns on each side
-#pragma unroll
-    for (int m = 0; m < NC; ++m) Vs[m][t + RV] = Vq[QOFF - 1][m];
-    if (t < 2 * RV) {
-      const int hc = t < RV ? i0 - 1 - RV + t : i0 + TX + 1 + (t - RV);
-      double raw[NC], v[NC];
-      load_raw<RC, NC>(a.qin, g.cs, gidx(g, clampi(hc, -H, g.nx + H - 1), j, 0), raw);
-      to_recon_vars<RC, NC>(raw, v, P.gamma);
-      const int slot = t < RV ? t : NT + RV + (t - RV);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Vs[m][slot] = v[m];
-    }
-    __syncthreads();
-    // i-reconstruction at the owned vertex
-    {
-      double W[2 * RV + 1][NC];
-#pragma unroll
-      for (int s = 0; s < 2 * RV + 1; ++s)
-#pragma unroll
-        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m][t + s];
-      double L[NC], R[NC];
-      RC::template vertex<NC>(W, P, L, R);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) { Lx[m][t] = L[m]; Rx[m][t] = R[m]; }
-    }
-    __syncthreads();
-    // i-faces f = i0 + t, t = 0..62: F(L(f-1), R(f))
-    if (t <= TX) {
-      double L[NC], R[NC], F[NC];
-#pragma unroll
-      for (int m = 0; m < NC; ++m) { L[m] = Lx[m][t]; R[m] = Rx[m][t + 1]; }
-      face_flux<FLUX, NC, 0>(L, R, P, F);
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Fx[m][t] = F[m];
-    }
-    // j-face j + 1 of the owned column
-    double Fn[NC];
+    loadV(j0 + RV, Vq[NW - 1]);
     RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-    face_flux<FLUX, NC, 1>(Lc, Rt, P, Fn);
-    __syncthreads();
-    if (owner && j < g.ny) {
-      const int64_t ci = gidx(g, c, j, 0);
-      double out[NC];
+    face_flux<FLUX, NC, 1>(Lc, Rt, P, Fc);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+
+    double nrm[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
+
+    for (int j = j0; j < j1; ++j) {
+      // shift: centre becomes j + 1; Vq[QOFF - 1] is V(j)
+#pragma unroll
+      for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+      loadV(j + 1 + RV, Vq[NW - 1]);
+      // row j into LDS: owned columns plus RV far-halo columns on each side
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Vs[m][t + RV] = Vq[QOFF - 1][m];
+      if (t < 2 * RV) {
+        const int hc = t < RV ? i0 - 1 - RV + t : i0 + TX + 1 + (t - RV);
+        double raw[NC], v[NC];
+        load_raw<RC, NC>(qin, g.cs, gidx(g, clampi(hc, -H, g.nx + H - 1), j, 0), raw);
+        to_recon_vars<RC, NC>(raw, v, P.gamma);
+        const int slot = t < RV ? t : NT + RV + (t - RV);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vs[m][slot] = v[m];
+      }
+      __syncthreads();
+      // i-reconstruction at the owned vertex
+      {
+        double W[2 * RV + 1][NC];
+#pragma unroll
+        for (int s = 0; s < 2 * RV + 1; ++s)
+#pragma unroll
+          for (int m = 0; m < NC; ++m) W[s][m] = Vs[m][t + s];
+        double L[NC], R[NC];
+        RC::template vertex<NC>(W, P, L, R);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) { Lx[m][t] = L[m]; Rx[m][t] = R[m]; }
+      }
+      __syncthreads();
+      // i-faces f = i0 + t, t = 0..62: F(L(f-1), R(f))
+      if (t <= TX) {
+        double L[NC], R[NC], F[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) { L[m] = Lx[m][t]; R[m] = Rx[m][t + 1]; }
+        face_flux<FLUX, NC, 0>(L, R, P, F);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fx[m][t] = F[m];
+      }
+      // j-face j + 1 of the owned column
+      double Fn[NC];
+      RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+      face_flux<FLUX, NC, 1>(Lc, Rt, P, Fn);
+      __syncthreads();
+      if (owner && j < g.ny) {
+        const int64_t ci = gidx(g, c, j, 0);
+        double out[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          double r;
+          if constexpr (kFast) {
+            r = -(Fx[m][t] - Fx[m][t - 1]) * g.rdx;
+            r = r - (Fn[m] - Fc[m]) * g.rdy;
+          } else {
+            r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
+            r = r - (Fn[m] - Fc[m]) / g.dy;
+          }
+          const double q0 = q0p[ci + m * g.cs];
+          double* accp = acc ? acc + ci + m * g.cs : nullptr;
+          out[m] = rk_apply(a.stage, kc, q0, qin[ci + m * g.cs], accp, r);
+          nrm[m] += r * r;
+        }
+        store_with_ghosts<NC, 2>(g, a.blocks, B, a.out_buf, qout, c, j, 0, out);
+        if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
+      __syncthreads();
+    }
+
+    if (a.norm_part) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        double r;
-        if constexpr (kFast) {
-          r = -(Fx[m][t] - Fx[m][t - 1]) * g.rdx;
-          r = r - (Fn[m] - Fc[m]) * g.rdy;
-        } else {
-          r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
-          r = r - (Fn[m] - Fc[m]) / g.dy;
-        }
-        const double q0 = a.q0[ci + m * g.cs];
-        double* accp = a.acc ? a.acc + ci + m * g.cs : nullptr;
-        out[m] = rk_apply(a.stage, kc, q0, a.qin[ci + m * g.cs], accp, r);
-        nrm[m] += r * r;
+        const double s = wg_sum<NT>(nrm[m], red);
+        if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
       }
-      store_with_ghosts<NC, 2>(g, a.qout, c, j, 0, out);
-      if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
-    }
-#pragma unroll
-    for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
-    __syncthreads();
-  }
-
-  if (a.norm_part) {
-#pragma unroll
-    for (int m = 0; m < NC; ++m) {
-      const double s = wg_sum<NT>(nrm[m], red);
-      if (t == 0) a.norm_part[(int64_t)blockIdx.x * NC + m] = s;
     }
   }
   if (a.smax) {
@@ -381,7 +436,9 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 
 // ============================================================================
 // 3-D stage: tile of TX (32 or 64) x TY columns, NT = TX * TY threads, every
-// thread owns the column (i0 + tx, j0 + ty) and marches along k.
+// thread owns the column (i0 + tx, j0 + ty) and marches along k.  A
+// workgroup loops over tiles of the launch's tile list (all blocks of the
+// context in one launch).
 //
 // Per plane k:
 //   A  convert the prefetched raw values: plane k (owned vertex + i/j halo)
@@ -421,230 +478,240 @@ void stage3d_kernel(StageArgs a) {
   double* Fx = Vs;                 // [NC][FXN] after phase B
   double* Fy = Vs + NC * FXN;      // [NC][FYN]
 
-  const BlockGeom& g = a.g;
   const Params& P = a.P;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int ti = bid % a.tiles_x;
-  const int tj = (bid / a.tiles_x) % a.tiles_y;
-  const int tk = bid / (a.tiles_x * a.tiles_y);
-  const int i0 = ti * TX, j0 = tj * TY;
-  const int k0 = tk * a.chunk, k1 = min(k0 + a.chunk, g.nz);
-  const int c = i0 + tx, r = j0 + ty;
-  const bool owner = c < g.nx && r < g.ny;
-  const int ci_ = clampi(c, -H, g.nx + H - 1), rj_ = clampi(r, -H, g.ny + H - 1);
   const double dt = a.dt_dev ? *a.dt_dev : a.dt;
   const RKCoef kc = rk_coef(a.stage, dt);
-
-  auto gaddr = [&](int i, int j, int k) {
-    return gidx(g, clampi(i, -H, g.nx + H - 1), clampi(j, -H, g.ny + H - 1),
-                clampi(k, -H, g.nz + H - 1));
-  };
-  // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
-  auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
-    if (e < NXH) {
-      const int row = e / (2 * H), s = e % (2 * H);
-      hi = s < H ? i0 - H + s : i0 + TX + (s - H);
-      hj = j0 + row;
-      slot = (row + H) * VW + (s < H ? s : TX + H + (s - H));
-    } else {
-      const int e2 = e - NXH;
-      const int col = e2 % TX, s = e2 / TX;
-      hi = i0 + col;
-      hj = s < H ? j0 - H + s : j0 + TY + (s - H);
-      slot = (s < H ? s : TY + H + (s - H)) * VW + col + H;
-    }
-  };
-
-  // ---- k-queue warm-up: vertices k0-1 and k0, face k0
-  double Vq[NW][NC];
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    double raw[NC];
-    load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 - 1 - QOFF + w), raw);
-    to_recon_vars<RC, NC>(raw, Vq[w], P.gamma);
-  }
-  double Lc[NC], Fc[NC], Lt[NC], Rt[NC];
-  RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
-  {
-    double raw[NC];
-#pragma unroll
-    for (int w = 0; w < NW - 1; ++w)
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-    load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 + RV), raw);
-    to_recon_vars<RC, NC>(raw, Vq[NW - 1], P.gamma);
-  }
-  RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-  face_flux<FLUX, NC, 2>(Lc, Rt, P, Fc);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
-
-  // ---- prefetch for plane k0
-  double oraw[NC];
-  load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k0 + 1 + RV), oraw);
-  double hraw[NHL][NC];
-  int hslot[NHL];
-#pragma unroll
-  for (int h = 0; h < NHL; ++h) {
-    const int e = t + h * NT;
-    hslot[h] = -1;
-    if (e < NHALO) {
-      int hi, hj;
-      halo_pos(e, hi, hj, hslot[h]);
-      load_raw<RC, NC>(a.qin, g.cs, gaddr(hi, hj, k0), hraw[h]);
-    }
-  }
-
-  double nrm[NC];
-#pragma unroll
-  for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
   double smax = 0.0;
 
-  for (int k = k0; k < k1; ++k) {
-    // ---- A: queue shift (centre k+1), plane k into LDS, next loads
+  for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
+    const int tile = a.tile0 + xcd_remap(it, a.ntiles);
+    const int bi = block_of_tile(a.blocks, a.nblocks, tile);
+    const BlockDesc& B = a.blocks[bi];
+    const BlockGeom g = B.g;
+    const double* __restrict__ qin = B.q[a.in_buf];
+    const double* __restrict__ q0p = B.q[a.q0_buf];
+    double* __restrict__ qout = B.q[a.out_buf];
+    double* __restrict__ acc = a.use_acc ? B.q[3] : nullptr;
+    const int lt = tile - B.tile_base;
+    const int ti = lt % B.tiles_x;
+    const int tj = (lt / B.tiles_x) % B.tiles_y;
+    const int tk = lt / (B.tiles_x * B.tiles_y);
+    const int i0 = ti * TX, j0 = tj * TY;
+    const int k0 = tk * B.chunk, k1 = min(k0 + B.chunk, g.nz);
+    const int c = i0 + tx, r = j0 + ty;
+    const bool owner = c < g.nx && r < g.ny;
+    const int ci_ = clampi(c, -H, g.nx + H - 1), rj_ = clampi(r, -H, g.ny + H - 1);
+
+    auto gaddr = [&](int i, int j, int k) {
+      return gidx(g, clampi(i, -H, g.nx + H - 1), clampi(j, -H, g.ny + H - 1),
+                  clampi(k, -H, g.nz + H - 1));
+    };
+    // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
+    auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
+      if (e < NXH) {
+        const int row = e / (2 * H), s = e % (2 * H);
+        hi = s < H ? i0 - H + s : i0 + TX + (s - H);
+        hj = j0 + row;
+        slot = (row + H) * VW + (s < H ? s : TX + H + (s - H));
+      } else {
+        const int e2 = e - NXH;
+        const int col = e2 % TX, s = e2 / TX;
+        hi = i0 + col;
+        hj = s < H ? j0 - H + s : j0 + TY + (s - H);
+        slot = (s < H ? s : TY + H + (s - H)) * VW + col + H;
+      }
+    };
+
+    // ---- k-queue warm-up: vertices k0-1 and k0, face k0
+    double Vq[NW][NC];
 #pragma unroll
-    for (int w = 0; w < NW - 1; ++w)
+    for (int w = 0; w < NW; ++w) {
+      double raw[NC];
+      load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 - 1 - QOFF + w), raw);
+      to_recon_vars<RC, NC>(raw, Vq[w], P.gamma);
+    }
+    double Lc[NC], Fc[NC], Lt[NC], Rt[NC];
+    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
 #pragma unroll
-      for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
-    to_recon_vars<RC, NC>(oraw, Vq[NW - 1], P.gamma);
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+    {
+      double raw[NC];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) Vs[m * VSN + (ty + H) * VW + tx + H] = Vq[QOFF - 1][m];
+      for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+      load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 + RV), raw);
+      to_recon_vars<RC, NC>(raw, Vq[NW - 1], P.gamma);
+    }
+    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+    face_flux<FLUX, NC, 2>(Lc, Rt, P, Fc);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+
+    // ---- prefetch for plane k0
+    double oraw[NC];
+    load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 + 1 + RV), oraw);
+    double hraw[NHL][NC];
+    int hslot[NHL];
 #pragma unroll
     for (int h = 0; h < NHL; ++h) {
-      if (hslot[h] >= 0) {
-        double v[NC];
-        to_recon_vars<RC, NC>(hraw[h], v, P.gamma);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) Vs[m * VSN + hslot[h]] = v[m];
+      const int e = t + h * NT;
+      hslot[h] = -1;
+      if (e < NHALO) {
+        int hi, hj;
+        halo_pos(e, hi, hj, hslot[h]);
+        load_raw<RC, NC>(qin, g.cs, gaddr(hi, hj, k0), hraw[h]);
       }
     }
-    if (k + 1 < k1) {
-      load_raw<RC, NC>(a.qin, g.cs, gaddr(ci_, rj_, k + 2 + RV), oraw);
+
+    double nrm[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
+
+    for (int k = k0; k < k1; ++k) {
+      // ---- A: queue shift (centre k+1), plane k into LDS, next loads
+#pragma unroll
+      for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+      to_recon_vars<RC, NC>(oraw, Vq[NW - 1], P.gamma);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Vs[m * VSN + (ty + H) * VW + tx + H] = Vq[QOFF - 1][m];
 #pragma unroll
       for (int h = 0; h < NHL; ++h) {
-        const int e = t + h * NT;
-        if (e < NHALO) {
-          int hi, hj, sl;
-          halo_pos(e, hi, hj, sl);
-          load_raw<RC, NC>(a.qin, g.cs, gaddr(hi, hj, k + 1), hraw[h]);
+        if (hslot[h] >= 0) {
+          double v[NC];
+          to_recon_vars<RC, NC>(hraw[h], v, P.gamma);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Vs[m * VSN + hslot[h]] = v[m];
         }
       }
-    }
-    __syncthreads();
+      if (k + 1 < k1) {
+        load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k + 2 + RV), oraw);
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) {
+          const int e = t + h * NT;
+          if (e < NHALO) {
+            int hi, hj, sl;
+            halo_pos(e, hi, hj, sl);
+            load_raw<RC, NC>(qin, g.cs, gaddr(hi, hj, k + 1), hraw[h]);
+          }
+        }
+      }
+      __syncthreads();
 
-    // ---- B: vertex reconstructions
-    for (int e = t; e < RT; e += NT) {
-      int src, stride, dst;
-      bool xdir;
-      if (e < NT) {                       // own vertex, i-direction
-        const int x = e % TX, y = e / TX;
-        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
-      } else if (e < 2 * NT) {            // own vertex, j-direction
-        const int e1 = e - NT, x = e1 % TX, y = e1 / TX;
-        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
-      } else if (e < 2 * NT + 2 * TY) {   // i-direction at columns i0-1 and i0+TX
-        const int e2 = e - 2 * NT, y = e2 >> 1, side = e2 & 1;
-        const int x = side ? TX : -1;
-        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
-      } else {                            // j-direction at rows j0-1 and j0+TY
-        const int e3 = e - 2 * NT - 2 * TY, x = e3 % TX, side = e3 / TX;
-        const int y = side ? TY : -1;
-        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
-      }
-      double W[2 * RV + 1][NC], L[NC], R[NC];
-#pragma unroll
-      for (int s = 0; s < 2 * RV + 1; ++s)
-#pragma unroll
-        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + src + (s - RV) * stride];
-      RC::template vertex<NC>(W, P, L, R);
-      if (xdir) {
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          LRx[0][m * LRX + dst] = L[m];
-          if constexpr (NLR == 2) LRx[NLR - 1][m * LRX + dst] = R[m];
+      // ---- B: vertex reconstructions
+      for (int e = t; e < RT; e += NT) {
+        int src, stride, dst;
+        bool xdir;
+        if (e < NT) {                       // own vertex, i-direction
+          const int x = e % TX, y = e / TX;
+          src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+        } else if (e < 2 * NT) {            // own vertex, j-direction
+          const int e1 = e - NT, x = e1 % TX, y = e1 / TX;
+          src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
+        } else if (e < 2 * NT + 2 * TY) {   // i-direction at columns i0-1 and i0+TX
+          const int e2 = e - 2 * NT, y = e2 >> 1, side = e2 & 1;
+          const int x = side ? TX : -1;
+          src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+        } else {                            // j-direction at rows j0-1 and j0+TY
+          const int e3 = e - 2 * NT - 2 * TY, x = e3 % TX, side = e3 / TX;
+          const int y = side ? TY : -1;
+          src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
         }
-      } else {
+        double W[2 * RV + 1][NC], L[NC], R[NC];
 #pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          LRy[0][m * LRY + dst] = L[m];
-          if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + dst] = R[m];
+        for (int s = 0; s < 2 * RV + 1; ++s)
+#pragma unroll
+          for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + src + (s - RV) * stride];
+        RC::template vertex<NC>(W, P, L, R);
+        if (xdir) {
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            LRx[0][m * LRX + dst] = L[m];
+            if constexpr (NLR == 2) LRx[NLR - 1][m * LRX + dst] = R[m];
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            LRy[0][m * LRY + dst] = L[m];
+            if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + dst] = R[m];
+          }
         }
       }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // ---- C: face fluxes (into the plane buffer, dead after B)
-    for (int e = t; e < FT; e += NT) {
-      double L[NC], R[NC], F[NC];
-      if (e < NT || (e >= 2 * NT && e < 2 * NT + TY)) {   // i-face s of row y
-        int y, s;
-        if (e < NT) { s = e % TX; y = e / TX; } else { s = TX; y = e - 2 * NT; }
+      // ---- C: face fluxes (into the plane buffer, dead after B)
+      for (int e = t; e < FT; e += NT) {
+        double L[NC], R[NC], F[NC];
+        if (e < NT || (e >= 2 * NT && e < 2 * NT + TY)) {   // i-face s of row y
+          int y, s;
+          if (e < NT) { s = e % TX; y = e / TX; } else { s = TX; y = e - 2 * NT; }
 #pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          L[m] = LRx[0][m * LRX + y * XRW + s];
-          R[m] = LRx[NLR - 1][m * LRX + y * XRW + s + 1];
+          for (int m = 0; m < NC; ++m) {
+            L[m] = LRx[0][m * LRX + y * XRW + s];
+            R[m] = LRx[NLR - 1][m * LRX + y * XRW + s + 1];
+          }
+          face_flux<FLUX, NC, 0>(L, R, P, F);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Fx[m * FXN + y * FXW + s] = F[m];
+        } else {                                            // j-face s of column x
+          int x, s;
+          if (e < 2 * NT) { const int e1 = e - NT; x = e1 % TX; s = e1 / TX; }
+          else { x = e - 2 * NT - TY; s = TY; }
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            L[m] = LRy[0][m * LRY + s * TX + x];
+            R[m] = LRy[NLR - 1][m * LRY + (s + 1) * TX + x];
+          }
+          face_flux<FLUX, NC, 1>(L, R, P, F);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Fy[m * FYN + s * TX + x] = F[m];
         }
-        face_flux<FLUX, NC, 0>(L, R, P, F);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) Fx[m * FXN + y * FXW + s] = F[m];
-      } else {                                            // j-face s of column x
-        int x, s;
-        if (e < 2 * NT) { const int e1 = e - NT; x = e1 % TX; s = e1 / TX; }
-        else { x = e - 2 * NT - TY; s = TY; }
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          L[m] = LRy[0][m * LRY + s * TX + x];
-          R[m] = LRy[NLR - 1][m * LRY + (s + 1) * TX + x];
-        }
-        face_flux<FLUX, NC, 1>(L, R, P, F);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) Fy[m * FYN + s * TX + x] = F[m];
       }
-    }
-    // k-face k+1 of the owned column
-    double Fn[NC];
-    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-    face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
-    __syncthreads();
+      // k-face k+1 of the owned column
+      double Fn[NC];
+      RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+      face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
+      __syncthreads();
 
-    // ---- D: update
-    if (owner) {
-      const int64_t cidx = gidx(g, c, r, k);
-      double out[NC];
+      // ---- D: update
+      if (owner) {
+        const int64_t cidx = gidx(g, c, r, k);
+        double out[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
+          const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
+          double rr;
+          if constexpr (kFast) {
+            rr = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
+          } else {
+            rr = 0.0 - fxd / g.dx;
+            rr = rr - fyd / g.dy;
+            rr = rr - (Fn[m] - Fc[m]) / g.dz;
+          }
+          const double q0 = q0p[cidx + m * g.cs];
+          double* accp = acc ? acc + cidx + m * g.cs : nullptr;
+          out[m] = rk_apply(a.stage, kc, q0, qin[cidx + m * g.cs], accp, rr);
+          nrm[m] += rr * rr;
+        }
+        store_with_ghosts<NC, 3>(g, a.blocks, B, a.out_buf, qout, c, r, k, out);
+        if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
+      __syncthreads();
+    }
+
+    if (a.norm_part) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
-        const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
-        double rr;
-        if constexpr (kFast) {
-          rr = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
-        } else {
-          rr = 0.0 - fxd / g.dx;
-          rr = rr - fyd / g.dy;
-          rr = rr - (Fn[m] - Fc[m]) / g.dz;
-        }
-        const double q0 = a.q0[cidx + m * g.cs];
-        double* accp = a.acc ? a.acc + cidx + m * g.cs : nullptr;
-        out[m] = rk_apply(a.stage, kc, q0, a.qin[cidx + m * g.cs], accp, rr);
-        nrm[m] += rr * rr;
+        const double s = wg_sum<NT>(nrm[m], red);
+        if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
       }
-      store_with_ghosts<NC, 3>(g, a.qout, c, r, k, out);
-      if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
-    }
-#pragma unroll
-    for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
-    __syncthreads();
-  }
-
-  if (a.norm_part) {
-#pragma unroll
-    for (int m = 0; m < NC; ++m) {
-      const double s = wg_sum<NT>(nrm[m], red);
-      if (t == 0) a.norm_part[(int64_t)blockIdx.x * NC + m] = s;
     }
   }
   if (a.smax) {

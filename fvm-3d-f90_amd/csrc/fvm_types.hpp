@@ -32,7 +32,21 @@ struct BlockGeom {
   int ng;                // ghost layers written per side
 };
 
-enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1 };
+// Face kinds.  BC_CONN: the face abuts another block; its ghost layers are
+// written by that block's stage kernel.  `shared` faces hold the interface
+// vertex in both blocks (a grid split through a vertex plane); abutting faces
+// join vertex n-1 of one block to vertex 0 of the next one spacing apart.
+enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1, BC_CONN = 2 };
+
+// One block as the stage kernels see it.
+struct BlockDesc {
+  double* q[4];          // state arrays: 0 = u^n, 1, 2 = stage buffers, 3 = RK4 accumulator
+  BlockGeom g;
+  int nbr[6];            // neighbour block per face (BC_CONN), else -1
+  int nbr_shared[6];     // 1: shared interface vertex, 0: abutting
+  int tiles_x, tiles_y, tiles_z, chunk;
+  int tile_base;         // first tile of this block in the launch's tile list
+};
 
 // RK stage kinds: the combine each launch applies
 enum : int {
@@ -46,18 +60,17 @@ enum : int {
 };
 
 struct StageArgs {
-  const double* qin;     // stencil input, ghosts valid
-  const double* q0;      // u^n, read at the cell only (may alias qout)
-  double* qout;          // stage output (interior + ghost scatter)
-  double* acc;           // RK4 accumulator (cell only)
-  const double* dt_dev;  // if non-null, dt is read here (CFL mode)
+  const BlockDesc* blocks;        // descriptor table (device memory)
+  int nblocks;                    // blocks in the table
+  int tile0, ntiles;              // tiles [tile0, tile0 + ntiles) of the table's tile list;
+                                  // a workgroup loops over them
+  int in_buf, q0_buf, out_buf;    // state array index (BlockDesc::q) per role
+  int use_acc;                    // RK4: accumulator q[3]
+  const double* dt_dev;           // if non-null, dt is read here (CFL mode)
   double dt;
-  BlockGeom g;
   Params P;
   int stage;
-  int tiles_x, tiles_y, tiles_z;  // tile grid; blockIdx.x is remapped over it
-  int chunk;                      // planes (3-D) or rows (2-D) per workgroup
-  double* norm_part;              // per-workgroup sums of rhs^2 (NC each), or null
+  double* norm_part;              // per-tile sums of rhs^2 (NC each), or null
   unsigned long long* smax;       // CFL wave-speed max of the output, or null
 };
 

@@ -24,6 +24,7 @@ LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
 NUMERICS_EXACT, NUMERICS_FAST = 0, 1
+LAUNCH_PER_BLOCK, LAUNCH_BATCHED, LAUNCH_PERSISTENT = 0, 1, 2
 
 # scheme names as the case file spells them (type_defs_scheme.f90:41-84)
 SCHEME_INDEX = {
@@ -38,7 +39,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_scheme_set", "fvm_block_create", "fvm_block_destroy", "fvm_block_set_bc",
            "fvm_upload", "fvm_download", "fvm_step", "fvm_norms", "fvm_sync",
            "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
-           "fvm_block_launch_shape")
+           "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode")
 
 
 class FvmError(RuntimeError):
@@ -76,6 +77,8 @@ def load_library(path=LIB_PATH):
         "fvm_timing_enable": ([vp, ci], ci),
         "fvm_set_numerics": ([vp, ci], ci),
         "fvm_autotune": ([vp, ci], ci),
+        "fvm_block_connect": ([vp, ci, vp, ci], ci),
+        "fvm_set_launch_mode": ([vp, ci], ci),
         "fvm_block_launch_shape": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                     ctypes.POINTER(ci)], ci),
         "fvm_timing_read": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
@@ -107,6 +110,10 @@ class Block:
 
     def set_bc(self, face, kind):
         _check(_lib.fvm_block_set_bc(self._h, face, kind))
+
+    def connect(self, face, other, shared=True):
+        """Face `face` of this block abuts the opposite face of `other`."""
+        _check(_lib.fvm_block_connect(self._h, face, other._h, int(bool(shared))))
 
     def set_periodic(self, dims=(0, 1, 2)):
         for d in dims:
@@ -178,6 +185,10 @@ class Context:
         b = Block(self, shape, spacing)
         self.blocks.append(b)
         return b
+
+    def launch_mode(self, mode):
+        """LAUNCH_BATCHED (default), LAUNCH_PERSISTENT or LAUNCH_PER_BLOCK."""
+        _check(_lib.fvm_set_launch_mode(self._h, mode))
 
     def autotune(self, reps=3):
         """Pick each block's tile shape and k-chunk by timing stage launches."""

@@ -107,8 +107,33 @@ int fvm_block_create(fvm_ctx *ctx, int64_t ni, int64_t nj, int64_t nk,
 int fvm_block_destroy(fvm_block *blk);
 
 /* Boundary kind of one block face: face 0..5 = i-, i+, j-, j+, k-, k+.
- * Default FVM_BC_EXTRAP (src/base/defs_block_data.f90:226-256). */
+ * Default FVM_BC_EXTRAP (src/base/defs_block_data.f90:226-256).  Periodic
+ * applies to both faces of the direction. */
 int fvm_block_set_bc(fvm_block *blk, int face, int kind);
+
+/* Block-to-block connectivity (the reference builds the metadata,
+ * ClassGeomBlock%connect src/base/defs_block_geom.f90:953-990 and donor ranges
+ * src/base/defs_face_geom.f90:43-63,361-403, but never exchanges data; here
+ * the ghost layers across the face are filled on the device, inside the stage
+ * kernel).  Face `face` of block a abuts the opposite face of block b; the two
+ * faces must conform (equal extents in the other directions, same
+ * orientation).  shared = 1: both blocks hold the interface vertex plane (a
+ * grid cut through a vertex plane, defs_block_geom.f90:610-611); shared = 0:
+ * the blocks are one spacing apart (e.g. a periodic wrap across blocks).
+ * Both blocks must belong to the same context. */
+int fvm_block_connect(fvm_block *a, int face, fvm_block *b, int shared);
+
+/* How a stage is launched over the blocks of a context (no reference
+ * counterpart):
+ *   FVM_LAUNCH_BATCHED (default): one launch per stage over the tile list of
+ *     all blocks (blocks of uneven size and independent cases alike);
+ *   FVM_LAUNCH_PERSISTENT: one launch of at most two workgroups per CU that
+ *     loop over the tile list;
+ *   FVM_LAUNCH_PER_BLOCK: one launch per block. */
+#define FVM_LAUNCH_PER_BLOCK  0
+#define FVM_LAUNCH_BATCHED    1
+#define FVM_LAUNCH_PERSISTENT 2
+int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 
 /* Host <-> device copies of the conserved state (host layout above).  Upload
  * also fills the device ghost layers (ClassFlowField%initial's ghost update,
