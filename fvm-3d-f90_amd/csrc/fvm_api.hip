@@ -9,8 +9,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fvm_gpu.h"
@@ -58,6 +62,9 @@ struct fvm_block {
   // launch shape
   int tile = 0, chunk = 16;
   int tiles_x = 0, tiles_y = 0, tiles_z = 0, ntiles = 0, tile_base = 0;
+  // independent solve (serving): own dt and CFL accumulator on the device
+  const double* dt_dev = nullptr;
+  unsigned long long* smax_dev = nullptr;
 };
 
 struct fvm_ctx {
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = q[ci + c * g.cs];
     const double w = wave_speed<NC>(v, g, gamma);
-    s = (w > s || w != w) ? w : s;
+    s = nan_max(s, w);
   }
   s = wg_max<256>(s, red);
   if (threadIdx.x == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(s));
@@ -172,6 +179,30 @@ __global__ void cfl_dt_kernel(unsigned long long* smax, double* dt, double cfl) 
   const double s = __longlong_as_double((long long)*smax);
   *dt = cfl / s;
   *smax = 0ull;
+}
+
+// Per-solve time step of the serving loop: dt = min(cfl / smax, t_end - t),
+// t += dt, and the CFL accumulator cleared for the step's last stage.
+struct SolveSlot {
+  double t, t_end, cfl, dt;
+  unsigned long long smax;
+  int steps;
+  int nan;
+};
+
+__global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SolveSlot& s = slots[ids[i]];
+  const double sm = __longlong_as_double((long long)s.smax);
+  double dt = s.cfl / sm;
+  const double rest = s.t_end - s.t;
+  if (!(dt > 0.0) || !(dt < 1.0e300)) { s.nan = 1; dt = 0.0; }
+  if (dt > rest) dt = rest;
+  s.dt = dt;
+  s.t = s.t + dt;
+  s.smax = 0ull;
+  s.steps = s.steps + 1;
 }
 
 }  // namespace
@@ -250,6 +281,8 @@ int sync_desc(fvm_ctx* ctx) {
     for (int f = 0; f < 6; ++f) { d.nbr[f] = b->nbr[f]; d.nbr_shared[f] = b->nbr_shared[f]; }
     d.tiles_x = b->tiles_x; d.tiles_y = b->tiles_y; d.tiles_z = b->tiles_z; d.chunk = b->chunk;
     d.tile_base = base;
+    d.dt_dev = b->dt_dev;
+    d.smax = b->smax_dev;
     b->tile_base = base;
     base += b->ntiles;
   }
@@ -885,6 +918,294 @@ int fvm_sync(fvm_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (int rc = set_device(ctx)) return rc;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Serving loop: continuous batching of independent solves
+// ============================================================================
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+  int64_t id = 0;
+  fvm_request req{};
+  std::vector<double> q0;          // initial state (host layout of fvm_upload, host_ng 0)
+  std::vector<double> result;      // final state once done
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  int steps = 0;
+  double t_final = 0;
+  double cell_updates = 0;
+  int status = 0;                  // 0 pending, 1 active, 2 done, -1 failed
+  int slot = -1;
+  fvm_ctx* group = nullptr;
+  fvm_block* blk = nullptr;
+  size_t bytes = 0;
+};
+
+// Kernel instantiation and scheme parameters: solves that agree on all of
+// them share one launch per stage.
+using GroupKey = std::tuple<int, int, int, int, int, int, double, double, double, int>;
+
+}  // namespace
+
+struct fvm_server {
+  int device = 0;
+  double budget = 0;        // HBM bytes solves may hold
+  double used = 0;
+  int max_active = 0;
+  Clock::time_point t0;
+  int64_t next_id = 1;
+  std::deque<Job*> pending;
+  std::vector<Job*> active;
+  std::map<int64_t, Job*> jobs;
+  std::map<GroupKey, fvm_ctx*> groups;
+  SolveSlot* d_slots = nullptr;
+  std::vector<SolveSlot> h_slots;
+  std::vector<int> free_slots;
+  int* d_ids = nullptr;     // slot ids of a group's active solves
+  double total_cell_updates = 0;
+};
+
+namespace {
+
+double now_s(fvm_server* s) {
+  return std::chrono::duration<double>(Clock::now() - s->t0).count();
+}
+
+int server_group(fvm_server* s, const fvm_request& r, fvm_ctx** out) {
+  const int dim = r.n[2] == 1 ? 2 : 3;
+  GroupKey key{dim, r.recon, r.limiter, r.flux, r.rk, r.numerics, r.kappa, r.entropy_fix, r.gamma,
+               r.ref_compat};
+  auto it = s->groups.find(key);
+  if (it != s->groups.end()) { *out = it->second; return FVM_OK; }
+  fvm_ctx* ctx = nullptr;
+  if (int rc = fvm_ctx_create(s->device, &ctx)) return rc;
+  if (int rc = fvm_scheme_set(ctx, r.recon, r.limiter, r.flux, r.rk, r.kappa, r.entropy_fix, r.gamma,
+                              r.ref_compat)) { fvm_ctx_destroy(ctx); return rc; }
+  if (int rc = fvm_set_numerics(ctx, r.numerics)) { fvm_ctx_destroy(ctx); return rc; }
+  s->groups[key] = ctx;
+  *out = ctx;
+  return FVM_OK;
+}
+
+// Admit pending jobs in arrival order while a slot and the HBM budget allow.
+int server_admit(fvm_server* s) {
+  while (!s->pending.empty()) {
+    Job* j = s->pending.front();
+    if (s->free_slots.empty() || s->used + (double)j->bytes > s->budget) break;
+    s->pending.pop_front();
+    fvm_ctx* g = nullptr;
+    if (int rc = server_group(s, j->req, &g)) return rc;
+    fvm_block* b = nullptr;
+    const fvm_request& r = j->req;
+    if (int rc = fvm_block_create(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2], &b))
+      return rc;
+    // one tile shape per group so its blocks share a launch
+    if (b->dim == 3) plan_launch(b, 1, -1);
+    for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
+      if (r.bc[f] == FVM_BC_PERIODIC)
+        if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
+    if (int rc = fvm_upload(b, j->q0.data(), 0)) return rc;
+    j->q0.clear();
+    j->q0.shrink_to_fit();
+    const int slot = s->free_slots.back();
+    s->free_slots.pop_back();
+    SolveSlot init{};
+    init.t = 0.0; init.t_end = r.t_end; init.cfl = r.cfl; init.dt = 0.0; init.smax = 0ull;
+    init.steps = 0; init.nan = 0;
+    HIP_TRY(hipMemcpyAsync(s->d_slots + slot, &init, sizeof(SolveSlot), hipMemcpyHostToDevice, g->stream));
+    b->dt_dev = &s->d_slots[slot].dt;
+    b->smax_dev = &s->d_slots[slot].smax;
+    g->desc_dirty = true;
+    if (int rc = sync_desc(g)) return rc;
+    if (int rc = fill_ghosts_all(g, 0)) return rc;
+    g->ghosts_dirty = false;
+    if (b->nc == 5)
+      hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
+    else
+      hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
+    HIP_TRY(hipGetLastError());
+    j->slot = slot;
+    j->group = g;
+    j->blk = b;
+    j->status = 1;
+    j->t_admit = now_s(s);
+    s->used += (double)j->bytes;
+    s->active.push_back(j);
+  }
+  return FVM_OK;
+}
+
+// One time step of every active solve: per group, the per-solve dt update
+// and one batched launch per RK stage; groups run on their own streams.
+int server_step(fvm_server* s) {
+  std::map<fvm_ctx*, std::vector<int>> ids;
+  for (Job* j : s->active) ids[j->group].push_back(j->slot);
+  std::vector<int> all_ids;
+  std::vector<std::pair<fvm_ctx*, int>> offs;
+  for (auto& kv : ids) {
+    offs.push_back({kv.first, (int)all_ids.size()});
+    all_ids.insert(all_ids.end(), kv.second.begin(), kv.second.end());
+  }
+  if (all_ids.empty()) return FVM_OK;
+  HIP_TRY(hipMemcpy(s->d_ids, all_ids.data(), all_ids.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (size_t gi = 0; gi < offs.size(); ++gi) {
+    fvm_ctx* g = offs[gi].first;
+    const int n = (int)ids[g].size();
+    if (int rc = sync_desc(g)) return rc;
+    hipLaunchKernelGGL(solve_dt_kernel, dim3((n + 63) / 64), dim3(64), 0, g->stream, s->d_slots,
+                       s->d_ids + offs[gi].second, n);
+    HIP_TRY(hipGetLastError());
+    const int nstage = g->rk == RK4 ? 4 : 3;
+    for (int st = 1; st <= nstage; ++st) {
+      StageArgs a{};
+      a.P = g->P;
+      a.dt = 0.0;
+      a.dt_dev = nullptr;
+      a.q0_buf = 0;
+      a.use_acc = g->rk == RK4 ? 1 : 0;
+      if (g->rk == RK4) {
+        const int ib[4] = {0, 1, 2, 1}, ob[4] = {1, 2, 1, 0};
+        const int sk[4] = {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4};
+        a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
+      } else {
+        const int ib[3] = {0, 1, 2}, ob[3] = {1, 2, 0};
+        const int sk[3] = {ST_SSP1, ST_SSP2, ST_SSP3};
+        a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
+      }
+      a.norm_part = nullptr;
+      a.smax = nullptr;
+      a.block_smax = st == nstage ? 1 : 0;
+      if (int rc = launch_stage_all(g, a)) return rc;
+    }
+  }
+  for (auto& kv : offs) HIP_TRY(hipStreamSynchronize(kv.first->stream));
+  return FVM_OK;
+}
+
+// Retire finished solves: download the state, release memory and slot.
+int server_retire(fvm_server* s) {
+  HIP_TRY(hipMemcpy(s->h_slots.data(), s->d_slots, s->h_slots.size() * sizeof(SolveSlot),
+                    hipMemcpyDeviceToHost));
+  std::vector<Job*> keep;
+  for (Job* j : s->active) {
+    const SolveSlot& sl = s->h_slots[j->slot];
+    const bool done = sl.t >= j->req.t_end * (1.0 - 1e-14) || sl.nan;
+    if (!done) { keep.push_back(j); continue; }
+    const fvm_request& r = j->req;
+    const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
+    const int nstage = r.rk == FVM_RK4 ? 4 : 3;
+    j->steps = sl.steps;
+    j->t_final = sl.t;
+    j->cell_updates = (double)ncell * nstage * sl.steps;
+    s->total_cell_updates += j->cell_updates;
+    j->result.assign((size_t)5 * ncell, 0.0);
+    if (int rc = fvm_download(j->blk, j->result.data(), 0)) return rc;
+    fvm_block_destroy(j->blk);
+    j->blk = nullptr;
+    s->free_slots.push_back(j->slot);
+    s->used -= (double)j->bytes;
+    j->status = sl.nan ? -1 : 2;
+    j->t_done = now_s(s);
+  }
+  s->active.swap(keep);
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_server** out) {
+  if (!out) return fail(FVM_EINVAL, "null output pointer");
+  if (max_active < 1 || max_active > 4096) return fail(FVM_EINVAL, "max_active %d out of range", max_active);
+  HIP_TRY(hipSetDevice(device));
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  fvm_server* s = new fvm_server();
+  s->device = device;
+  s->budget = hbm_budget_bytes > 0 ? std::min(hbm_budget_bytes, 0.9 * (double)free_b) : 0.8 * (double)free_b;
+  s->max_active = max_active;
+  s->t0 = Clock::now();
+  s->h_slots.assign(max_active, SolveSlot{});
+  for (int i = max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
+  HIP_TRY(hipMalloc(&s->d_slots, max_active * sizeof(SolveSlot)));
+  HIP_TRY(hipMemset(s->d_slots, 0, max_active * sizeof(SolveSlot)));
+  HIP_TRY(hipMalloc(&s->d_ids, max_active * sizeof(int)));
+  *out = s;
+  return FVM_OK;
+}
+
+int fvm_server_destroy(fvm_server* s) {
+  if (!s) return FVM_OK;
+  (void)hipSetDevice(s->device);
+  for (auto& kv : s->groups) fvm_ctx_destroy(kv.second);
+  for (auto& kv : s->jobs) delete kv.second;
+  (void)hipFree(s->d_slots);
+  (void)hipFree(s->d_ids);
+  delete s;
+  return FVM_OK;
+}
+
+int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
+  if (!s || !r || !r->q0) return fail(FVM_EINVAL, "null server, request or initial state");
+  if (r->n[0] < 4 || r->n[1] < 4 || r->n[2] < 1) return fail(FVM_EINVAL, "bad grid size");
+  if (!(r->t_end > 0.0) || !(r->cfl > 0.0)) return fail(FVM_EINVAL, "t_end and cfl must be positive");
+  const int dim = r->n[2] == 1 ? 2 : 3;
+  const int nc = dim == 3 ? 5 : 4;
+  const double cs = (double)(r->n[0] + 6) * (r->n[1] + 6) * (dim == 3 ? r->n[2] + 6 : 1);
+  Job* j = new Job();
+  j->id = s->next_id++;
+  j->req = *r;
+  j->req.q0 = nullptr;
+  const size_t ncell = (size_t)r->n[0] * r->n[1] * r->n[2];
+  j->q0.assign(r->q0, r->q0 + 5 * ncell);
+  j->bytes = (size_t)(4.0 * nc * cs * sizeof(double));
+  if ((double)j->bytes > s->budget) {
+    delete j;
+    return fail(FVM_ENOMEM, "request needs %.3g bytes, budget is %.3g", (double)j->bytes, s->budget);
+  }
+  j->t_submit = now_s(s);
+  s->jobs[j->id] = j;
+  s->pending.push_back(j);
+  if (id) *id = j->id;
+  return FVM_OK;
+}
+
+int fvm_server_poll(fvm_server* s, int iterations, int* n_pending, int* n_active, int* n_done) {
+  if (!s) return fail(FVM_EINVAL, "null server");
+  HIP_TRY(hipSetDevice(s->device));
+  for (int it = 0; it < iterations; ++it) {
+    if (int rc = server_admit(s)) return rc;
+    if (s->active.empty()) break;
+    if (int rc = server_step(s)) return rc;
+    if (int rc = server_retire(s)) return rc;
+  }
+  int done = 0;
+  for (auto& kv : s->jobs) done += kv.second->status == 2 || kv.second->status == -1;
+  if (n_pending) *n_pending = (int)s->pending.size();
+  if (n_active) *n_active = (int)s->active.size();
+  if (n_done) *n_done = done;
+  return FVM_OK;
+}
+
+int fvm_server_result(fvm_server* s, int64_t id, double* q_out, double info[8]) {
+  if (!s) return fail(FVM_EINVAL, "null server");
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return fail(FVM_EINVAL, "unknown request id %lld", (long long)id);
+  Job* j = it->second;
+  if (info) {
+    info[0] = j->t_submit; info[1] = j->t_admit; info[2] = j->t_done;
+    info[3] = j->steps; info[4] = j->cell_updates; info[5] = j->t_final;
+    info[6] = j->status; info[7] = (double)j->bytes;
+  }
+  if (j->status == 0 || j->status == 1) return fail(FVM_ESTATE, "request %lld not finished", (long long)id);
+  if (j->status < 0) return fail(FVM_ENAN, "request %lld diverged (NaN or non-positive dt)", (long long)id);
+  if (q_out) std::memcpy(q_out, j->result.data(), j->result.size() * sizeof(double));
   return FVM_OK;
 }
 

@@ -153,6 +153,10 @@ __device__ __forceinline__ void to_recon_vars(const double raw[NC], double v[NC]
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// max(a, b) where a NaN operand wins, so a diverged state reaches the CFL
+// accumulator (fmax would drop it).
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
 // Wave speed sum_d (|u_d| + c) / dr_d of a conserved state (CFL time step).
 template <int NC>
 __device__ __forceinline__ double wave_speed(const double q[NC], const BlockGeom& g, double gamma) {
@@ -260,7 +264,7 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
 
 template <int NT>
 __device__ __forceinline__ double wg_max(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) red[wid] = v;
   __syncthreads();
@@ -268,8 +272,7 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
   if (threadIdx.x == 0) {
     s = red[0];
     for (int w = 1; w < NT / 64; ++w) {
-      const double x = red[w];
-      s = (x > s || x != x) ? x : s;   // NaN wins
+      s = nan_max(s, red[w]);
     }
   }
   __syncthreads();
@@ -296,15 +299,16 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 
   const Params& P = a.P;
   const int t = threadIdx.x;
-  const double dt = a.dt_dev ? *a.dt_dev : a.dt;
-  const RKCoef kc = rk_coef(a.stage, dt);
-  double smax = 0.0;
+  const bool want_smax = a.smax != nullptr || a.block_smax;
 
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
     const int bi = block_of_tile(a.blocks, a.nblocks, tile);
     const BlockDesc& B = a.blocks[bi];
     const BlockGeom g = B.g;
+    const double dt = B.dt_dev ? *B.dt_dev : (a.dt_dev ? *a.dt_dev : a.dt);
+    const RKCoef kc = rk_coef(a.stage, dt);
+    double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
     const double* __restrict__ q0p = B.q[a.q0_buf];
     double* __restrict__ qout = B.q[a.out_buf];
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
           nrm[m] += r * r;
         }
         store_with_ghosts<NC, 2>(g, a.blocks, B, a.out_buf, qout, c, j, 0, out);
-        if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
+        if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
@@ -427,10 +431,11 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
         if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
       }
     }
-  }
-  if (a.smax) {
-    const double s = wg_max<NT>(smax, red);
-    if (t == 0) atomicMax(a.smax, (unsigned long long)__double_as_longlong(s));
+    if (want_smax) {
+      const double s = wg_max<NT>(smax, red);
+      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
+      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+    }
   }
 }
 
@@ -481,15 +486,16 @@ void stage3d_kernel(StageArgs a) {
   const Params& P = a.P;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  const double dt = a.dt_dev ? *a.dt_dev : a.dt;
-  const RKCoef kc = rk_coef(a.stage, dt);
-  double smax = 0.0;
+  const bool want_smax = a.smax != nullptr || a.block_smax;
 
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
     const int bi = block_of_tile(a.blocks, a.nblocks, tile);
     const BlockDesc& B = a.blocks[bi];
     const BlockGeom g = B.g;
+    const double dt = B.dt_dev ? *B.dt_dev : (a.dt_dev ? *a.dt_dev : a.dt);
+    const RKCoef kc = rk_coef(a.stage, dt);
+    double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
     const double* __restrict__ q0p = B.q[a.q0_buf];
     double* __restrict__ qout = B.q[a.out_buf];
@@ -699,7 +705,7 @@ void stage3d_kernel(StageArgs a) {
           nrm[m] += rr * rr;
         }
         store_with_ghosts<NC, 3>(g, a.blocks, B, a.out_buf, qout, c, r, k, out);
-        if (a.smax) smax = fmax(smax, wave_speed<NC>(out, g, P.gamma));
+        if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
@@ -713,10 +719,11 @@ void stage3d_kernel(StageArgs a) {
         if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
       }
     }
-  }
-  if (a.smax) {
-    const double s = wg_max<NT>(smax, red);
-    if (t == 0) atomicMax(a.smax, (unsigned long long)__double_as_longlong(s));
+    if (want_smax) {
+      const double s = wg_max<NT>(smax, red);
+      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
+      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+    }
   }
 }
 

@@ -46,6 +46,8 @@ struct BlockDesc {
   int nbr_shared[6];     // 1: shared interface vertex, 0: abutting
   int tiles_x, tiles_y, tiles_z, chunk;
   int tile_base;         // first tile of this block in the launch's tile list
+  const double* dt_dev;            // per-block dt (independent solves), or null
+  unsigned long long* smax;        // per-block CFL max accumulator, or null
 };
 
 // RK stage kinds: the combine each launch applies
@@ -72,6 +74,7 @@ struct StageArgs {
   int stage;
   double* norm_part;              // per-tile sums of rhs^2 (NC each), or null
   unsigned long long* smax;       // CFL wave-speed max of the output, or null
+  int block_smax;                 // 1: accumulate into each block's own smax instead
 };
 
 }  // namespace fvm

@@ -39,11 +39,24 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_scheme_set", "fvm_block_create", "fvm_block_destroy", "fvm_block_set_bc",
            "fvm_upload", "fvm_download", "fvm_step", "fvm_norms", "fvm_sync",
            "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
-           "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode")
+           "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
+           "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
+           "fvm_server_result")
 
 
 class FvmError(RuntimeError):
     pass
+
+
+class Request(ctypes.Structure):
+    """fvm_request of include/fvm_gpu.h."""
+    _fields_ = [("n", ctypes.c_int64 * 3), ("spacing", ctypes.c_double * 3),
+                ("recon", ctypes.c_int), ("limiter", ctypes.c_int), ("flux", ctypes.c_int),
+                ("rk", ctypes.c_int), ("kappa", ctypes.c_double),
+                ("entropy_fix", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("ref_compat", ctypes.c_int), ("numerics", ctypes.c_int),
+                ("bc", ctypes.c_int * 6), ("t_end", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("q0", ctypes.c_void_p)]
 
 
 _lib = None
@@ -79,6 +92,11 @@ def load_library(path=LIB_PATH):
         "fvm_autotune": ([vp, ci], ci),
         "fvm_block_connect": ([vp, ci, vp, ci], ci),
         "fvm_set_launch_mode": ([vp, ci], ci),
+        "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
+        "fvm_server_destroy": ([vp], ci),
+        "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
+        "fvm_server_poll": ([vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+        "fvm_server_result": ([vp, i64, vp, ctypes.POINTER(cd)], ci),
         "fvm_block_launch_shape": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                     ctypes.POINTER(ci)], ci),
         "fvm_timing_read": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
@@ -216,3 +234,58 @@ class Context:
         ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         _check(_lib.fvm_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cu)))
         return ms.value, n.value, cu.value
+
+
+class Server:
+    """Continuous-batching server of independent solves (fvm_server_*)."""
+
+    def __init__(self, device=0, hbm_budget=0.0, max_active=64):
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.fvm_server_create(device, float(hbm_budget), int(max_active), ctypes.byref(h)))
+        self._h = h
+        self.shapes = {}
+
+    def close(self):
+        if self._h:
+            _lib.fvm_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, q0, spacing, t_end, cfl, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE,
+               rk=RK_SSP3, kappa=0.33333, entropy_fix=0.2, gamma=1.4, ref_compat=1,
+               numerics=NUMERICS_EXACT, periodic=(False, False, False)):
+        """q0: (5, nk, nj, ni) conserved state.  Returns the request id."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        _, nk, nj, ni = q0.shape
+        bc = [0] * 6
+        for d in range(3):
+            if periodic[d]:
+                bc[2 * d] = bc[2 * d + 1] = BC_PERIODIC
+        r = Request((ni, nj, nk), tuple(float(x) for x in spacing), recon, limiter, flux, rk,
+                    kappa, entropy_fix, gamma, ref_compat, numerics, tuple(bc), float(t_end),
+                    float(cfl), q0.ctypes.data)
+        rid = ctypes.c_int64()
+        _check(_lib.fvm_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        self.shapes[rid.value] = q0.shape
+        return rid.value
+
+    def poll(self, iterations=1):
+        p, a, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.fvm_server_poll(self._h, int(iterations), ctypes.byref(p), ctypes.byref(a),
+                                    ctypes.byref(d)))
+        return p.value, a.value, d.value
+
+    def result(self, rid):
+        """(final state, info dict) of a finished request."""
+        q = np.zeros(self.shapes[rid])
+        info = (ctypes.c_double * 8)()
+        _check(_lib.fvm_server_result(self._h, rid, q.ctypes.data, info))
+        keys = ("t_submit", "t_admit", "t_done", "steps", "cell_updates", "t_final", "status",
+                "bytes")
+        return q, dict(zip(keys, info[:]))

@@ -173,6 +173,40 @@ int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk)
 int fvm_timing_enable(fvm_ctx *ctx, int enable);
 int fvm_timing_read(fvm_ctx *ctx, double *stage_ms, int64_t *launches, double *cell_updates);
 
+/* ---- serving: continuous batching of independent solves ---------------
+ * (no reference counterpart).  A server owns the device state of many
+ * independent solves.  Each fvm_server_poll iteration admits waiting
+ * requests in arrival order while a slot and the HBM budget allow, advances
+ * every active solve by one time step -- solves with the same scheme and
+ * dimension share one batched launch per RK stage, each with its own CFL
+ * time step (dt = min(cfl / max wave speed, t_end - t)) -- and retires the
+ * solves that reached t_end, copying their state to host memory and freeing
+ * their device memory.  A solve's result does not depend on what else is
+ * active (same kernels, per-solve dt; tested bit for bit). */
+typedef struct fvm_server fvm_server;
+typedef struct fvm_request {
+  int64_t n[3];             /* vertices; n[2] == 1: 2-D                         */
+  double spacing[3];
+  int recon, limiter, flux, rk;
+  double kappa, entropy_fix, gamma;
+  int ref_compat, numerics;
+  int bc[6];                /* FVM_BC_EXTRAP / FVM_BC_PERIODIC per face         */
+  double t_end, cfl;
+  const double *q0;         /* (5, nk, nj, ni) conserved state, copied at submit */
+} fvm_request;
+
+/* hbm_budget_bytes <= 0: 80 % of the free device memory. */
+int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_server **out);
+int fvm_server_destroy(fvm_server *srv);
+int fvm_server_submit(fvm_server *srv, const fvm_request *req, int64_t *id);
+/* Run up to `iterations` scheduler iterations (admit, one step, retire);
+ * returns early when nothing is active. */
+int fvm_server_poll(fvm_server *srv, int iterations, int *n_pending, int *n_active, int *n_done);
+/* Final state (same layout as q0) and info[8] = {submit, admit, done (s since
+ * server creation), steps, cell-updates, final time, status (2 done, -1
+ * diverged), device bytes}. */
+int fvm_server_result(fvm_server *srv, int64_t id, double *q_out, double info[8]);
+
 /* Wait for all work of the context. */
 int fvm_sync(fvm_ctx *ctx);
 
