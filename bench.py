@@ -1,19 +1,39 @@
 """Benchmark of the explicit residual + RK path on one MI355X.
 
-Workload (default): 3-D Euler isentropic vortex, 128^3 vertices on a periodic
+Default workload: 3-D Euler isentropic vortex, 128^3 vertices on a periodic
 [0, 10]^3 box, Yee vortex of strength 5 in a (1, 1, 0) free stream, MUSCL
 with the van Albada limiter + Roe flux, classical RK4, CFL 0.8, FP64.
 One bench step = one full RK4 time step (4 fused stage launches + the CFL
 dt update) over the whole grid; the state is resident in HBM throughout.
 
 Metric: FP64 cell-updates per second per RK stage
-        = 128^3 vertices x 4 stages x K steps / wall time of the K steps.
-Roofline: the dominant kernel (the fused 3-D stage) against HBM, with
-algorithmic bytes of 160 B per cell-update (RK4 reads/writes 3, 5, 5, 3
-five-component FP64 arrays over its four stages) divided by the mean stage
-launch duration measured with HIP events on the library's stream.
+        = vertices x stages x steps / wall time of the steps.
+Roofline: the dominant kernel (the fused stage) against HBM, with
+algorithmic bytes per cell-update of 160 B for RK4 (it reads/writes 3, 5,
+5, 3 five-component FP64 arrays over its four stages), 106.7 B for 3-D
+SSP-RK3 (2, 3, 3 arrays) and 85.3 B for 2-D SSP-RK3 (four components),
+divided by the mean stage launch duration measured with HIP events on the
+library's stream.
 CPU baseline: the oracle (a scalar C restatement of the reference
-algorithm) on one host core, one RK4 step of the same 128^3 workload.
+algorithm) on one host core, a bounded sample of the same workload.
+
+Other workloads (--workload):
+  riemann400  2-D four-quadrant Riemann problem, 400 x 400 vertices, the
+              reference's quadrant states, MUSCL-minmod + Roe, SSP-RK3,
+              CFL 0.5, transmissive boundaries, run to t = 0.25.
+  blob27      27 blocks in a 3 x 3 x 3 conforming layout, block sizes drawn
+              from U{32..96} per direction (seed 3: 5.45 M vertices),
+              density-blob state, WENO5-JS + AUSM+, SSP-RK3, CFL 0.5,
+              block-to-block ghost exchange on the device; one batched
+              (persistent when --persistent) launch per stage against one
+              launch per block.
+  serving     Poisson arrivals (50 / s, seeded) of independent solves:
+              2-D Riemann 64^2..512^2 (the reference's quadrant states)
+              and 3-D vortex / blob 32^3..128^3
+              (log-uniform sizes), schemes MUSCL-minmod + Roe or WENO5 +
+              AUSM+, each to a fixed end time, through the continuous-
+              batching server; against the same solves run one after the
+              other through plain contexts.
 """
 import argparse
 import json
@@ -21,26 +41,40 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fvm-3d-f90_amd"))
 
 HBM_PEAK_GBS = 8000.0
 B_ALG = {"rk4_3d": 160.0, "ssp3_3d": 8.0 / 3.0 * 40.0, "ssp3_2d": 8.0 / 3.0 * 32.0}
+METRIC = "FP64 cell-updates/s per RK stage"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (vortex128: 1000, blob27: 200; riemann400 runs to t_end)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=128, help="vertices per direction")
-    ap.add_argument("--cfl", type=float, default=0.8)
+    ap.add_argument("--workload", default="vortex128",
+                    choices=("vortex128", "riemann400", "blob27", "serving"))
+    ap.add_argument("--n", type=int, default=None, help="vertices per direction")
+    ap.add_argument("--cfl", type=float, default=None)
+    ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--numerics", choices=("exact", "fast"), default="fast",
                     help="exact: bit-identical to the reference arithmetic; fast: FMA + "
                          "reciprocals, <= 1e-12 rel. after 1 step, <= 1e-9 after 1000 (tested)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--persistent", action="store_true",
+                    help="blob27: persistent launch (grid of 2 workgroups per CU) instead of "
+                         "one workgroup per tile")
+    ap.add_argument("--requests", type=int, default=96, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: arrivals per second")
+    ap.add_argument("--steps-per-sync", type=int, default=4,
+                    help="serving: time steps per host synchronisation")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -59,20 +93,403 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_launch": x.numel() * 8, "launches": 5}))
 
 
-def cpu_baseline(p, h, args):
-    """The oracle on one host core, a bounded sample of the same workload."""
+def oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    n = args.n
+    return O
+
+
+def numerics_code(fvmgpu, args):
+    return fvmgpu.NUMERICS_FAST if args.numerics == "fast" else fvmgpu.NUMERICS_EXACT
+
+
+def roofline(stage_ms, launches, cells_per_launch, b_alg, traffic=None, kernel=""):
+    avg_launch_s = stage_ms * 1e-3 / max(launches, 1)
+    achieved = b_alg * cells_per_launch / avg_launch_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+            "avg_launch_us": avg_launch_s * 1e6, "launches": launches,
+            "bytes_per_cell_update": b_alg}
+
+
+def line(args, value, steps, sec, config, roof, cpu):
+    return {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": sec * 1e3 / max(steps, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+# ---------------------------------------------------------------------------
+# vortex128: the headline
+# ---------------------------------------------------------------------------
+def bench_vortex(args):
+    import fvmgpu
+    from fvmgpu import cases
+    n = args.n or 128
+    steps = args.steps or 1000
+    cfl = args.cfl or 0.8
+    p, h, _ = cases.isentropic_vortex(n, seed=args.seed or 0)
+    q = cases.to_conserved(p)
+    ctx = fvmgpu.Context(0)
+    ctx.scheme(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_VANALBADA, flux=fvmgpu.ROE,
+               rk=fvmgpu.RK4, kappa=0.33333, entropy_fix=0.2, gamma=1.4)
+    blk = ctx.block((n, n, n), (h, h, h))
+    blk.set_periodic()
+    blk.upload(q)
+    ctx.numerics(numerics_code(fvmgpu, args))
+    if not args.no_tune:
+        ctx.autotune(reps=3)
+    tile = blk.launch_shape()
+    if args.warmup:
+        ctx.step(args.warmup, cfl=cfl)
+    ctx.sync()
+
+    ctx.timing(True)
+    ctx.sync()
     t0 = time.perf_counter()
-    O.oracle_solve(p, 3, (h, h, h), recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE,
-                   rk=O.RK4, kappa=0.33333, entfix=0.2, dt=0.0, cfl=args.cfl,
-                   nsteps=args.cpu_steps, p_from_q=1, bc=(O.BC_PERIODIC,) * 6)
+    dt_last = ctx.step(steps, cfl=cfl)
+    ctx.sync()
+    t1 = time.perf_counter()
+    stage_ms, launches, _ = ctx.timing_read()
+    ctx.timing(False)
+    norms, nan = ctx.norms()
+    if nan:
+        raise SystemExit(f"bench.py: residual NaN or diverged, norms {norms}")
+
+    sec = t1 - t0
+    ncell = float(n) ** 3
+    value = ncell * 4 * steps / sec
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "stage3d_muscl_roe_vortex128_traffic.json")
+    if os.path.exists(tpath) and n == 128:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof = roofline(stage_ms, launches, ncell, B_ALG["rk4_3d"], traffic,
+                    f"fvm::{args.numerics}::stage3d_kernel<ReconMuscl, 300, {tile[0]}, {tile[1]}>")
+    config = {"workload": f"isentropic-vortex-3d {n}^3 periodic, MUSCL-vanAlbada + Roe, "
+                          f"RK4, CFL {cfl}, FP64",
+              "grid": [n, n, n], "stages_per_step": 4, "dt_last": dt_last,
+              "numerics": args.numerics, "tile_xy_chunk": list(tile),
+              "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)}
+    cpu = None
+    if not args.no_cpu:
+        O = oracle()
+        t0 = time.perf_counter()
+        O.oracle_solve(p, 3, (h, h, h), recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE,
+                       rk=O.RK4, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
+                       nsteps=args.cpu_steps, p_from_q=1, bc=(O.BC_PERIODIC,) * 6)
+        csec = time.perf_counter() - t0
+        cells = ncell * 4 * args.cpu_steps
+        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload "
+                         f"({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+    return line(args, value, steps, sec, config, roof, cpu)
+
+
+# ---------------------------------------------------------------------------
+# riemann400
+# ---------------------------------------------------------------------------
+def bench_riemann(args):
+    import fvmgpu
+    from fvmgpu import cases
+    n = args.n or 400
+    cfl = args.cfl or 0.5
+    t_end = 0.25
+    p, dx = cases.riemann2d(n, seed=args.seed or 0)
+    q = cases.to_conserved(p)
+    ctx = fvmgpu.Context(0)
+    ctx.scheme(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_MINMOD, flux=fvmgpu.ROE,
+               rk=fvmgpu.RK_SSP3, kappa=0.33333, entropy_fix=0.2, gamma=1.4)
+    blk = ctx.block((n, n, 1), (dx, dx, 1.0))
+    blk.upload(q)
+    ctx.numerics(numerics_code(fvmgpu, args))
+    if not args.no_tune:
+        ctx.autotune(reps=3)
+    tile = blk.launch_shape()
+    if args.warmup:
+        ctx.step(args.warmup, cfl=cfl)
+    blk.upload(q)                      # restart from the initial state
+    ctx.set_time(0.0, t_end)
+    ctx.sync()
+
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    chunk = 64
+    while True:
+        dt_last = ctx.step(chunk, cfl=cfl)
+        t, steps = ctx.time()
+        if t >= t_end * (1 - 1e-14):
+            break
+        # size the next batch from the last dt so few skipped steps are queued
+        chunk = int(min(64, max(1, np.ceil((t_end - t) / dt_last) + 1)))
+    ctx.sync()
+    t1 = time.perf_counter()
+    stage_ms, launches, cu = ctx.timing_read()
+    ctx.timing(False)
+    norms, nan = ctx.norms()
+    if nan:
+        raise SystemExit(f"bench.py: residual NaN or diverged, norms {norms}")
+    _, steps = ctx.time()
+    steps -= args.warmup
+    sec = t1 - t0
+    ncell = float(n) * n
+    value = ncell * 3 * steps / sec
+    # launches that the halt skipped are in the event count but did no work
+    roof = roofline(stage_ms, steps * 3, ncell, B_ALG["ssp3_2d"], None,
+                    f"fvm::{args.numerics}::stage2d_kernel<ReconMuscl, 300>")
+    config = {"workload": f"riemann-2d {n}x{n} four-quadrant (reference states), MUSCL-minmod "
+                          f"+ Roe, SSP-RK3, CFL {cfl}, transmissive, t_end {t_end}, FP64",
+              "grid": [n, n, 1], "stages_per_step": 3, "t_final": t, "numerics": args.numerics,
+              "tile_xy_chunk": list(tile),
+              "pct_hbm_roofline": 100.0 * value * B_ALG["ssp3_2d"] / (HBM_PEAK_GBS * 1e9)}
+    cpu = None
+    if not args.no_cpu:
+        O = oracle()
+        cs = max(args.cpu_steps, 10)
+        t0 = time.perf_counter()
+        O.oracle_solve(p, 2, (dx, dx, 1.0), recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE,
+                       rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl, nsteps=cs,
+                       p_from_q=1)
+        csec = time.perf_counter() - t0
+        cells = ncell * 3 * cs
+        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{cs} SSP-RK3 steps of the {n}x{n} Riemann workload "
+                         f"({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+    return line(args, value, steps, sec, config, roof, cpu)
+
+
+# ---------------------------------------------------------------------------
+# blob27
+# ---------------------------------------------------------------------------
+def build_blob27(fvmgpu, cases, seed, mode, args):
+    sizes, offs, sp, states = cases.blob_blocks(seed)
+    ctx = fvmgpu.Context(0)
+    ctx.scheme(recon=fvmgpu.WENO, limiter=fvmgpu.LIM_NONE, flux=fvmgpu.AUSMPLUS,
+               rk=fvmgpu.RK_SSP3, kappa=0.33333, entropy_fix=0.2, gamma=1.4, ref_compat=0)
+    ctx.numerics(numerics_code(fvmgpu, args))
+    ctx.launch_mode(mode)
+    blocks = {}
+    for key, pb in states.items():
+        nk, nj, ni, _ = pb.shape
+        b = ctx.block((ni, nj, nk), sp)
+        b.upload(cases.to_conserved(pb))
+        blocks[key] = b
+    for (a, b_, c), blk in blocks.items():
+        idx = [a, b_, c]
+        for d in range(3):
+            if idx[d] + 1 < 3:
+                nb = list(idx)
+                nb[d] += 1
+                blk.connect(2 * d + 1, blocks[tuple(nb)], shared=True)
+    return ctx, blocks, sizes, sp, states
+
+
+def time_steps(ctx, steps, cfl, warmup):
+    if warmup:
+        ctx.step(warmup, cfl=cfl)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    ctx.step(steps, cfl=cfl)
+    ctx.sync()
     sec = time.perf_counter() - t0
-    cells = float(n) ** 3 * 4 * args.cpu_steps
-    return {"value": cells / sec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload "
-                      f"({cells:.3g} cell-updates, {sec:.1f} s, oracle/fvm_oracle.c)"}
+    stage_ms, launches, _ = ctx.timing_read()
+    ctx.timing(False)
+    norms, nan = ctx.norms()
+    if nan:
+        raise SystemExit(f"bench.py: residual NaN or diverged, norms {norms}")
+    return sec, stage_ms, launches
+
+
+def bench_blob27(args):
+    import fvmgpu
+    from fvmgpu import cases
+    seed = 3 if args.seed is None else args.seed
+    steps = args.steps or 200
+    cfl = args.cfl or 0.5
+    batched = fvmgpu.LAUNCH_PERSISTENT if args.persistent else fvmgpu.LAUNCH_BATCHED
+    results = {}
+    for name, mode in (("batched", batched), ("per_block", fvmgpu.LAUNCH_PER_BLOCK)):
+        ctx, blocks, sizes, sp, states = build_blob27(fvmgpu, cases, seed, mode, args)
+        if not args.no_tune:
+            ctx.autotune(reps=2)
+        shapes = sorted(set(b.launch_shape() for b in blocks.values()))
+        sec, stage_ms, launches = time_steps(ctx, steps, cfl, args.warmup)
+        results[name] = (sec, stage_ms, launches, shapes)
+        ctx.close()
+    ncell = float(sum(v.shape[0] * v.shape[1] * v.shape[2] for v in states.values()))
+    sec, stage_ms, launches, shapes = results["batched"]
+    value = ncell * 3 * steps / sec
+    pb_sec = results["per_block"][0]
+    roof = roofline(stage_ms, launches, ncell, B_ALG["ssp3_3d"], None,
+                    f"fvm::{args.numerics}::stage3d_kernel<ReconWeno, 1303, *, *> over 27 blocks")
+    config = {"workload": f"blob-3d 27 blocks 3x3x3 (sizes {sizes}, seed {seed}), WENO5-JS + "
+                          f"AUSM+, SSP-RK3, CFL {cfl}, device ghost exchange, FP64",
+              "vertices": ncell, "stages_per_step": 3, "numerics": args.numerics,
+              "launch": "persistent" if args.persistent else "batched",
+              "tile_xy_chunk": [list(s) for s in shapes],
+              "per_block_launches_value": ncell * 3 * steps / pb_sec,
+              "speedup_vs_per_block": pb_sec / sec,
+              "pct_hbm_roofline": 100.0 * value * B_ALG["ssp3_3d"] / (HBM_PEAK_GBS * 1e9)}
+    cpu = None
+    if not args.no_cpu:
+        O = oracle()
+        key = max(states, key=lambda k: states[k].size)
+        pb = states[key]
+        t0 = time.perf_counter()
+        O.oracle_solve(pb, 3, sp, recon=O.WENO, limiter=O.LIM_NONE, flux=O.AUSMPLUS,
+                       rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
+                       nsteps=args.cpu_steps, p_from_q=1, weno_textbook=1)
+        csec = time.perf_counter() - t0
+        cells = float(pb.shape[0] * pb.shape[1] * pb.shape[2]) * 3 * args.cpu_steps
+        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} SSP-RK3 steps of the largest block {pb.shape[:3]} "
+                         f"alone ({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+    return line(args, value, steps, sec, config, roof, cpu)
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+# end times per case kind: long enough that the offered load (50 / s)
+# exceeds what the GPU finishes, so the served rate measures capacity
+SERVE_T_END = {"riemann2d": 0.1, "vortex3d": 1.0, "blob3d": 0.1}
+
+
+def serving_requests(fvmgpu, cases, seed, nreq, rate):
+    """The request stream: arrival times and fvmgpu.Server.submit arguments."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    schemes = (dict(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_MINMOD, flux=fvmgpu.ROE,
+                    rk=fvmgpu.RK_SSP3, ref_compat=1),
+               dict(recon=fvmgpu.WENO, limiter=fvmgpu.LIM_NONE, flux=fvmgpu.AUSMPLUS,
+                    rk=fvmgpu.RK_SSP3, ref_compat=0))
+    reqs = []
+    for i in range(nreq):
+        kind = rng.integers(3)            # 0: Riemann 2-D, 1: vortex 3-D, 2: blob 3-D
+        sch = dict(schemes[rng.integers(2)])
+        if kind == 0:
+            n = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
+            # the reference's quadrant states (Lax-Liu configuration 3): the
+            # drawn states of other seeds include near-vacuum rarefactions in
+            # which Roe's linearisation loses positivity
+            p, dx = cases.riemann2d(n, seed=0)
+            r = dict(q0=cases.to_conserved(p), spacing=(dx, dx, 1.0),
+                     t_end=SERVE_T_END["riemann2d"], cfl=0.5,
+                     label=f"riemann2d {n}^2")
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(32), np.log(128)))))
+            if kind == 1:
+                p, h, _ = cases.isentropic_vortex(n, seed=1000 + i)
+                r = dict(q0=cases.to_conserved(p), spacing=(h, h, h),
+                         t_end=SERVE_T_END["vortex3d"], cfl=0.5,
+                         periodic=(True, True, True), label=f"vortex3d {n}^3")
+            else:
+                x = np.linspace(0.0, 1.0, n)
+                p = cases.blob(x, x, x, seed=1000 + i)
+                h = 1.0 / (n - 1)
+                r = dict(q0=cases.to_conserved(p), spacing=(h, h, h),
+                         t_end=SERVE_T_END["blob3d"], cfl=0.5,
+                         label=f"blob3d {n}^3")
+        r.update(sch)
+        reqs.append(r)
+    return arrivals, reqs
+
+
+def bench_serving(args):
+    import fvmgpu
+    from fvmgpu import cases
+    seed = 0 if args.seed is None else args.seed
+    arrivals, reqs = serving_requests(fvmgpu, cases, seed, args.requests, args.rate)
+    numerics = numerics_code(fvmgpu, args)
+
+    def submit_args(r):
+        a = {k: v for k, v in r.items() if k != "label"}
+        a["numerics"] = numerics
+        return a
+
+    # warm-up: one tiny solve per (dimension, scheme) so no first-launch cost
+    # lands in the timed run
+    srv = fvmgpu.Server(max_active=256)
+    for sch in ({k: reqs[0][k] for k in ("recon", "limiter", "flux", "rk", "ref_compat")},
+                dict(recon=fvmgpu.WENO, limiter=fvmgpu.LIM_NONE, flux=fvmgpu.AUSMPLUS,
+                     rk=fvmgpu.RK_SSP3, ref_compat=0),
+                dict(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_MINMOD, flux=fvmgpu.ROE,
+                     rk=fvmgpu.RK_SSP3, ref_compat=1)):
+        p2, dx = cases.riemann2d(24, seed=1)
+        srv.submit(q0=cases.to_conserved(p2), spacing=(dx, dx, 1.0), t_end=1e-3, cfl=0.5,
+                   numerics=numerics, **sch)
+        x = np.linspace(0.0, 1.0, 12)
+        srv.submit(q0=cases.to_conserved(cases.blob(x, x, x)), spacing=(1 / 11,) * 3, t_end=1e-3,
+                   cfl=0.5, numerics=numerics, **sch)
+    while srv.poll(100, args.steps_per_sync)[1]:
+        pass
+    srv.close()
+    srv = fvmgpu.Server(max_active=256)
+    t0 = time.perf_counter()
+    ids = []
+    nxt = 0
+    done = 0
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            ids.append(srv.submit(**submit_args(reqs[nxt])))
+            nxt += 1
+        pend, act, done = srv.poll(1, args.steps_per_sync)
+        if act == 0 and pend == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.001)))
+    makespan = time.perf_counter() - t0
+    lat, cell_updates, steps = [], 0.0, 0
+    for i, rid in zip(range(len(reqs)), ids):
+        try:
+            _, info = srv.result(rid)
+        except fvmgpu.FvmError as e:
+            raise SystemExit(f"bench.py: request {i} ({reqs[i]['label']}) failed: {e}")
+        lat.append(info["t_done"] - arrivals[i])
+        cell_updates += info["cell_updates"]
+        steps += int(info["steps"])
+    srv.close()
+    served_value = cell_updates / makespan
+
+    # sequential plain solves: one context per request, run to t_end
+    ts = time.perf_counter()
+    for r in reqs:
+        _, nk, nj, ni = r["q0"].shape
+        with fvmgpu.Context(0) as ctx:
+            ctx.scheme(recon=r["recon"], limiter=r["limiter"], flux=r["flux"], rk=r["rk"],
+                       ref_compat=r["ref_compat"])
+            ctx.numerics(numerics)
+            b = ctx.block((ni, nj, nk), r["spacing"])
+            if any(r.get("periodic", ())):
+                b.set_periodic()
+            b.upload(r["q0"])
+            ctx.set_time(0.0, r["t_end"])
+            chunk = 16
+            while True:
+                dt = ctx.step(chunk, cfl=r["cfl"])
+                t, _ = ctx.time()
+                if t >= r["t_end"] * (1 - 1e-14):
+                    break
+                chunk = int(min(64, max(1, np.ceil((r["t_end"] - t) / dt) + 1)))
+            b.download()
+    seq_sec = time.perf_counter() - ts
+    seq_value = cell_updates / seq_sec
+
+    lat = np.array(lat)
+    config = {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.rate}/s (seed {seed}); "
+                          "riemann2d 64^2..512^2, vortex3d / blob3d 32^3..128^3, log-uniform "
+                          f"sizes, end times {SERVE_T_END}, MUSCL-minmod + Roe or WENO5 + "
+                          "AUSM+, SSP-RK3, CFL 0.5, FP64",
+              "requests": len(reqs), "total_steps": steps, "cell_updates": cell_updates,
+              "makespan_s": makespan, "p50_time_to_solution_s": float(np.percentile(lat, 50)),
+              "p99_time_to_solution_s": float(np.percentile(lat, 99)),
+              "sequential_plain_value": seq_value, "sequential_plain_s": seq_sec,
+              "speedup_vs_sequential": served_value / seq_value,
+              "steps_per_sync": args.steps_per_sync, "numerics": args.numerics}
+    return line(args, served_value, steps, makespan, config, None, None)
 
 
 def main():
@@ -84,79 +501,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         sys.exit(2)
-    import fvmgpu
-    from fvmgpu import cases
-
-    n = args.n
-    p, h, _ = cases.isentropic_vortex(n, seed=0)
-    q = cases.to_conserved(p)
-    ctx = fvmgpu.Context(0)
-    ctx.scheme(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_VANALBADA, flux=fvmgpu.ROE,
-               rk=fvmgpu.RK4, kappa=0.33333, entropy_fix=0.2, gamma=1.4)
-    blk = ctx.block((n, n, n), (h, h, h))
-    blk.set_periodic()
-    blk.upload(q)
-    ctx.numerics(fvmgpu.NUMERICS_FAST if args.numerics == "fast" else fvmgpu.NUMERICS_EXACT)
-    if not args.no_tune:
-        ctx.autotune(reps=3)
-    tile = blk.launch_shape()
-    if args.warmup:
-        ctx.step(args.warmup, cfl=args.cfl)
-    ctx.sync()
-
-    ctx.timing(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    dt_last = ctx.step(args.steps, cfl=args.cfl)
-    ctx.sync()
-    t1 = time.perf_counter()
-    stage_ms, launches, cell_updates = ctx.timing_read()
-    ctx.timing(False)
-    norms, nan = ctx.norms()
-    if nan:
-        raise SystemExit(f"bench.py: residual NaN or diverged, norms {norms}")
-
-    sec = t1 - t0
-    ncell = float(n) ** 3
-    value = ncell * 4 * args.steps / sec
-    avg_launch_s = stage_ms * 1e-3 / max(launches, 1)
-    achieved = B_ALG["rk4_3d"] * ncell / avg_launch_s / 1e9
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stage3d_muscl_roe_vortex128_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    out = {
-        "metric": "FP64 cell-updates/s per RK stage",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"isentropic-vortex-3d {n}^3 periodic, MUSCL-vanAlbada + Roe, "
-                               f"RK4, CFL {args.cfl}, FP64",
-                   "grid": [n, n, n], "stages_per_step": 4, "dt_last": dt_last,
-                   "numerics": args.numerics, "tile_xy_chunk": list(tile),
-                   "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"fvm::{args.numerics}::stage3d_kernel<ReconMuscl, 300, "
-                               f"{tile[0]}, {tile[1]}>",
-                     "avg_launch_us": avg_launch_s * 1e6, "launches": launches,
-                     "bytes_per_cell_update": B_ALG["rk4_3d"]},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, h, args)
-    print(json.dumps(out), flush=True)
+    fn = {"vortex128": bench_vortex, "riemann400": bench_riemann, "blob27": bench_blob27,
+          "serving": bench_serving}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

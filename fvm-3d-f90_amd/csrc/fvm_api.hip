@@ -65,6 +65,7 @@ struct fvm_block {
   // independent solve (serving): own dt and CFL accumulator on the device
   const double* dt_dev = nullptr;
   unsigned long long* smax_dev = nullptr;
+  const int* halt_dev = nullptr;
 };
 
 struct fvm_ctx {
@@ -82,8 +83,14 @@ struct fvm_ctx {
   bool desc_dirty = true;                // table must be rebuilt before a launch
   bool ghosts_dirty = true;              // u^n ghost layers must be refilled
   int total_tiles = 0;
-  double* d_dt = nullptr;                 // CFL dt of the current step
-  unsigned long long* d_smax = nullptr;   // wave-speed max of the current state
+  bool has_conn = false;                 // some block face is connected to another block
+  SolveSlot* d_slot = nullptr;            // device time-stepping state (CFL mode)
+  SolveSlot h_slot{};                     // host staging of d_slot
+  double* d_dt = nullptr;                 // &d_slot->dt: CFL dt of the current step
+  unsigned long long* d_smax = nullptr;   // &d_slot->smax: wave-speed max of the current state
+  double time = 0.0;                      // solution time after the last step
+  double t_end = HUGE_VAL;                // CFL steps stop here (halt), +inf: never
+  int64_t steps_done = 0;
   double* d_norm = nullptr;               // per-tile partial sums of rhs^2
   size_t norm_cap = 0;                    // doubles
   std::vector<double> h_norm;
@@ -173,36 +180,38 @@ __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__
   if (threadIdx.x == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(s));
 }
 
-// dt = cfl / smax for the step about to start; clears the accumulator so the
-// step's last stage can collect the next maximum.
-__global__ void cfl_dt_kernel(unsigned long long* smax, double* dt, double cfl) {
-  const double s = __longlong_as_double((long long)*smax);
-  *dt = cfl / s;
-  *smax = 0ull;
-}
+// Relative tolerance on t_end: a run whose time is this close counts as done
+// (t + (t_end - t) can round one ulp short of t_end).
+constexpr double kEndTol = 1.0e-14;
 
-// Per-solve time step of the serving loop: dt = min(cfl / smax, t_end - t),
-// t += dt, and the CFL accumulator cleared for the step's last stage.
-struct SolveSlot {
-  double t, t_end, cfl, dt;
-  unsigned long long smax;
-  int steps;
-  int nan;
-};
-
-__global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  SolveSlot& s = slots[ids[i]];
+// Time step of one solve for the step about to start: dt = min(cfl / smax,
+// t_end - t), t += dt; clears the wave-speed accumulator so the step's last
+// stage collects the next maximum.  Once t reaches t_end the slot halts and
+// the step's stage launches return at once.
+__device__ __forceinline__ void advance_slot(SolveSlot& s) {
+  if (s.halt) return;
+  // (an infinite t_end never ends: inf - tol * inf would be NaN)
+  if (isfinite(s.t_end) && !(s.t < s.t_end - kEndTol * fabs(s.t_end))) { s.halt = 1; return; }
   const double sm = __longlong_as_double((long long)s.smax);
   double dt = s.cfl / sm;
+  if (!(dt > 0.0) || !(dt < 1.0e300)) s.nan = 1;
   const double rest = s.t_end - s.t;
-  if (!(dt > 0.0) || !(dt < 1.0e300)) { s.nan = 1; dt = 0.0; }
   if (dt > rest) dt = rest;
   s.dt = dt;
   s.t = s.t + dt;
   s.smax = 0ull;
   s.steps = s.steps + 1;
+}
+
+__global__ void cfl_dt_kernel(SolveSlot* slot) { advance_slot(*slot); }
+
+// Fixed time step: the stage kernels read dt from the slot in either mode.
+__global__ void set_dt_kernel(SolveSlot* slot, double dt) { slot->dt = dt; }
+
+// The same for the solves of a serving group (slot ids listed in ids).
+__global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) advance_slot(slots[ids[i]]);
 }
 
 }  // namespace
@@ -273,16 +282,19 @@ int sync_desc(fvm_ctx* ctx) {
   if (!ctx->desc_dirty) return FVM_OK;
   std::vector<BlockDesc> h(ctx->blocks.size());
   int base = 0;
+  ctx->has_conn = false;
   for (size_t i = 0; i < ctx->blocks.size(); ++i) {
     fvm_block* b = ctx->blocks[i];
     BlockDesc& d = h[i];
+    for (int f = 0; f < 6; ++f) ctx->has_conn = ctx->has_conn || b->g.bc[f] == BC_CONN;
     for (int k = 0; k < 4; ++k) d.q[k] = b->q[k];
     d.g = b->g;
     for (int f = 0; f < 6; ++f) { d.nbr[f] = b->nbr[f]; d.nbr_shared[f] = b->nbr_shared[f]; }
     d.tiles_x = b->tiles_x; d.tiles_y = b->tiles_y; d.tiles_z = b->tiles_z; d.chunk = b->chunk;
     d.tile_base = base;
-    d.dt_dev = b->dt_dev;
+    d.dt_dev = b->dt_dev ? b->dt_dev : ctx->d_dt;
     d.smax = b->smax_dev;
+    d.halt = b->halt_dev;
     b->tile_base = base;
     base += b->ntiles;
   }
@@ -333,15 +345,15 @@ template <class Fam>
 hipError_t launch_family(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
   if (dim == 2) {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s2(ctx->flux, a, grid, ctx->stream);
-      case RECON_MUSCL: return Fam::m2(ctx->flux, a, grid, ctx->stream);
-      case RECON_WENO: return Fam::w2(ctx->flux, a, grid, ctx->stream);
+      case RECON_SIMPLE: return Fam::s2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
     }
   } else {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s3(ctx->flux, tile, a, grid, ctx->stream);
-      case RECON_MUSCL: return Fam::m3(ctx->flux, tile, a, grid, ctx->stream);
-      case RECON_WENO: return Fam::w3(ctx->flux, tile, a, grid, ctx->stream);
+      case RECON_SIMPLE: return Fam::s3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
     }
   }
   return hipErrorInvalidValue;
@@ -429,9 +441,10 @@ int fvm_ctx_create(int device, fvm_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cu = prop.multiProcessorCount;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  HIP_TRY(hipMalloc(&ctx->d_dt, sizeof(double)));
-  HIP_TRY(hipMalloc(&ctx->d_smax, sizeof(unsigned long long)));
-  HIP_TRY(hipMemset(ctx->d_smax, 0, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&ctx->d_slot, sizeof(SolveSlot)));
+  HIP_TRY(hipMemset(ctx->d_slot, 0, sizeof(SolveSlot)));
+  ctx->d_dt = &ctx->d_slot->dt;
+  ctx->d_smax = &ctx->d_slot->smax;
   *out = ctx;
   g_err.clear();
   return FVM_OK;
@@ -447,8 +460,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   (void)hipFree(ctx->d_desc);
-  (void)hipFree(ctx->d_dt);
-  (void)hipFree(ctx->d_smax);
+  (void)hipFree(ctx->d_slot);
   (void)hipFree(ctx->d_norm);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -712,7 +724,14 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   if (int rc = ensure_norm_capacity(ctx, need)) return rc;
 
   if (cfl_mode) {
-    HIP_TRY(hipMemsetAsync(ctx->d_smax, 0, sizeof(unsigned long long), ctx->stream));
+    // the device slot restarts from the host's time; the wave-speed maximum
+    // of u^n is taken afresh (later steps take it from their last stage)
+    ctx->h_slot = SolveSlot{};
+    ctx->h_slot.t = ctx->time;
+    ctx->h_slot.t_end = ctx->t_end;
+    ctx->h_slot.cfl = cfl;
+    HIP_TRY(hipMemcpyAsync(ctx->d_slot, &ctx->h_slot, sizeof(SolveSlot), hipMemcpyHostToDevice,
+                           ctx->stream));
     for (fvm_block* b : ctx->blocks) {
       if (b->nc == 5)
         hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g, ctx->P.gamma, ctx->d_smax);
@@ -720,6 +739,13 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
         hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g, ctx->P.gamma, ctx->d_smax);
       HIP_TRY(hipGetLastError());
     }
+  }
+  // with an end time a step may be skipped on the device, so every step
+  // leaves its norm partials (the last step that ran is the one reported)
+  const bool may_halt = cfl_mode && std::isfinite(ctx->t_end);
+  if (!cfl_mode) {
+    hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot, dt);
+    HIP_TRY(hipGetLastError());
   }
 
   const int nstage = ctx->rk == RK4 ? 4 : 3;
@@ -734,15 +760,13 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   }
   for (int s = 0; s < nsteps; ++s) {
     if (cfl_mode) {
-      hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_smax, ctx->d_dt, cfl);
+      hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot);
       HIP_TRY(hipGetLastError());
     }
     for (int st = 1; st <= nstage; ++st) {
       const bool last = st == nstage;
       StageArgs a{};
       a.P = ctx->P;
-      a.dt = cfl_mode ? 0.0 : dt;
-      a.dt_dev = cfl_mode ? ctx->d_dt : nullptr;
       a.q0_buf = 0;
       a.use_acc = 0;
       if (ctx->rk == RK4) {
@@ -760,8 +784,9 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
           default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
         }
       }
-      a.norm_part = (last && s == nsteps - 1) ? ctx->d_norm : nullptr;
+      a.norm_part = (last && (s == nsteps - 1 || may_halt)) ? ctx->d_norm : nullptr;
       a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
+      a.halt = cfl_mode ? &ctx->d_slot->halt : nullptr;
       if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
       if (int rc = launch_stage_all(ctx, a)) return rc;
       if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
@@ -774,9 +799,20 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   ctx->h_norm.resize(need);
   HIP_TRY(hipMemcpyAsync(ctx->h_norm.data(), ctx->d_norm, need * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
-  double dt_host = dt;
-  if (cfl_mode) HIP_TRY(hipMemcpyAsync(&dt_host, ctx->d_dt, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (cfl_mode)
+    HIP_TRY(hipMemcpyAsync(&ctx->h_slot, ctx->d_slot, sizeof(SolveSlot), hipMemcpyDeviceToHost,
+                           ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  double dt_host = dt;
+  int ran = nsteps;
+  if (cfl_mode) {
+    dt_host = ctx->h_slot.dt;
+    ran = ctx->h_slot.steps;
+    ctx->time = ctx->h_slot.t;
+  } else {
+    for (int s = 0; s < nsteps; ++s) ctx->time = ctx->time + dt;
+  }
+  ctx->steps_done += ran;
   if (dt_last) *dt_last = dt_host;
   if (ctx->timing) {
     for (size_t e = 0; e < nev; e += 2) {
@@ -786,7 +822,7 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     }
     ctx->stage_launches += (int64_t)(nev / 2);
     for (fvm_block* b : ctx->blocks)
-      ctx->cell_updates += (double)b->g.nx * b->g.ny * b->g.nz * nstage * nsteps;
+      ctx->cell_updates += (double)b->g.nx * b->g.ny * b->g.nz * nstage * ran;
   }
   double res_abs[6] = {0, 0, 0, 0, 0, 0};
   for (fvm_block* b : ctx->blocks) {
@@ -811,10 +847,30 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     if (std::isnan(res_abs[e])) nan = 1;
   }
   if (res_abs[5] > 1.0e10) nan = 1;
-  if (cfl_mode && !(dt_host > 0.0 && std::isfinite(dt_host))) nan = 1;
+  if (cfl_mode && ctx->h_slot.nan) nan = 1;
+  if (ran == 0) {
+    // the run had already reached its end time: nothing new to report
+    ctx->have_norms = false;
+    return FVM_OK;
+  }
   ctx->nan_flag = nan;
   ctx->have_norms = true;
-  if (cfl_mode && nan && !std::isfinite(dt_host)) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
+  if (cfl_mode && ctx->h_slot.nan) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
+  return FVM_OK;
+}
+
+int fvm_set_time(fvm_ctx* ctx, double t, double t_end) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!std::isfinite(t)) return fail(FVM_EINVAL, "time must be finite");
+  ctx->time = t;
+  ctx->t_end = (t_end > t && std::isfinite(t_end)) ? t_end : HUGE_VAL;
+  return FVM_OK;
+}
+
+int fvm_time(fvm_ctx* ctx, double* t, int64_t* steps) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (t) *t = ctx->time;
+  if (steps) *steps = ctx->steps_done;
   return FVM_OK;
 }
 
@@ -847,6 +903,8 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot, 1.0e-6);
+  HIP_TRY(hipGetLastError());
   const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
   const int dim = ctx->blocks[0]->dim;
   const bool together = ctx->launch_mode != FVM_LAUNCH_PER_BLOCK;
@@ -863,7 +921,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
         for (fvm_block* b : sel) plan_launch(b, dim == 3 ? tile : -1, chunk);
         if (int rc = sync_desc(ctx)) return rc;
         StageArgs a{};
-        a.P = ctx->P; a.dt = 1.0e-6; a.dt_dev = nullptr;
+        a.P = ctx->P;
         a.in_buf = 0; a.q0_buf = 0; a.out_buf = 1;
         a.use_acc = ctx->rk == RK4 ? 1 : 0;
         a.stage = ctx->rk == RK4 ? ST_RK4_1 : ST_SSP1;
@@ -1014,12 +1072,14 @@ int server_admit(fvm_server* s) {
     j->q0.shrink_to_fit();
     const int slot = s->free_slots.back();
     s->free_slots.pop_back();
-    SolveSlot init{};
-    init.t = 0.0; init.t_end = r.t_end; init.cfl = r.cfl; init.dt = 0.0; init.smax = 0ull;
-    init.steps = 0; init.nan = 0;
+    SolveSlot& init = s->h_slots[slot];   // staging that outlives the async copy
+    init = SolveSlot{};
+    init.t_end = r.t_end;
+    init.cfl = r.cfl;
     HIP_TRY(hipMemcpyAsync(s->d_slots + slot, &init, sizeof(SolveSlot), hipMemcpyHostToDevice, g->stream));
     b->dt_dev = &s->d_slots[slot].dt;
     b->smax_dev = &s->d_slots[slot].smax;
+    b->halt_dev = &s->d_slots[slot].halt;
     g->desc_dirty = true;
     if (int rc = sync_desc(g)) return rc;
     if (int rc = fill_ghosts_all(g, 0)) return rc;
@@ -1040,9 +1100,11 @@ int server_admit(fvm_server* s) {
   return FVM_OK;
 }
 
-// One time step of every active solve: per group, the per-solve dt update
-// and one batched launch per RK stage; groups run on their own streams.
-int server_step(fvm_server* s) {
+// `nsteps` time steps of every active solve with one host synchronisation:
+// per group, the per-solve dt update and one batched launch per RK stage;
+// groups run on their own streams.  A solve that reaches its end time within
+// the run halts on the device and its tiles are skipped until it is retired.
+int server_step(fvm_server* s, int nsteps) {
   std::map<fvm_ctx*, std::vector<int>> ids;
   for (Job* j : s->active) ids[j->group].push_back(j->slot);
   std::vector<int> all_ids;
@@ -1057,30 +1119,27 @@ int server_step(fvm_server* s) {
     fvm_ctx* g = offs[gi].first;
     const int n = (int)ids[g].size();
     if (int rc = sync_desc(g)) return rc;
-    hipLaunchKernelGGL(solve_dt_kernel, dim3((n + 63) / 64), dim3(64), 0, g->stream, s->d_slots,
-                       s->d_ids + offs[gi].second, n);
-    HIP_TRY(hipGetLastError());
     const int nstage = g->rk == RK4 ? 4 : 3;
-    for (int st = 1; st <= nstage; ++st) {
-      StageArgs a{};
-      a.P = g->P;
-      a.dt = 0.0;
-      a.dt_dev = nullptr;
-      a.q0_buf = 0;
-      a.use_acc = g->rk == RK4 ? 1 : 0;
-      if (g->rk == RK4) {
-        const int ib[4] = {0, 1, 2, 1}, ob[4] = {1, 2, 1, 0};
-        const int sk[4] = {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4};
-        a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
-      } else {
-        const int ib[3] = {0, 1, 2}, ob[3] = {1, 2, 0};
-        const int sk[3] = {ST_SSP1, ST_SSP2, ST_SSP3};
-        a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
+    for (int step = 0; step < nsteps; ++step) {
+      hipLaunchKernelGGL(solve_dt_kernel, dim3((n + 63) / 64), dim3(64), 0, g->stream, s->d_slots,
+                         s->d_ids + offs[gi].second, n);
+      HIP_TRY(hipGetLastError());
+      for (int st = 1; st <= nstage; ++st) {
+        StageArgs a{};
+        a.P = g->P;
+        a.use_acc = g->rk == RK4 ? 1 : 0;
+        if (g->rk == RK4) {
+          const int ib[4] = {0, 1, 2, 1}, ob[4] = {1, 2, 1, 0};
+          const int sk[4] = {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4};
+          a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
+        } else {
+          const int ib[3] = {0, 1, 2}, ob[3] = {1, 2, 0};
+          const int sk[3] = {ST_SSP1, ST_SSP2, ST_SSP3};
+          a.in_buf = ib[st - 1]; a.out_buf = ob[st - 1]; a.stage = sk[st - 1];
+        }
+        a.block_smax = st == nstage ? 1 : 0;
+        if (int rc = launch_stage_all(g, a)) return rc;
       }
-      a.norm_part = nullptr;
-      a.smax = nullptr;
-      a.block_smax = st == nstage ? 1 : 0;
-      if (int rc = launch_stage_all(g, a)) return rc;
     }
   }
   for (auto& kv : offs) HIP_TRY(hipStreamSynchronize(kv.first->stream));
@@ -1094,7 +1153,7 @@ int server_retire(fvm_server* s) {
   std::vector<Job*> keep;
   for (Job* j : s->active) {
     const SolveSlot& sl = s->h_slots[j->slot];
-    const bool done = sl.t >= j->req.t_end * (1.0 - 1e-14) || sl.nan;
+    const bool done = sl.halt || sl.nan || !(sl.t < j->req.t_end - kEndTol * std::fabs(j->req.t_end));
     if (!done) { keep.push_back(j); continue; }
     const fvm_request& r = j->req;
     const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
@@ -1176,13 +1235,16 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   return FVM_OK;
 }
 
-int fvm_server_poll(fvm_server* s, int iterations, int* n_pending, int* n_active, int* n_done) {
+int fvm_server_poll(fvm_server* s, int iterations, int steps_per_sync, int* n_pending, int* n_active,
+                    int* n_done) {
   if (!s) return fail(FVM_EINVAL, "null server");
+  if (steps_per_sync < 1 || steps_per_sync > 1024)
+    return fail(FVM_EINVAL, "steps_per_sync %d out of range [1, 1024]", steps_per_sync);
   HIP_TRY(hipSetDevice(s->device));
   for (int it = 0; it < iterations; ++it) {
     if (int rc = server_admit(s)) return rc;
     if (s->active.empty()) break;
-    if (int rc = server_step(s)) return rc;
+    if (int rc = server_step(s, steps_per_sync)) return rc;
     if (int rc = server_retire(s)) return rc;
   }
   int done = 0;

@@ -1,5 +1,7 @@
 // fvm_launch.hpp -- host-side launchers of the stage kernels (one translation
 // unit per reconstruction family so the instantiations build in parallel).
+// `conn`: some block of the launch has a face connected to another block
+// (selects the kernel variant that also writes the neighbour's ghosts).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,21 +14,21 @@ constexpr int kNumTiles3d = 3;
 constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}};
 
 namespace exact {
-hipError_t launch_stage2d_simple(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage2d_muscl(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage2d_weno(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_simple(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_muscl(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_weno(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_muscl(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 
 }  // namespace exact
 namespace fast {
-hipError_t launch_stage2d_simple(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage2d_muscl(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage2d_weno(int flux, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_simple(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_muscl(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_stage3d_weno(int flux, int tile, const StageArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_muscl(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 
 }  // namespace fast
 

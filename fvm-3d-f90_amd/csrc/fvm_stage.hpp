@@ -170,9 +170,9 @@ __device__ __forceinline__ double wave_speed(const double q[NC], const BlockGeom
 
 // Index of (i, j, k) in a block whose geometry is read from the descriptor
 // table (neighbour writes).
-__device__ __forceinline__ int64_t gidx_of(const BlockDesc* __restrict__ tab, int blk, int i,
-                                           int j, int k) {
-  const BlockGeom& g = tab[blk].g;
+template <class Tab>
+__device__ __forceinline__ int64_t gidx_of(Tab tab, int blk, int i, int j, int k) {
+  const auto& g = tab[blk].g;
   return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
 }
 
@@ -181,9 +181,8 @@ __device__ __forceinline__ int64_t gidx_of(const BlockDesc* __restrict__ tab, in
 // vertex n periods away) and, across connected faces, the neighbour block's
 // ghosts it donates to (shared interface vertex: vertex n-1-l of this block is
 // ghost -l of the next; abutting blocks: vertex n-l is ghost -l).
-template <int NC, int NDIR>
-__device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, const BlockDesc* __restrict__ tab,
-                                                  const BlockDesc& B, int out_buf,
+template <int NC, int NDIR, bool CONN, class Tab, class Desc>
+__device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, const Desc& B, int out_buf,
                                                   double* __restrict__ out, int i, int j, int k,
                                                   const double v[NC]) {
   const int64_t base = gidx(g, i, j, k);
@@ -191,6 +190,12 @@ __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, const Bloc
   for (int c = 0; c < NC; ++c) out[base + c * g.cs] = v[c];
   const int n[3] = {g.nx, g.ny, g.nz};
   const int x[3] = {i, j, k};
+  // only vertices within ng + 1 of a face have ghost images; whole waves of
+  // interior vertices skip the rest
+  bool near = false;
+#pragma unroll
+  for (int d = 0; d < NDIR; ++d) near = near || x[d] <= g.ng || x[d] >= n[d] - 1 - g.ng;
+  if (!near) return;
 #pragma unroll
   for (int d = 0; d < NDIR; ++d) {
     const int64_t stride = d == 0 ? 1 : (d == 1 ? g.px : g.pxy);
@@ -214,35 +219,60 @@ __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, const Bloc
 #pragma unroll
         for (int c = 0; c < NC; ++c) out[o + c * g.cs] = v[c];
       }
-      // neighbour across the low face: its ghost n_nb - 1 + l
-      if (blo == BC_CONN && x[d] == (B.nbr_shared[2 * d] ? l : l - 1)) {
-        const int nb = B.nbr[2 * d];
-        int y[3] = {x[0], x[1], x[2]};
-        const int nn[3] = {tab[nb].g.nx, tab[nb].g.ny, tab[nb].g.nz};
-        y[d] = nn[d] - 1 + l;
-        double* dst = tab[nb].q[out_buf];
-        const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
-        const int64_t cs = tab[nb].g.cs;
+      if constexpr (CONN) {
+        // neighbour across the low face: its ghost n_nb - 1 + l
+        if (blo == BC_CONN && x[d] == (B.nbr_shared[2 * d] ? l : l - 1)) {
+          const int nb = B.nbr[2 * d];
+          int y[3] = {x[0], x[1], x[2]};
+          const int nn[3] = {tab[nb].g.nx, tab[nb].g.ny, tab[nb].g.nz};
+          y[d] = nn[d] - 1 + l;
+          double* dst = tab[nb].q[out_buf];
+          const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
+          const int64_t cs = tab[nb].g.cs;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
-      }
-      // neighbour across the high face: its ghost -l
-      if (bhi == BC_CONN && x[d] == (B.nbr_shared[2 * d + 1] ? n[d] - 1 - l : n[d] - l)) {
-        const int nb = B.nbr[2 * d + 1];
-        int y[3] = {x[0], x[1], x[2]};
-        y[d] = -l;
-        double* dst = tab[nb].q[out_buf];
-        const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
-        const int64_t cs = tab[nb].g.cs;
+          for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
+        }
+        // neighbour across the high face: its ghost -l
+        if (bhi == BC_CONN && x[d] == (B.nbr_shared[2 * d + 1] ? n[d] - 1 - l : n[d] - l)) {
+          const int nb = B.nbr[2 * d + 1];
+          int y[3] = {x[0], x[1], x[2]};
+          y[d] = -l;
+          double* dst = tab[nb].q[out_buf];
+          const int64_t o = gidx_of(tab, nb, y[0], y[1], y[2]);
+          const int64_t cs = tab[nb].g.cs;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
+          for (int c = 0; c < NC; ++c) dst[o + c * cs] = v[c];
+        }
       }
     }
   }
 }
 
 // Block of a tile in the launch's tile list (uniform per workgroup).
-__device__ __forceinline__ int block_of_tile(const BlockDesc* __restrict__ tab, int nblocks, int tile) {
+// The descriptor table is read through the constant address space: the
+// stage writes only state arrays, never the table, and loads marked
+// invariant let the compiler re-load descriptor fields instead of keeping
+// them live in registers across the k-march.
+typedef const __attribute__((address_space(4))) BlockDesc* DescTable;
+__device__ __forceinline__ DescTable as_const_table(const BlockDesc* p) { return (DescTable)p; }
+
+// Member-wise copy out of the constant address space.
+template <class G>
+__device__ __forceinline__ BlockGeom load_geom(const G& s) {
+  BlockGeom g;
+  g.nx = s.nx; g.ny = s.ny; g.nz = s.nz;
+  g.ox = s.ox; g.oy = s.oy; g.oz = s.oz;
+  g.px = s.px; g.pxy = s.pxy; g.cs = s.cs;
+  g.dx = s.dx; g.dy = s.dy; g.dz = s.dz;
+  g.rdx = s.rdx; g.rdy = s.rdy; g.rdz = s.rdz;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) g.bc[f] = s.bc[f];
+  g.ng = s.ng;
+  return g;
+}
+
+template <class Tab>
+__device__ __forceinline__ int block_of_tile(Tab tab, int nblocks, int tile) {
   int b = 0;
   while (b + 1 < nblocks && tab[b + 1].tile_base <= tile) ++b;
   return b;
@@ -285,7 +315,7 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
 // its column, reconstructs its vertex along i, and for t in 1..62 updates
 // cell c.  A workgroup loops over tiles of the launch's tile list.
 // ============================================================================
-template <class RC, int FLUX>
+template <class RC, int FLUX, bool CONN>
 __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
   constexpr int NC = 4, TX = 62, NT = 64;
   constexpr int RV = RC::RV, H = RC::H;
@@ -301,12 +331,15 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
   const int t = threadIdx.x;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
+  if (a.halt && *a.halt) return;
+  const DescTable tab = as_const_table(a.blocks);
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
-    const int bi = block_of_tile(a.blocks, a.nblocks, tile);
-    const BlockDesc& B = a.blocks[bi];
-    const BlockGeom g = B.g;
-    const double dt = B.dt_dev ? *B.dt_dev : (a.dt_dev ? *a.dt_dev : a.dt);
+    const int bi = block_of_tile(tab, a.nblocks, tile);
+    const auto& B = tab[bi];
+    if (B.halt && *B.halt) continue;   // uniform over the workgroup
+    const BlockGeom g = load_geom(B.g);
+    const double dt = *B.dt_dev;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
@@ -416,7 +449,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
           out[m] = rk_apply(a.stage, kc, q0, qin[ci + m * g.cs], accp, r);
           nrm[m] += r * r;
         }
-        store_with_ghosts<NC, 2>(g, a.blocks, B, a.out_buf, qout, c, j, 0, out);
+        store_with_ghosts<NC, 2, CONN>(g, tab, B, a.out_buf, qout, c, j, 0, out);
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
       }
 #pragma unroll
@@ -459,7 +492,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 #ifndef FVM_WPE
 #define FVM_WPE 2
 #endif
-template <class RC, int FLUX, int TX, int TY>
+template <class RC, int FLUX, int TX, int TY, bool CONN>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FVM_WPE)))
 void stage3d_kernel(StageArgs a) {
   constexpr int NC = 5, NT = TX * TY;
@@ -488,12 +521,15 @@ void stage3d_kernel(StageArgs a) {
   const int tx = t % TX, ty = t / TX;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
+  if (a.halt && *a.halt) return;
+  const DescTable tab = as_const_table(a.blocks);
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
-    const int bi = block_of_tile(a.blocks, a.nblocks, tile);
-    const BlockDesc& B = a.blocks[bi];
-    const BlockGeom g = B.g;
-    const double dt = B.dt_dev ? *B.dt_dev : (a.dt_dev ? *a.dt_dev : a.dt);
+    const int bi = block_of_tile(tab, a.nblocks, tile);
+    const auto& B = tab[bi];
+    if (B.halt && *B.halt) continue;   // uniform over the workgroup
+    const BlockGeom g = load_geom(B.g);
+    const double dt = *B.dt_dev;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
@@ -704,7 +740,7 @@ void stage3d_kernel(StageArgs a) {
           out[m] = rk_apply(a.stage, kc, q0, qin[cidx + m * g.cs], accp, rr);
           nrm[m] += rr * rr;
         }
-        store_with_ghosts<NC, 3>(g, a.blocks, B, a.out_buf, qout, c, r, k, out);
+        store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
       }
 #pragma unroll

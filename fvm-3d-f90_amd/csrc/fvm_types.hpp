@@ -46,8 +46,10 @@ struct BlockDesc {
   int nbr_shared[6];     // 1: shared interface vertex, 0: abutting
   int tiles_x, tiles_y, tiles_z, chunk;
   int tile_base;         // first tile of this block in the launch's tile list
-  const double* dt_dev;            // per-block dt (independent solves), or null
+  const double* dt_dev;            // the block's dt for this step (device memory: the
+                                   // context's time slot, or the solve's own in serving)
   unsigned long long* smax;        // per-block CFL max accumulator, or null
+  const int* halt;                 // per-block: if set, the block's tiles are skipped
 };
 
 // RK stage kinds: the combine each launch applies
@@ -68,13 +70,24 @@ struct StageArgs {
                                   // a workgroup loops over them
   int in_buf, q0_buf, out_buf;    // state array index (BlockDesc::q) per role
   int use_acc;                    // RK4: accumulator q[3]
-  const double* dt_dev;           // if non-null, dt is read here (CFL mode)
-  double dt;
   Params P;
   int stage;
   double* norm_part;              // per-tile sums of rhs^2 (NC each), or null
   unsigned long long* smax;       // CFL wave-speed max of the output, or null
   int block_smax;                 // 1: accumulate into each block's own smax instead
+  const int* halt;                // if non-null and set, the launch does nothing (run ended)
+};
+
+// Time-stepping state of one solve, kept on the device so a run of steps
+// needs no host round trip: dt = min(cfl / smax, t_end - t), t += dt, and
+// halt once t reaches t_end.
+struct SolveSlot {
+  double t, t_end, cfl, dt;
+  unsigned long long smax;        // CFL wave-speed max (bits of a non-negative double)
+  int steps;
+  int nan;                        // dt was NaN, zero or infinite
+  int halt;                       // t reached t_end: later steps are skipped
+  int pad;
 };
 
 }  // namespace fvm

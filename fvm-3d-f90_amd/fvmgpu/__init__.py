@@ -41,7 +41,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
            "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
-           "fvm_server_result")
+           "fvm_server_result", "fvm_set_time", "fvm_time")
 
 
 class FvmError(RuntimeError):
@@ -92,10 +92,13 @@ def load_library(path=LIB_PATH):
         "fvm_autotune": ([vp, ci], ci),
         "fvm_block_connect": ([vp, ci, vp, ci], ci),
         "fvm_set_launch_mode": ([vp, ci], ci),
+        "fvm_set_time": ([vp, cd, cd], ci),
+        "fvm_time": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
-        "fvm_server_poll": ([vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+        "fvm_server_poll": ([vp, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)],
+                            ci),
         "fvm_server_result": ([vp, i64, vp, ctypes.POINTER(cd)], ci),
         "fvm_block_launch_shape": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                     ctypes.POINTER(ci)], ci),
@@ -218,6 +221,17 @@ class Context:
         _check(_lib.fvm_step(self._h, int(nsteps), float(dt), float(cfl), ctypes.byref(out)))
         return out.value
 
+    def set_time(self, t=0.0, t_end=0.0):
+        """Current time and end time (t_end <= t: none).  CFL steps clip to
+        t_end and skip steps past it on the device."""
+        _check(_lib.fvm_set_time(self._h, float(t), float(t_end)))
+
+    def time(self):
+        """(time, steps run) after the last step."""
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        _check(_lib.fvm_time(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
     def norms(self):
         out = (ctypes.c_double * 6)()
         flag = ctypes.c_int(0)
@@ -275,10 +289,11 @@ class Server:
         self.shapes[rid.value] = q0.shape
         return rid.value
 
-    def poll(self, iterations=1):
+    def poll(self, iterations=1, steps_per_sync=1):
+        """Scheduler iterations; returns (pending, active, done) counts."""
         p, a, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(_lib.fvm_server_poll(self._h, int(iterations), ctypes.byref(p), ctypes.byref(a),
-                                    ctypes.byref(d)))
+        _check(_lib.fvm_server_poll(self._h, int(iterations), int(steps_per_sync), ctypes.byref(p),
+                                    ctypes.byref(a), ctypes.byref(d)))
         return p.value, a.value, d.value
 
     def result(self, rid):

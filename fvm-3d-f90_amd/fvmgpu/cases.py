@@ -134,3 +134,54 @@ def smooth_3d(shape, seed=0, amp=0.2):
     p[..., 3] = 0.25 * np.sin(2 * np.pi * x + ph[4])
     p[..., 4] = 1.0 + amp * np.cos(2 * np.pi * (x + y + z) + ph[5])
     return p
+
+
+def blob(x, y, z, seed=0, nblob=8, amp=0.2, umax=0.5):
+    """Density-blob state on the unit cube at the coordinate vectors x, y, z:
+    rho = 1 + amp * sum of nblob Gaussians (centres U[0.15, 0.85]^3, sigma
+    U[0.05, 0.2]), p = 1, and a smooth divergence-free-ish velocity built from
+    a few sine modes scaled so |u| < umax.  Returns (nk, nj, ni, 5)."""
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(0.15, 0.85, (nblob, 3))
+    sig = rng.uniform(0.05, 0.2, nblob)
+    ph = rng.uniform(0.0, 2.0 * np.pi, 6)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    rho = np.ones(X.shape)
+    for (cx, cy, cz), s in zip(cen, sig):
+        r2 = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2
+        rho += amp * np.exp(-0.5 * r2 / (s * s))
+    a = umax / np.sqrt(3.0) * 0.99
+    p = np.empty(X.shape + (5,))
+    p[..., 0] = rho
+    p[..., 1] = a * np.sin(2 * np.pi * Y + ph[0]) * np.cos(2 * np.pi * Z + ph[1])
+    p[..., 2] = a * np.sin(2 * np.pi * Z + ph[2]) * np.cos(2 * np.pi * X + ph[3])
+    p[..., 3] = a * np.sin(2 * np.pi * X + ph[4]) * np.cos(2 * np.pi * Y + ph[5])
+    p[..., 4] = 1.0
+    return p
+
+
+def blob_blocks(seed=0, layout=(3, 3, 3), lo=32, hi=96):
+    """Conforming multi-block layout of the unit cube: block (a, b, c) has
+    nx[a] x ny[b] x nz[c] vertices, sizes drawn from U{lo..hi} with the seed;
+    neighbouring blocks share their interface vertex plane, one spacing per
+    direction.  Returns (sizes per direction, global vertex offsets per
+    direction, spacing, per-block primitive states keyed (a, b, c))."""
+    rng = np.random.default_rng(seed)
+    sizes = [list(int(v) for v in rng.integers(lo, hi + 1, layout[d])) for d in range(3)]
+    offs, coords, spacing = [], [], []
+    for d in range(3):
+        o = np.concatenate(([0], np.cumsum(np.array(sizes[d]) - 1)[:-1]))
+        ntot = int(sum(sizes[d]) - (layout[d] - 1))
+        h = 1.0 / (ntot - 1)
+        offs.append([int(v) for v in o])
+        coords.append(h * np.arange(ntot, dtype=np.float64))
+        spacing.append(h)
+    full = blob(coords[0], coords[1], coords[2], seed=seed)
+    states = {}
+    for a in range(layout[0]):
+        for b in range(layout[1]):
+            for c in range(layout[2]):
+                x0, y0, z0 = offs[0][a], offs[1][b], offs[2][c]
+                states[(a, b, c)] = full[z0:z0 + sizes[2][c], y0:y0 + sizes[1][b],
+                                         x0:x0 + sizes[0][a]].copy()
+    return sizes, offs, tuple(spacing), states

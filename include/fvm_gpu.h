@@ -151,6 +151,15 @@ int fvm_download(fvm_block *blk, double *q, int host_ng);
  * last stage of the last step are kept for fvm_norms. */
 int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
 
+/* Solution time and end time (the time loop's time/total_time,
+ * src/process/solver_sub.f90:196-206).  With an end time, CFL steps clip
+ * the last dt to t_end - t and the steps of an fvm_step call that would
+ * start at or past t_end are skipped on the device (no host round trip per
+ * step).  t_end <= t or infinite: no end.  fvm_time reports the time after
+ * the last step and the number of steps run since the context was made. */
+int fvm_set_time(fvm_ctx *ctx, double t, double t_end);
+int fvm_time(fvm_ctx *ctx, double *t, int64_t *steps);
+
 /* Residual norms of the last stage of the last fvm_step call, as
  * ClassIteration%compute_residual computes them
  * (src/solve/class_iteration.f90:210-248): out[0..4] per equation,
@@ -199,9 +208,12 @@ typedef struct fvm_request {
 int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_server **out);
 int fvm_server_destroy(fvm_server *srv);
 int fvm_server_submit(fvm_server *srv, const fvm_request *req, int64_t *id);
-/* Run up to `iterations` scheduler iterations (admit, one step, retire);
- * returns early when nothing is active. */
-int fvm_server_poll(fvm_server *srv, int iterations, int *n_pending, int *n_active, int *n_done);
+/* Run up to `iterations` scheduler iterations; each admits pending
+ * requests, advances every active solve by `steps_per_sync` steps with one
+ * host synchronisation (a solve reaching t_end halts on the device), and
+ * retires finished solves.  Returns early when nothing is active. */
+int fvm_server_poll(fvm_server *srv, int iterations, int steps_per_sync, int *n_pending,
+                    int *n_active, int *n_done);
 /* Final state (same layout as q0) and info[8] = {submit, admit, done (s since
  * server creation), steps, cell-updates, final time, status (2 done, -1
  * diverged), device bytes}. */

@@ -28,12 +28,12 @@ def mixed_requests():
     return reqs
 
 
-def run(reqs, max_active=64):
+def run(reqs, max_active=64, steps_per_sync=1):
     out = []
     with fvmgpu.Server(max_active=max_active) as srv:
         ids = [srv.submit(**r) for r in reqs]
         while True:
-            pend, act, done = srv.poll(50)
+            pend, act, done = srv.poll(50, steps_per_sync)
             if done == len(ids):
                 break
         for i in ids:
@@ -99,3 +99,40 @@ def test_served_equals_plain_context():
         assert info["steps"] == steps
         assert info["t_final"] == t
         assert np.array_equal(q, qp)
+
+
+def test_steps_per_sync():
+    """Several steps per host synchronisation: solves that finish mid-run
+    halt on the device; results are unchanged."""
+    reqs = mixed_requests()
+    one = run(reqs)
+    many = run(reqs, steps_per_sync=7)
+    for (qa, ia), (qb, ib) in zip(one, many):
+        assert ia["steps"] == ib["steps"]
+        assert np.array_equal(qa, qb)
+
+
+def test_plain_end_time():
+    """A plain context with an end time clips the last CFL step and skips
+    the steps past t_end on the device: same result as the served solve."""
+    for r in mixed_requests()[1::2]:
+        (q, info), = run([r])
+        q0 = r["q0"]
+        _, nk, nj, ni = q0.shape
+        with fvmgpu.Context(0) as ctx:
+            ctx.scheme(recon=r["recon"], limiter=r["limiter"], flux=r["flux"], rk=r["rk"],
+                       ref_compat=r.get("ref_compat", 1))
+            b = ctx.block((ni, nj, nk), r["spacing"])
+            if any(r.get("periodic", ())):
+                b.set_periodic([d for d in range(3) if r["periodic"][d]])
+            b.upload(q0)
+            ctx.set_time(0.0, r["t_end"])
+            for _ in range(1000):
+                ctx.step(5, cfl=r["cfl"])
+                t, steps = ctx.time()
+                if t >= r["t_end"] * (1 - 1e-14):
+                    break
+            norms, nan = ctx.norms()
+            assert steps == info["steps"] and t == info["t_final"]
+            assert np.array_equal(b.download(), q)
+            assert not nan and norms[5] > 0.0
