@@ -27,6 +27,11 @@ Other workloads (--workload):
               block-to-block ghost exchange on the device; one batched
               (persistent when --persistent) launch per stage against one
               launch per block.
+  tgv192      3-D Navier-Stokes Taylor-Green vortex, 192^3 vertices on the
+              periodic [0, 2 pi)^3 box, Re 1600, Ma 0.1, Pr 0.71, Sutherland
+              viscosity, WENO5-JS + Van Leer FVS, central viscous fluxes,
+              RK4, CFL 0.5; reports the kinetic-energy decay rate over the
+              timed steps (--t-end 10 runs the whole case).
   serving     Poisson arrivals (50 / s, seeded) of independent solves:
               2-D Riemann 64^2..512^2 (the reference's quadrant states)
               and 3-D vortex / blob 32^3..128^3
@@ -58,7 +63,9 @@ def parse():
                     help="timed steps (vortex128: 1000, blob27: 200; riemann400 runs to t_end)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="vortex128",
-                    choices=("vortex128", "riemann400", "blob27", "serving"))
+                    choices=("vortex128", "riemann400", "blob27", "tgv192", "serving"))
+    ap.add_argument("--t-end", type=float, default=None,
+                    help="tgv192: run to this time instead of --steps steps")
     ap.add_argument("--n", type=int, default=None, help="vertices per direction")
     ap.add_argument("--cfl", type=float, default=None)
     ap.add_argument("--seed", type=int, default=None)
@@ -351,6 +358,84 @@ def bench_blob27(args):
 
 
 # ---------------------------------------------------------------------------
+# tgv192
+# ---------------------------------------------------------------------------
+def bench_tgv(args):
+    import fvmgpu
+    from fvmgpu import cases
+    n = args.n or 192
+    cfl = args.cfl or 0.5
+    steps = args.steps or 100
+    p, h = cases.tgv(n)
+    visc = cases.tgv_params(re=1600.0, ma=0.1, pr=0.71)
+    q = cases.to_conserved(p)
+    ctx = fvmgpu.Context(0)
+    ctx.scheme(recon=fvmgpu.WENO, limiter=fvmgpu.LIM_NONE, flux=fvmgpu.VANLEER, rk=fvmgpu.RK4,
+               kappa=0.33333, entropy_fix=0.2, gamma=1.4, ref_compat=0)
+    ctx.numerics(numerics_code(fvmgpu, args))
+    ctx.viscous(*visc)
+    blk = ctx.block((n, n, n), (h, h, h))
+    blk.set_periodic()
+    blk.upload(q)
+    if not args.no_tune:
+        ctx.autotune(reps=2)
+    tile = blk.launch_shape()
+    if args.warmup:
+        ctx.step(args.warmup, cfl=cfl)
+    ctx.sync()
+    t_a, steps_a = ctx.time()
+    ek_a = cases.kinetic_energy(blk.download())
+    if args.t_end:
+        ctx.set_time(t_a, args.t_end)
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    if args.t_end:
+        while True:
+            dt = ctx.step(64, cfl=cfl)
+            t, _ = ctx.time()
+            if t >= args.t_end * (1 - 1e-14):
+                break
+    else:
+        ctx.step(steps, cfl=cfl)
+    ctx.sync()
+    sec = time.perf_counter() - t0
+    stage_ms, launches, _ = ctx.timing_read()
+    ctx.timing(False)
+    norms, nan = ctx.norms()
+    if nan:
+        raise SystemExit(f"bench.py: residual NaN or diverged, norms {norms}")
+    t_b, steps_b = ctx.time()
+    steps = steps_b - steps_a
+    ek_b = cases.kinetic_energy(blk.download())
+    ncell = float(n) ** 3
+    value = ncell * 4 * steps / sec
+    roof = roofline(stage_ms, steps * 4, ncell, B_ALG["rk4_3d"], None,
+                    f"fvm::{args.numerics}::viscous3d_kernel<32, 8> + stage3d_kernel<ReconWeno, "
+                    f"301, {tile[0]}, {tile[1]}> per stage")
+    config = {"workload": f"taylor-green-3d {n}^3 periodic [0,2pi)^3, Re 1600, Ma 0.1, Pr 0.71, "
+                          "Sutherland, WENO5-JS + Van Leer FVS + central viscous, RK4, "
+                          f"CFL {cfl}, FP64",
+              "grid": [n, n, n], "stages_per_step": 4, "numerics": args.numerics,
+              "tile_xy_chunk": list(tile), "t_start": t_a, "t_final": t_b,
+              "kinetic_energy_start": ek_a, "kinetic_energy_final": ek_b,
+              "kinetic_energy_decay_rate": (ek_a - ek_b) / (t_b - t_a),
+              "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)}
+    cpu = None
+    if not args.no_cpu:
+        O = oracle()
+        t0 = time.perf_counter()
+        O.oracle_solve(p, 3, (h, h, h), recon=O.WENO, limiter=O.LIM_NONE, flux=O.VANLEER,
+                       rk=O.RK4, dt=0.0, cfl=cfl, nsteps=1, p_from_q=1, bc=(O.BC_PERIODIC,) * 6,
+                       weno_textbook=1, visc=visc)
+        csec = time.perf_counter() - t0
+        cells = ncell * 4
+        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"1 RK4 step of the {n}^3 TGV workload ({cells:.3g} cell-updates, "
+                         f"{csec:.1f} s, oracle/fvm_oracle.c)"}
+    return line(args, value, steps, sec, config, roof, cpu)
+
+
+# ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
 # end times per case kind: long enough that the offered load (50 / s)
@@ -502,7 +587,7 @@ def main():
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         sys.exit(2)
     fn = {"vortex128": bench_vortex, "riemann400": bench_riemann, "blob27": bench_blob27,
-          "serving": bench_serving}[args.workload]
+          "tgv192": bench_tgv, "serving": bench_serving}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

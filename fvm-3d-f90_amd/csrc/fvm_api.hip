@@ -66,6 +66,7 @@ struct fvm_block {
   const double* dt_dev = nullptr;
   unsigned long long* smax_dev = nullptr;
   const int* halt_dev = nullptr;
+  double* rv = nullptr;                                  // viscous residual (nc - 1 components)
 };
 
 struct fvm_ctx {
@@ -84,6 +85,8 @@ struct fvm_ctx {
   bool ghosts_dirty = true;              // u^n ghost layers must be refilled
   int total_tiles = 0;
   bool has_conn = false;                 // some block face is connected to another block
+  // central viscous terms (fvm_viscous_set); mu_ref <= 0: inviscid
+  double visc_mu_ref = 0.0, visc_t_ref = 1.0, visc_s = 0.0, visc_pr = 0.72, visc_r = 1.0;
   SolveSlot* d_slot = nullptr;            // device time-stepping state (CFL mode)
   SolveSlot h_slot{};                     // host staging of d_slot
   double* d_dt = nullptr;                 // &d_slot->dt: CFL dt of the current step
@@ -221,6 +224,8 @@ __global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
 // ============================================================================
 namespace {
 
+int alloc_rv(fvm_block* b);
+
 int check_ctx(fvm_ctx* ctx) {
   if (!ctx) return fail(FVM_EINVAL, "null context");
   return FVM_OK;
@@ -276,6 +281,18 @@ void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   if (b->ctx) b->ctx->desc_dirty = true;
 }
 
+// Viscous residual buffer of a block: nc - 1 components (momentum, energy).
+int alloc_rv(fvm_block* b) {
+  if (b->rv) return FVM_OK;
+  const size_t bytes = (size_t)(b->nc - 1) * b->g.cs * sizeof(double);
+  if (hipMalloc(&b->rv, bytes) != hipSuccess) {
+    b->rv = nullptr;
+    return fail(FVM_ENOMEM, "cannot allocate %zu bytes of viscous residual", bytes);
+  }
+  HIP_TRY(hipMemsetAsync(b->rv, 0, bytes, b->ctx->stream));
+  return FVM_OK;
+}
+
 // Upload the descriptor table (blocks in context order, tiles numbered
 // block after block).
 int sync_desc(fvm_ctx* ctx) {
@@ -295,6 +312,7 @@ int sync_desc(fvm_ctx* ctx) {
     d.dt_dev = b->dt_dev ? b->dt_dev : ctx->d_dt;
     d.smax = b->smax_dev;
     d.halt = b->halt_dev;
+    d.rv = ctx->visc_mu_ref > 0.0 ? b->rv : nullptr;
     b->tile_base = base;
     base += b->ntiles;
   }
@@ -402,7 +420,33 @@ int grid_for(fvm_ctx* ctx, int ntiles) {
 
 // One RK stage over every block: a single launch over the whole tile list
 // (batched or persistent), or one launch per block.
+int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
+  ViscArgs v{};
+  v.V.mu_ref = ctx->visc_mu_ref;
+  v.V.t_ref = ctx->visc_t_ref;
+  v.V.s = ctx->visc_s;
+  v.V.r_gas = ctx->visc_r;
+  v.V.gamma = ctx->P.gamma;
+  v.V.cpr = ctx->P.gamma * ctx->visc_r / ((ctx->P.gamma - 1.0) * ctx->visc_pr);
+  for (fvm_block* b : ctx->blocks) {
+    v.g = b->g;
+    v.q = b->q[in_buf];
+    v.rv = b->rv;
+    v.tiles_x = (b->g.nx + kViscTile[0] - 1) / kViscTile[0];
+    v.tiles_y = (b->g.ny + kViscTile[1] - 1) / kViscTile[1];
+    v.chunk = 16;
+    const int tz = b->dim == 3 ? (b->g.nz + v.chunk - 1) / v.chunk : 1;
+    const int grid = v.tiles_x * v.tiles_y * tz;
+    const hipError_t e = ctx->fast ? fvm::fast::launch_viscous(b->dim, v, grid, ctx->stream)
+                                   : fvm::exact::launch_viscous(b->dim, v, grid, ctx->stream);
+    if (e != hipSuccess) return fail(FVM_EHIP, "viscous launch: %s", hipGetErrorString(e));
+  }
+  return FVM_OK;
+}
+
 int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
+  if (ctx->visc_mu_ref > 0.0)
+    if (int rc = launch_viscous_all(ctx, a.in_buf)) return rc;
   a.blocks = ctx->d_desc;
   a.nblocks = (int)ctx->blocks.size();
   if (ctx->launch_mode != FVM_LAUNCH_PER_BLOCK && batchable(ctx)) {
@@ -456,6 +500,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (fvm_block* b : ctx->blocks) {
     for (double* p : b->q) (void)hipFree(p);
+    if (b->rv) (void)hipFree(b->rv);
     delete b;
   }
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
@@ -549,6 +594,8 @@ int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx
     }
     HIP_TRY(hipMemsetAsync(b->q[k], 0, bytes, ctx->stream));
   }
+  if (ctx->visc_mu_ref > 0.0)
+    if (int rc = alloc_rv(b)) { for (double* p : b->q) (void)hipFree(p); delete b; return rc; }
   plan_launch(b);
   ctx->blocks.push_back(b);
   ctx->desc_dirty = true;
@@ -570,6 +617,7 @@ int fvm_block_destroy(fvm_block* b) {
       else if (o->nbr[f] > idx) --o->nbr[f];
     }
   for (double* p : b->q) (void)hipFree(p);
+  if (b->rv) (void)hipFree(b->rv);
   ctx->blocks.erase(std::remove(ctx->blocks.begin(), ctx->blocks.end(), b), ctx->blocks.end());
   delete b;
   ctx->desc_dirty = true;
@@ -597,6 +645,8 @@ int fvm_block_set_bc(fvm_block* b, int face, int kind) {
 int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
   if (!a || !b) return fail(FVM_EINVAL, "null block");
   if (a->ctx != b->ctx) return fail(FVM_EINVAL, "blocks belong to different contexts");
+  if (a->ctx->visc_mu_ref > 0.0)
+    return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
   if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
   if (a->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
   const int d = face / 2, bface = face ^ 1;
@@ -856,6 +906,33 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   ctx->nan_flag = nan;
   ctx->have_norms = true;
   if (cfl_mode && ctx->h_slot.nan) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
+  return FVM_OK;
+}
+
+int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland_s, double prandtl,
+                    double r_gas) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (mu_ref <= 0.0) {
+    ctx->visc_mu_ref = 0.0;
+    ctx->desc_dirty = true;
+    return FVM_OK;
+  }
+  if (!(t_ref > 0.0) || !(prandtl > 0.0) || !(r_gas > 0.0) || !(sutherland_s >= 0.0) ||
+      !std::isfinite(mu_ref))
+    return fail(FVM_EINVAL, "viscous parameters need t_ref, prandtl, r_gas > 0 and s >= 0");
+  for (fvm_block* b : ctx->blocks)
+    for (int f = 0; f < 6; ++f)
+      if (b->g.bc[f] == BC_CONN)
+        return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
+  if (int rc = set_device(ctx)) return rc;
+  for (fvm_block* b : ctx->blocks)
+    if (int rc = alloc_rv(b)) return rc;
+  ctx->visc_mu_ref = mu_ref;
+  ctx->visc_t_ref = t_ref;
+  ctx->visc_s = sutherland_s;
+  ctx->visc_pr = prandtl;
+  ctx->visc_r = r_gas;
+  ctx->desc_dirty = true;
   return FVM_OK;
 }
 

@@ -12,6 +12,8 @@ namespace fvm {
 // 3-D tile shapes the library is built with: id -> (TX, TY)
 constexpr int kNumTiles3d = 3;
 constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}};
+// viscous-residual tile (TX, TY); 3-D launches march over k-chunks
+constexpr int kViscTile[2] = {32, 8};
 
 namespace exact {
 hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
@@ -20,6 +22,7 @@ hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid
 hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_viscous(int dim, const ViscArgs& a, int grid, hipStream_t s);
 
 }  // namespace exact
 namespace fast {
@@ -29,6 +32,7 @@ hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid
 hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_viscous(int dim, const ViscArgs& a, int grid, hipStream_t s);
 
 }  // namespace fast
 

@@ -340,6 +340,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
     if (B.halt && *B.halt) continue;   // uniform over the workgroup
     const BlockGeom g = load_geom(B.g);
     const double dt = *B.dt_dev;
+    const double* __restrict__ rvp = B.rv;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
@@ -444,6 +445,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
             r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
             r = r - (Fn[m] - Fc[m]) / g.dy;
           }
+          if (m > 0 && rvp) r = r + rvp[(int64_t)(m - 1) * g.cs + ci];
           const double q0 = q0p[ci + m * g.cs];
           double* accp = acc ? acc + ci + m * g.cs : nullptr;
           out[m] = rk_apply(a.stage, kc, q0, qin[ci + m * g.cs], accp, r);
@@ -530,6 +532,7 @@ void stage3d_kernel(StageArgs a) {
     if (B.halt && *B.halt) continue;   // uniform over the workgroup
     const BlockGeom g = load_geom(B.g);
     const double dt = *B.dt_dev;
+    const double* __restrict__ rvp = B.rv;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
     const double* __restrict__ qin = B.q[a.in_buf];
@@ -735,6 +738,7 @@ void stage3d_kernel(StageArgs a) {
             rr = rr - fyd / g.dy;
             rr = rr - (Fn[m] - Fc[m]) / g.dz;
           }
+          if (m > 0 && rvp) rr = rr + rvp[(int64_t)(m - 1) * g.cs + cidx];
           const double q0 = q0p[cidx + m * g.cs];
           double* accp = acc ? acc + cidx + m * g.cs : nullptr;
           out[m] = rk_apply(a.stage, kc, q0, qin[cidx + m * g.cs], accp, rr);

@@ -50,6 +50,24 @@ struct BlockDesc {
                                    // context's time slot, or the solve's own in serving)
   unsigned long long* smax;        // per-block CFL max accumulator, or null
   const int* halt;                 // per-block: if set, the block's tiles are skipped
+  const double* rv;                // viscous residual of the stage input (NC-1 components:
+                                   // momentum, energy; stride g.cs), or null: inviscid
+};
+
+// Central viscous terms: Sutherland viscosity, T = p / (rho R), k = mu cp/Pr.
+struct ViscParams {
+  double mu_ref, t_ref, s;         // Sutherland: mu_ref (T/t_ref)^1.5 (t_ref + s) / (T + s)
+  double r_gas;
+  double cpr;                      // cp / Pr = gamma R / ((gamma - 1) Pr)
+  double gamma;
+};
+
+struct ViscArgs {
+  BlockGeom g;
+  const double* q;                 // stage input state
+  double* rv;                      // out: viscous residual, NC-1 components
+  int tiles_x, tiles_y, chunk;     // launch decomposition (3-D: k-chunks of `chunk` planes)
+  ViscParams V;
 };
 
 // RK stage kinds: the combine each launch applies

@@ -1,0 +1,222 @@
+// fvm_visc.hpp -- central second-order viscous residual (exact or fast
+// numerics, selected by FVM_FAST as for the stage kernels).
+//
+// The reference reserves viscous_flux_sub (src/solve/class_space_discret.f90:
+// 671-675) and leaves it empty; this is the build's own definition of the
+// term, the same one oracle/fvm_oracle.c restates on the CPU:
+//   - per vertex: u, v, w, T = p / (rho R), mu = Sutherland(T);
+//   - a stencil point outside the block maps into it per direction
+//     (periodic: wrap by the nodal period n; otherwise clamp to the boundary
+//     vertex), so no edge or corner ghost is read;
+//   - face between a and b = a + e_d: means of the two vertices, normal
+//     derivative (phi_b - phi_a) / dr_d, tangential derivatives the mean of
+//     the central differences at a and b, /(4 dr_t);
+//   - tau_ij = mu (d_j u_i + d_i u_j) - 2/3 mu div(u) delta_ij,
+//     F_d = (tau_d1, tau_d2, tau_d3, u_i tau_di + mu cp/Pr dT/dx_d);
+//   - rv = sum_d (F_d(+) - F_d(-)) / dr_d for momentum and energy.
+// The stage kernel adds rv to its inviscid residual.  Each workgroup stages
+// the primitive quantities of its tile (+1 halo) in LDS: a ring of three
+// k-planes in 3-D, one plane in 2-D.
+#pragma once
+#include "fvm_types.hpp"
+
+namespace fvm {
+#if FVM_FAST
+namespace fast {
+#else
+namespace exact {
+#endif
+
+namespace visc {
+
+#if FVM_FAST
+constexpr bool kFastV = true;
+#else
+constexpr bool kFastV = false;
+#endif
+
+// index of a stencil point along one direction: periodic wrap or clamp
+__device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
+  if (x < 0) return lo_bc == BC_PERIODIC ? ((x % n) + n) % n : 0;
+  if (x >= n) return hi_bc == BC_PERIODIC ? x % n : n - 1;
+  return x;
+}
+
+// u, v, w, T, mu of the conserved state at vertex (i, j, k) (already mapped)
+template <int NC>
+__device__ __forceinline__ void prims(const double* __restrict__ q, const BlockGeom& g, int i, int j,
+                                      int k, const ViscParams& V, double out[5]) {
+  const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
+  const double rho = q[c];
+  const double u = q[c + g.cs] / rho, v = q[c + 2 * g.cs] / rho;
+  const double w = NC == 5 ? q[c + 3 * g.cs] / rho : 0.0;
+  const double e = q[c + (NC - 1) * g.cs];
+  // class_flow_field.f90:383-396 operation order
+  const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
+  const double t = p / (rho * V.r_gas);
+  const double tr = t / V.t_ref;
+  out[0] = u;
+  out[1] = v;
+  out[2] = w;
+  out[3] = t;
+  out[4] = V.mu_ref * tr * sqrt(tr) * (V.t_ref + V.s) / (t + V.s);
+}
+
+// Face flux in direction D between vertex A (offset O along D from the
+// centre) and A + e_D.  at(q, dx, dy, dz) reads quantity q (u, v, w, T, mu)
+// at an offset from the centre vertex.
+template <int DIM, int D, int O, class At>
+__device__ __forceinline__ void face(const At& at, const BlockGeom& g, const ViscParams& V,
+                                     double F[4]) {
+  constexpr int a0 = D == 0 ? O : 0, a1 = D == 1 ? O : 0, a2 = D == 2 ? O : 0;
+  constexpr int b0 = a0 + (D == 0), b1 = a1 + (D == 1), b2 = a2 + (D == 2);
+  const double dr[3] = {g.dx, g.dy, g.dz};
+  const double rdr[3] = {g.rdx, g.rdy, g.rdz};
+  double gq[3][4];
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) gq[e][m] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double diff = at(m, b0, b1, b2) - at(m, a0, a1, a2);
+    gq[D][m] = kFastV ? diff * rdr[D] : diff / dr[D];
+  }
+#pragma unroll
+  for (int e = 0; e < DIM; ++e) {
+    if (e == D) continue;
+    const int s0 = e == 0, s1 = e == 1, s2 = e == 2;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double sum = (at(m, a0 + s0, a1 + s1, a2 + s2) - at(m, a0 - s0, a1 - s1, a2 - s2)) +
+                         (at(m, b0 + s0, b1 + s1, b2 + s2) - at(m, b0 - s0, b1 - s1, b2 - s2));
+      gq[e][m] = kFastV ? sum * (0.25 * rdr[e]) : sum / (4.0 * dr[e]);
+    }
+  }
+  const double mu = 0.5 * (at(4, a0, a1, a2) + at(4, b0, b1, b2));
+  const double uf0 = 0.5 * (at(0, a0, a1, a2) + at(0, b0, b1, b2));
+  const double uf1 = 0.5 * (at(1, a0, a1, a2) + at(1, b0, b1, b2));
+  const double uf2 = 0.5 * (at(2, a0, a1, a2) + at(2, b0, b1, b2));
+  const double div = gq[0][0] + gq[1][1] + gq[2][2];
+  double tau[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double t = mu * (gq[i][D] + gq[D][i]);
+    if (i == D) t = t - (2.0 / 3.0) * mu * div;
+    tau[i] = t;
+  }
+  F[0] = tau[0];
+  F[1] = tau[1];
+  F[2] = tau[2];
+  F[3] = uf0 * tau[0] + uf1 * tau[1] + uf2 * tau[2] + mu * V.cpr * gq[D][3];
+}
+
+template <int DIM, class At>
+__device__ __forceinline__ void residual(const At& at, const BlockGeom& g, const ViscParams& V,
+                                         double acc[4]) {
+  double Fp[4], Fm[4];
+  face<DIM, 0, 0>(at, g, V, Fp);
+  face<DIM, 0, -1>(at, g, V, Fm);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) acc[m] = kFastV ? (Fp[m] - Fm[m]) * g.rdx : (Fp[m] - Fm[m]) / g.dx;
+  face<DIM, 1, 0>(at, g, V, Fp);
+  face<DIM, 1, -1>(at, g, V, Fm);
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + (Fp[m] - Fm[m]) / g.dy;
+  if constexpr (DIM == 3) {
+    face<DIM, 2, 0>(at, g, V, Fp);
+    face<DIM, 2, -1>(at, g, V, Fm);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdz : acc[m] + (Fp[m] - Fm[m]) / g.dz;
+  }
+}
+
+}  // namespace visc
+
+// 3-D: tile TX x TY columns marching over `chunk` k-planes, three-plane ring.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
+  constexpr int NT = TX * TY, W = TX + 2, HH = TY + 2, PN = W * HH;
+  __shared__ double S[3][5][PN];
+  const BlockGeom& g = a.g;
+  const ViscParams& V = a.V;
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  const int tiles_xy = a.tiles_x * a.tiles_y;
+  const int tk = blockIdx.x / tiles_xy, rem = blockIdx.x % tiles_xy;
+  const int i0 = (rem % a.tiles_x) * TX, j0 = (rem / a.tiles_x) * TY;
+  const int k0 = tk * a.chunk, k1 = min(k0 + a.chunk, g.nz);
+  const int i = i0 + tx, j = j0 + ty;
+
+  auto load_plane = [&](int k, int slot) {
+    const int mk = visc::vmap(k, g.nz, g.bc[4], g.bc[5]);
+    for (int e = t; e < PN; e += NT) {
+      const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
+      const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
+      double v[5];
+      visc::prims<5>(a.q, g, mi, mj, mk, V, v);
+#pragma unroll
+      for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
+    }
+  };
+  load_plane(k0 - 1, 0);
+  load_plane(k0, 1);
+  int sm = 0, sc = 1, sp = 2;   // ring slots of k-1, k, k+1
+  for (int k = k0; k < k1; ++k) {
+    load_plane(k + 1, sp);
+    __syncthreads();
+    if (i < g.nx && j < g.ny) {
+      const int ctr = (ty + 1) * W + (tx + 1);
+      auto at = [&](int m, int dx, int dy, int dz) -> double {
+        const int slot = dz < 0 ? sm : (dz > 0 ? sp : sc);
+        return S[slot][m][ctr + dy * W + dx];
+      };
+      double acc[4];
+      visc::residual<3>(at, g, V, acc);
+      const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a.rv[(int64_t)m * g.cs + c] = acc[m];
+    }
+    __syncthreads();
+    const int s0 = sm;
+    sm = sc;
+    sc = sp;
+    sp = s0;
+  }
+}
+
+// 2-D: tile TX x TY vertices, one plane.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
+  constexpr int NT = TX * TY, W = TX + 2, HH = TY + 2, PN = W * HH;
+  __shared__ double S[5][PN];
+  const BlockGeom& g = a.g;
+  const ViscParams& V = a.V;
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  const int i0 = (blockIdx.x % a.tiles_x) * TX, j0 = (blockIdx.x / a.tiles_x) * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  for (int e = t; e < PN; e += NT) {
+    const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
+    const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
+    double v[5];
+    visc::prims<4>(a.q, g, mi, mj, 0, V, v);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) S[m][e] = v[m];
+  }
+  __syncthreads();
+  if (i < g.nx && j < g.ny) {
+    const int ctr = (ty + 1) * W + (tx + 1);
+    auto at = [&](int m, int dx, int dy, int) -> double { return S[m][ctr + dy * W + dx]; };
+    double acc[4];
+    visc::residual<2>(at, g, V, acc);
+    const int64_t c = (int64_t)(j + g.oy) * g.px + (i + g.ox);
+    // 2-D state: rho, rho u, rho v, E -> residual components rho u, rho v, E
+    a.rv[c] = acc[0];
+    a.rv[g.cs + c] = acc[1];
+    a.rv[2 * g.cs + c] = acc[3];
+  }
+}
+
+}  // namespace exact / fast
+}  // namespace fvm

@@ -41,7 +41,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
            "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
-           "fvm_server_result", "fvm_set_time", "fvm_time")
+           "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set")
 
 
 class FvmError(RuntimeError):
@@ -93,6 +93,7 @@ def load_library(path=LIB_PATH):
         "fvm_block_connect": ([vp, ci, vp, ci], ci),
         "fvm_set_launch_mode": ([vp, ci], ci),
         "fvm_set_time": ([vp, cd, cd], ci),
+        "fvm_viscous_set": ([vp, cd, cd, cd, cd, cd], ci),
         "fvm_time": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
@@ -220,6 +221,11 @@ class Context:
         out = ctypes.c_double(0.0)
         _check(_lib.fvm_step(self._h, int(nsteps), float(dt), float(cfl), ctypes.byref(out)))
         return out.value
+
+    def viscous(self, mu_ref, t_ref=1.0, sutherland_s=0.0, prandtl=0.72, r_gas=1.0):
+        """Central viscous fluxes with Sutherland's law; mu_ref <= 0: off."""
+        _check(_lib.fvm_viscous_set(self._h, float(mu_ref), float(t_ref), float(sutherland_s),
+                                    float(prandtl), float(r_gas)))
 
     def set_time(self, t=0.0, t_end=0.0):
         """Current time and end time (t_end <= t: none).  CFL steps clip to

@@ -185,3 +185,42 @@ def blob_blocks(seed=0, layout=(3, 3, 3), lo=32, hi=96):
                 states[(a, b, c)] = full[z0:z0 + sizes[2][c], y0:y0 + sizes[1][b],
                                          x0:x0 + sizes[0][a]].copy()
     return sizes, offs, tuple(spacing), states
+
+
+# Taylor-Green vortex (Brachet et al.; the laminar-transition benchmark
+# case): non-dimensional with rho0 = V0 = L = 1, T0 = 1, gas constant
+# R = 1 / (gamma Ma^2) so p0 = rho0 R T0 and the Mach number is Ma;
+# mu_ref = rho0 V0 L / Re at T0; Sutherland constant 110.4 K over 273.15 K.
+TGV_SUTHERLAND_S = 110.4 / 273.15
+
+
+def tgv_params(re=1600.0, ma=0.1, pr=0.71, gamma=1.4):
+    """(mu_ref, t_ref, sutherland_s, prandtl, r_gas) of the TGV case."""
+    return (1.0 / re, 1.0, TGV_SUTHERLAND_S, pr, 1.0 / (gamma * ma * ma))
+
+
+def tgv(n, ma=0.1, gamma=1.4):
+    """TGV on the periodic box [0, 2 pi)^3, n^3 vertices x = i h, h = 2 pi / n
+    (vertex n is vertex 0):
+      u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+      p = p0 + (cos 2x + cos 2y)(cos 2z + 2) / 16,  rho = p / (R T0).
+    Returns (p (n, n, n, 5), h)."""
+    r = 1.0 / (gamma * ma * ma)
+    h = 2.0 * np.pi / n
+    x = h * np.arange(n, dtype=np.float64)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    pr = r + (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0) / 16.0
+    p = np.empty(X.shape + (5,))
+    p[..., 0] = pr / r
+    p[..., 1] = np.sin(X) * np.cos(Y) * np.cos(Z)
+    p[..., 2] = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    p[..., 3] = 0.0
+    p[..., 4] = pr
+    return p, h
+
+
+def kinetic_energy(q):
+    """Volume-mean kinetic energy 0.5 rho |u|^2 / rho0 of a conserved state
+    (5, nk, nj, ni) on a periodic box (each vertex one cell)."""
+    rho = q[0]
+    return float(np.mean(0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / rho))

@@ -151,6 +151,15 @@ int fvm_download(fvm_block *blk, double *q, int host_ng);
  * last stage of the last step are kept for fvm_norms. */
 int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
 
+/* Central second-order viscous fluxes with Sutherland's law (no reference
+ * counterpart: viscous_flux_sub, src/solve/class_space_discret.f90:671-675,
+ * is an empty stub and its call at src/solve/class_time_discret.f90:182 is
+ * commented out).  mu = mu_ref (T/t_ref)^(3/2) (t_ref + s)/(T + s),
+ * T = p/(rho r_gas), heat conductivity mu cp/prandtl, cp = gamma r_gas/(gamma-1).
+ * mu_ref <= 0 switches the terms off.  Not available on connected blocks. */
+int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland_s,
+                    double prandtl, double r_gas);
+
 /* Solution time and end time (the time loop's time/total_time,
  * src/process/solver_sub.f90:196-206).  With an end time, CFL steps clip
  * the last dt to t_end - t and the steps of an fvm_step call that would

@@ -599,6 +599,158 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
     }
 }
 
+/* ======================================================================== */
+/*  Central viscous fluxes (no reference counterpart)                       */
+/* ======================================================================== */
+/* The reference reserves viscous_flux_sub (class_space_discret.f90:671-675)
+ * but leaves it empty and its call commented out (class_time_discret.f90:
+ * 182).  This restatement defines the build's viscous term:
+ *   - primitives u, v, w, T = p / (rho R) and mu = Sutherland(T) per vertex;
+ *     a stencil point outside the block maps into it per direction (periodic:
+ *     wrap by n, the nodal period; otherwise clamp to the boundary vertex),
+ *     so no edge or corner ghost is ever read;
+ *   - at the face between vertices a and b = a + e_d: face values are the
+ *     two-point means, the normal derivative (phi_b - phi_a) / dr_d and each
+ *     tangential derivative the mean of the central differences at a and b,
+ *     ((phi_a+ - phi_a-) + (phi_b+ - phi_b-)) / (4 dr_t);
+ *   - tau_ij = mu (d_j u_i + d_i u_j) - 2/3 mu div(u) delta_ij, heat flux
+ *     -k dT/dx_d with k = mu cp / Pr; face flux
+ *     F_d = (0, tau_d1, tau_d2, tau_d3, u_i tau_di + k dT/dx_d);
+ *   - rv = sum_d (F_d(a + e_d/2) - F_d(a - e_d/2)) / dr_d, added to the
+ *     inviscid rhs after its three directions. */
+
+double orc_sutherland(double t, double mu_ref, double t_ref, double s)
+{
+    double tr = t / t_ref;
+    return mu_ref * tr * sqrt(tr) * (t_ref + s) / (t + s);
+}
+
+static inline int64_t vmap(int64_t x, int64_t n, int lo_bc, int hi_bc)
+{
+    if (x < 0) return lo_bc == ORC_BC_PERIODIC ? x + n : 0;
+    if (x >= n) return hi_bc == ORC_BC_PERIODIC ? x - n : n - 1;
+    return x;
+}
+
+/* primitive quantities per real vertex: u, v, w, T, mu */
+typedef struct { double u, v, w, t, mu; } vprim_t;
+
+static void visc_prims(const int64_t n[3], const orc_config *c, const double *p_aos,
+                       int64_t stride_pt, vprim_t *out)
+{
+    int64_t nreal = n[0] * n[1] * n[2];
+    for (int64_t v = 0; v < nreal; ++v) {
+        const double *p = p_aos + stride_pt * v;
+        vprim_t *o = out + v;
+        o->u = p[1];
+        o->v = p[2];
+        o->w = p[3];
+        o->t = p[4] / (p[0] * c->visc_r);
+        o->mu = orc_sutherland(o->t, c->visc_mu_ref, c->visc_t_ref, c->visc_s);
+    }
+}
+
+/* face flux between vertex a and a + e_d; xs = the vertex index triples
+ * resolved through vmap by the caller (pts[which][..]) */
+static void visc_face(const vprim_t *P, const int64_t n[3], const orc_config *c, int d,
+                      const int64_t a[3], double F[4])
+{
+    int dim = c->dim;
+    int64_t b[3] = {a[0], a[1], a[2]};
+    b[d] += 1;
+#define VIDX(x, y, z) (vmap((x), n[0], c->bc[0], c->bc[1]) + n[0] * (vmap((y), n[1], c->bc[2], c->bc[3]) \
+                       + n[1] * (dim == 3 ? vmap((z), n[2], c->bc[4], c->bc[5]) : 0)))
+    const vprim_t *A = P + VIDX(a[0], a[1], a[2]);
+    const vprim_t *B = P + VIDX(b[0], b[1], b[2]);
+    double g[3][4];   /* g[e][q]: d/dx_e of q in (u, v, w, T) at the face */
+    for (int e = 0; e < 3; ++e)
+        for (int q = 0; q < 4; ++q) g[e][q] = 0.0;
+    {
+        const double va[4] = {A->u, A->v, A->w, A->t}, vb[4] = {B->u, B->v, B->w, B->t};
+        for (int q = 0; q < 4; ++q) g[d][q] = (vb[q] - va[q]) / c->dr[d];
+    }
+    for (int e = 0; e < dim; ++e) {
+        if (e == d) continue;
+        int64_t ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]};
+        int64_t bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
+        ap[e] += 1; am[e] -= 1; bp[e] += 1; bm[e] -= 1;
+        const vprim_t *Ap = P + VIDX(ap[0], ap[1], ap[2]), *Am = P + VIDX(am[0], am[1], am[2]);
+        const vprim_t *Bp = P + VIDX(bp[0], bp[1], bp[2]), *Bm = P + VIDX(bm[0], bm[1], bm[2]);
+        const double vap[4] = {Ap->u, Ap->v, Ap->w, Ap->t}, vam[4] = {Am->u, Am->v, Am->w, Am->t};
+        const double vbp[4] = {Bp->u, Bp->v, Bp->w, Bp->t}, vbm[4] = {Bm->u, Bm->v, Bm->w, Bm->t};
+        for (int q = 0; q < 4; ++q)
+            g[e][q] = ((vap[q] - vam[q]) + (vbp[q] - vbm[q])) / (4.0 * c->dr[e]);
+    }
+#undef VIDX
+    const double mu = 0.5 * (A->mu + B->mu);
+    const double uf[3] = {0.5 * (A->u + B->u), 0.5 * (A->v + B->v), 0.5 * (A->w + B->w)};
+    const double div = g[0][0] + g[1][1] + g[2][2];
+    double tau[3];
+    for (int i = 0; i < 3; ++i) {
+        /* tau_{d i} = mu (d_i u_d + d_d u_i) - 2/3 mu div delta_{di} */
+        double t = mu * (g[i][d] + g[d][i]);
+        if (i == d) t = t - (2.0 / 3.0) * mu * div;
+        tau[i] = t;
+    }
+    const double cpr = c->gamma * c->visc_r / ((c->gamma - 1.0) * c->visc_pr);
+    F[0] = tau[0];
+    F[1] = tau[1];
+    F[2] = tau[2];
+    F[3] = uf[0] * tau[0] + uf[1] * tau[1] + uf[2] * tau[2] + mu * cpr * g[d][3];
+}
+
+static void visc_rhs(const int64_t n[3], const orc_config *c, const vprim_t *P, double *rv)
+{
+    int64_t nreal = n[0] * n[1] * n[2];
+    for (int64_t k = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t ri = i + n[0] * (j + n[1] * k);
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int d = 0; d < c->dim; ++d) {
+                    int64_t a[3] = {i, j, k}, am[3] = {i, j, k};
+                    am[d] -= 1;
+                    double Fp[4], Fm[4];
+                    visc_face(P, n, c, d, a, Fp);
+                    visc_face(P, n, c, d, am, Fm);
+                    for (int q = 0; q < 4; ++q)
+                        acc[q] = d == 0 ? (Fp[q] - Fm[q]) / c->dr[d]
+                                        : acc[q] + (Fp[q] - Fm[q]) / c->dr[d];
+                }
+                rv[ri] = 0.0;
+                for (int q = 0; q < 4; ++q) rv[(q + 1) * nreal + ri] = acc[q];
+            }
+}
+
+void orc_viscous_residual(const int64_t n[3], const orc_config *c, const double *p_real,
+                          double *rv)
+{
+    int64_t nreal = n[0] * n[1] * n[2];
+    vprim_t *P = malloc(sizeof(vprim_t) * nreal);
+    visc_prims(n, c, p_real, 5, P);
+    visc_rhs(n, c, P, rv);
+    free(P);
+}
+
+/* rhs += viscous residual of the block's current primitives */
+static void viscous_residual(blk_t *b, const orc_config *c, double *rhs)
+{
+    int64_t nreal = b->n[0] * b->n[1] * b->n[2];
+    double *preal = malloc(sizeof(double) * 5 * nreal);
+    double *rv = malloc(sizeof(double) * NEQ * nreal);
+    vprim_t *P = malloc(sizeof(vprim_t) * nreal);
+    for (int64_t k = 0; k < b->n[2]; ++k)
+        for (int64_t j = 0; j < b->n[1]; ++j)
+            for (int64_t i = 0; i < b->n[0]; ++i)
+                memcpy(preal + 5 * (i + b->n[0] * (j + b->n[1] * k)),
+                       b->p + 5 * gidx(b, i + NG, j + NG, k + b->gz), 5 * sizeof(double));
+    visc_prims(b->n, c, preal, 5, P);
+    visc_rhs(b->n, c, P, rv);
+    for (int e = 1; e < NEQ; ++e)
+        for (int64_t v = 0; v < nreal; ++v) rhs[e * nreal + v] = rhs[e * nreal + v] + rv[e * nreal + v];
+    free(preal); free(rv); free(P);
+}
+
 double orc_cfl_dt(const int64_t n[3], int dim, const double *q_real,
                   double gamma, const double dr[3], double cfl)
 {
@@ -697,6 +849,7 @@ int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl
         for (int s = 1; s <= nstage; ++s) {
             memset(rhs, 0, sizeof(double) * NEQ * nreal);
             inviscid_residual(&b, c, rhs);
+            if (c->visc_mu_ref > 0.0) viscous_residual(&b, c, rhs);
             gather_real(&b, qm);
             if (c->rk == ORC_RK4) {
                 orc_rk4_stage(s, dt, NEQ * nreal, q0, qm, acc, rhs);

@@ -60,7 +60,22 @@ typedef struct orc_config {
     int bc[6];            /* i-,i+,j-,j+,k-,k+                               */
     double kappa, entfix, gamma;
     double dr[3];
+    /* central viscous fluxes (no reference counterpart: viscous_flux_sub is
+     * an empty stub there); visc_mu_ref <= 0 switches them off */
+    double visc_mu_ref;   /* Sutherland reference viscosity at visc_t_ref      */
+    double visc_t_ref;
+    double visc_s;        /* Sutherland constant (same units as T)             */
+    double visc_pr;       /* Prandtl number                                    */
+    double visc_r;        /* gas constant: T = p / (rho R), cp = gamma R/(gamma-1) */
 } orc_config;
+
+/* Sutherland's law mu(T) = mu_ref (T/T_ref)^(3/2) (T_ref + S) / (T + S). */
+double orc_sutherland(double t, double mu_ref, double t_ref, double s);
+
+/* Viscous residual of a primitive field alone (for tests): p AoS
+ * (5, n0, n1, n2) real vertices, rv (n0, n1, n2, 5) out (component 0 = 0). */
+void orc_viscous_residual(const int64_t n[3], const orc_config *cfg, const double *p_real,
+                          double *rv);
 
 /* Runs nsteps steps of fixed dt (dt > 0) or CFL dt (dt <= 0, cfl > 0) on one
  * block of n[0] x n[1] x n[2] real vertices.

@@ -25,7 +25,9 @@ BC_EXTRAP, BC_PERIODIC = 0, 1
 class OrcConfig(ctypes.Structure):
     _fields_ = [("dim", ci), ("recon", ci), ("limiter", ci), ("flux", ci), ("rk", ci),
                 ("weno_textbook", ci), ("p_from_q", ci), ("bc", ci * 6),
-                ("kappa", cd), ("entfix", cd), ("gamma", cd), ("dr", cd * 3)]
+                ("kappa", cd), ("entfix", cd), ("gamma", cd), ("dr", cd * 3),
+                ("visc_mu_ref", cd), ("visc_t_ref", cd), ("visc_s", cd), ("visc_pr", cd),
+                ("visc_r", cd)]
 
 
 def _ptr(a):
@@ -61,6 +63,10 @@ def oracle():
         lib.orc_cfl_dt.restype = cd
         lib.orc_rk4_stage.argtypes = [ci, cd, i64, P, P, P, P]
         lib.orc_rk4_stage.restype = None
+        lib.orc_sutherland.argtypes = [cd, cd, cd, cd]
+        lib.orc_sutherland.restype = cd
+        lib.orc_viscous_residual.argtypes = [P, ctypes.c_void_p, P, P]
+        lib.orc_viscous_residual.restype = None
         _oracle = lib
     return _oracle
 
@@ -139,8 +145,9 @@ def runge_kutta(lib, m, dt, u0, um, rm):
 # ---------------------------------------------------------------- whole block
 def oracle_solve(p_real, dim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=RK_SSP3,
                  kappa=0.33333, entfix=0.2, gamma=1.4, dt=2e-4, cfl=0.0, nsteps=1,
-                 p_from_q=1, bc=(BC_EXTRAP,) * 6, weno_textbook=0):
-    """p_real: (nk, nj, ni, 5) primitives on real vertices.
+                 p_from_q=1, bc=(BC_EXTRAP,) * 6, weno_textbook=0, visc=None):
+    """p_real: (nk, nj, ni, 5) primitives on real vertices.  visc: None or
+    (mu_ref, t_ref, sutherland_s, prandtl, r_gas) for the viscous terms.
     Returns dict with q (5, nk+2gz, nj+6, ni+6), rhs (5, nk, nj, ni),
     res (nsteps, 6), dt (nsteps)."""
     p_real = np.ascontiguousarray(p_real, dtype=np.float64)
@@ -148,7 +155,7 @@ def oracle_solve(p_real, dim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=
     gz = 3 if dim == 3 else 0
     n = np.array([ni, nj, nk], dtype=np.int64)
     cfg = OrcConfig(dim, recon, limiter, flux, rk, weno_textbook, p_from_q,
-                    (ci * 6)(*bc), kappa, entfix, gamma, (cd * 3)(*dr))
+                    (ci * 6)(*bc), kappa, entfix, gamma, (cd * 3)(*dr), *(visc or (0.0,) * 5))
     q = np.zeros((5, nk + 2 * gz, nj + 6, ni + 6))
     rhs = np.zeros((5, nk, nj, ni))
     res = np.zeros((max(nsteps, 1), 6))
@@ -189,3 +196,20 @@ def cfl_dt(q_real, dim, dr, gamma=1.4, cfl=1.0):
     n = np.array(q_real.shape[1:][::-1], dtype=np.int64)
     dr = np.ascontiguousarray(dr, dtype=np.float64)
     return oracle().orc_cfl_dt(_ptr(n), dim, _ptr(q_real), gamma, _ptr(dr), cfl)
+
+
+def sutherland(t, mu_ref, t_ref, s):
+    return oracle().orc_sutherland(t, mu_ref, t_ref, s)
+
+
+def viscous_residual(p_real, dim, dr, visc, gamma=1.4, bc=(BC_EXTRAP,) * 6):
+    """Viscous residual alone of a primitive field p_real (nk, nj, ni, 5);
+    returns (5, nk, nj, ni) (component 0 is zero)."""
+    p_real = np.ascontiguousarray(p_real, dtype=np.float64)
+    nk, nj, ni, _ = p_real.shape
+    n = np.array([ni, nj, nk], dtype=np.int64)
+    cfg = OrcConfig(dim, MUSCL, LIM_MINMOD, ROE, RK_SSP3, 0, 1, (ci * 6)(*bc), 0.33333, 0.2,
+                    gamma, (cd * 3)(*dr), *visc)
+    rv = np.zeros((5, nk, nj, ni))
+    oracle().orc_viscous_residual(_ptr(n), ctypes.byref(cfg), _ptr(p_real), _ptr(rv))
+    return rv
