@@ -43,6 +43,25 @@ __device__ __forceinline__ double frcp(double x) {
   }
 }
 
+// Square root.  exact: IEEE sqrt.  fast: the hardware reciprocal square
+// root estimate refined by one Goldschmidt iteration and a final Newton
+// correction (relative error near one ulp for the positive normal operands it
+// sees: products of densities, squared sound speeds); sqrt(0) = 0.
+__device__ __forceinline__ double fsqrt(double x) {
+  if constexpr (kFast) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x == 0.0 ? 0.0 : g;
+  } else {
+    return sqrt(x);
+  }
+}
+
 // Fortran SIGN(1.0_dp, x)
 __device__ __forceinline__ double fsign1(double x) { return copysign(1.0, x); }
 // Fortran MIN(a, b) for finite reals
@@ -152,6 +171,40 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
   return w0 * q0 + w1 * q1 + w2 * q2;
 }
 
+// Textbook WENO5-JS at both faces of u2 (fast family): the value at the
+// right face (stencil u0..u4) and the mirrored value at the left face
+// (stencil u4..u0).  The mirrored smoothness indicators are the forward ones
+// in reverse order (each indicator is symmetric under stencil reversal), so
+// they and the pairwise weight products are computed once.
+__device__ __forceinline__ void weno_js_pair(double u0, double u1, double u2, double u3, double u4,
+                                             double& right, double& left) {
+  const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
+  const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
+  const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+  const double t0 = u0 - 2.0 * u1 + u2, s0 = u0 - 4.0 * u1 + 3.0 * u2;
+  const double t1 = u1 - 2.0 * u2 + u3, s1 = u1 - u3;
+  const double t2 = u2 - 2.0 * u3 + u4, s2 = 3.0 * u2 - 4.0 * u3 + u4;
+  const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
+  const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
+  const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
+  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+  const double P12 = E1 * E2, P02 = E0 * E2, P01 = E0 * E1;
+  {
+    const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
+    const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
+    const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
+    const double b0 = c_0 * P12, b1 = c_1 * P02, b2 = c_2 * P01;
+    right = (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
+  }
+  {
+    const double q0 = c13 * u4 - c76 * u3 + c116 * u2;
+    const double q1 = -(c16 * u3) + c56 * u2 + c13 * u1;
+    const double q2 = c13 * u2 + c56 * u1 - c16 * u0;
+    const double b0 = c_0 * P01, b1 = c_1 * P02, b2 = c_2 * P12;
+    left = (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
+  }
+}
+
 // ---- Harten fix, src/scheme/scheme_FDS_Roe.f90:167-177 ----------------------
 __device__ __forceinline__ double entropy_correct(double lambda, double delta) {
   if constexpr (kFast) return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) * frcp(delta);
@@ -182,7 +235,7 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
                            : P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
   const double drho = rho_r - rho_l, du = u_r - u_l, dv = v_r - v_l, dw = w_r - w_l;
   const double dpress = p_r - p_l, dVl = Vc_r - Vc_l;
-  const double rho_b = sqrt(rho_l * rho_r);
+  const double rho_b = fsqrt(rho_l * rho_r);
   const double den = rho_l + rho_b;
   double u_b, v_b, w_b, H_b;
   if constexpr (kFast) {
@@ -201,14 +254,25 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   double q2_b;
   if constexpr (NC == 5) q2_b = u_b * u_b + v_b * v_b + w_b * w_b;
   else q2_b = u_b * u_b + v_b * v_b;
-  const double c_b = sqrt((gamma - 1.0) * (H_b - 0.5 * q2_b));
+  const double c2_b = (gamma - 1.0) * (H_b - 0.5 * q2_b);
+  const double c_b = fsqrt(c2_b);
   const double delta = P.entfix * fabs(Vl_b + c_b);
-  const double lam1 = entropy_correct(fabs(Vl_b - c_b), delta);
-  const double lam2 = entropy_correct(fabs(Vl_b), delta);
-  const double lam5 = entropy_correct(fabs(Vl_b + c_b), delta);
+  double lam1, lam2, lam5;
+  if constexpr (kFast) {
+    // one reciprocal of delta serves the three Harten corrections
+    const double hrd = 0.5 * frcp(delta), dd = delta * delta;
+    const double a1 = fabs(Vl_b - c_b), a2 = fabs(Vl_b), a5 = fabs(Vl_b + c_b);
+    lam1 = a1 > delta ? a1 : (a1 * a1 + dd) * hrd;
+    lam2 = a2 > delta ? a2 : (a2 * a2 + dd) * hrd;
+    lam5 = a5 > delta ? a5 : (a5 * a5 + dd) * hrd;
+  } else {
+    lam1 = entropy_correct(fabs(Vl_b - c_b), delta);
+    lam2 = entropy_correct(fabs(Vl_b), delta);
+    lam5 = entropy_correct(fabs(Vl_b + c_b), delta);
+  }
   double d1, d2, d5;
   if constexpr (kFast) {
-    const double ic2 = frcp(c_b * c_b), hic2 = 0.5 * ic2;
+    const double ic2 = frcp(c2_b), hic2 = 0.5 * ic2;
     d1 = lam1 * (dpress - rho_b * c_b * dVl) * hic2;
     d2 = lam2 * (drho - dpress * ic2);
     d5 = lam5 * (dpress + rho_b * c_b * dVl) * hic2;
@@ -268,8 +332,8 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
   double q2_l, q2_r;
   if constexpr (NC == 5) { q2_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; q2_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { q2_l = vl[0] * vl[0] + vl[1] * vl[1]; q2_r = vr[0] * vr[0] + vr[1] * vr[1]; }
-  const double c_l = kFast ? sqrt(gamma * p_l * frcp(rho_l)) : sqrt(gamma * p_l / rho_l);
-  const double c_r = kFast ? sqrt(gamma * p_r * frcp(rho_r)) : sqrt(gamma * p_r / rho_r);
+  const double c_l = kFast ? fsqrt(gamma * p_l * frcp(rho_l)) : sqrt(gamma * p_l / rho_l);
+  const double c_r = kFast ? fsqrt(gamma * p_r * frcp(rho_r)) : sqrt(gamma * p_r / rho_r);
   double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
   double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
   M_l = (M_l >= 1.0) ? M_l : (M_l <= -1.0) ? 0.0 : 0.25 * (M_l + 1.0) * (M_l + 1.0);
@@ -318,8 +382,8 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   const double ir_l = kFast ? frcp(rho_l) : 0.0, ir_r = kFast ? frcp(rho_r) : 0.0;
   const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
   const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-  const double c_l = kFast ? sqrt(gamma * p_l * ir_l) : sqrt(gamma * p_l / rho_l);
-  const double c_r = kFast ? sqrt(gamma * p_r * ir_r) : sqrt(gamma * p_r / rho_r);
+  const double c_l = kFast ? fsqrt(gamma * p_l * ir_l) : sqrt(gamma * p_l / rho_l);
+  const double c_r = kFast ? fsqrt(gamma * p_r * ir_r) : sqrt(gamma * p_r / rho_r);
   const double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
   const double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
   // Mach split, beta = 0: the beta term adds an exact +0 for |M| < 1
@@ -366,15 +430,31 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
   double ke_l, ke_r;
   if constexpr (NC == 5) { ke_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; ke_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
-  const double H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-  const double H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-  const double cs2_l = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_l;
-  const double cs2_r = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_r;
-  const double cs_l = sqrt(cs2_l), cs_r = sqrt(cs2_r);
-  const double ct_l = cs2_l / fmax(cs_l, Vc_l);
-  const double ct_r = cs2_r / fmax(cs_r, -Vc_r);
-  const double c12 = fmin(ct_l, ct_r);
-  const double M_l = Vc_l / c12, M_r = Vc_r / c12;
+  double H_l, H_r, c12, M_l, M_r;
+  if constexpr (kFast) {
+    H_l = P.g_gm1 * p_l * frcp(rho_l) + 0.5 * ke_l;
+    H_r = P.g_gm1 * p_r * frcp(rho_r) + 0.5 * ke_r;
+    const double k2 = 2.0 * (gamma - 1.0) * frcp(gamma + 1.0);
+    const double cs2_l = k2 * H_l, cs2_r = k2 * H_r;
+    const double cs_l = fsqrt(cs2_l), cs_r = fsqrt(cs2_r);
+    const double ct_l = cs2_l * frcp(fmax(cs_l, Vc_l));
+    const double ct_r = cs2_r * frcp(fmax(cs_r, -Vc_r));
+    c12 = fmin(ct_l, ct_r);
+    const double rc = frcp(c12);
+    M_l = Vc_l * rc;
+    M_r = Vc_r * rc;
+  } else {
+    H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
+    H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
+    const double cs2_l = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_l;
+    const double cs2_r = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_r;
+    const double cs_l = sqrt(cs2_l), cs_r = sqrt(cs2_r);
+    const double ct_l = cs2_l / fmax(cs_l, Vc_l);
+    const double ct_r = cs2_r / fmax(cs_r, -Vc_r);
+    c12 = fmin(ct_l, ct_r);
+    M_l = Vc_l / c12;
+    M_r = Vc_r / c12;
+  }
   double m4p, m4m, p5p, p5m;
   if (fabs(M_l) >= 1.0) {
     m4p = 0.5 * (M_l + fabs(M_l));

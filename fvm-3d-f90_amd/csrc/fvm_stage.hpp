@@ -77,6 +77,46 @@ __device__ __forceinline__ double rk_apply(int stage, const RKCoef& k, double q0
   }
 }
 
+// The same combine on values loaded ahead: x is the stage input value (SSP
+// stages 2, 3) or the accumulator (RK4 stages 2-4); acc_out receives the new
+// accumulator of RK4 stages 1-3.
+__device__ __forceinline__ double rk_apply_v(int stage, const RKCoef& k, double q0, double x, double r,
+                                             double& acc_out) {
+  acc_out = 0.0;
+  switch (stage) {
+    case ST_SSP1: return q0 + k.c * r;
+    case ST_SSP2:
+    case ST_SSP3: return k.a * q0 + k.b * x + k.c * r;
+    case ST_RK4_1: acc_out = q0 + k.c * r; return q0 + k.d * r;
+    case ST_RK4_2:
+    case ST_RK4_3: acc_out = x + k.c * r; return q0 + k.d * r;
+    default: return x + k.c * r;
+  }
+}
+
+// Loads the operands rk_apply_v needs at one vertex (uniform stage).
+template <int NC>
+__device__ __forceinline__ void rk_prefetch(int stage, bool owner, int64_t ci, int64_t cs,
+                                            const double* __restrict__ q0p,
+                                            const double* __restrict__ qin,
+                                            const double* __restrict__ acc, double q0[NC],
+                                            double x[NC]) {
+#pragma unroll
+  for (int m = 0; m < NC; ++m) { q0[m] = 0.0; x[m] = 0.0; }
+  if (!owner) return;
+  if (stage != ST_RK4_4) {
+#pragma unroll
+    for (int m = 0; m < NC; ++m) q0[m] = q0p[ci + m * cs];
+  }
+  if (stage == ST_SSP2 || stage == ST_SSP3) {
+#pragma unroll
+    for (int m = 0; m < NC; ++m) x[m] = qin[ci + m * cs];
+  } else if (stage == ST_RK4_2 || stage == ST_RK4_3 || stage == ST_RK4_4) {
+#pragma unroll
+    for (int m = 0; m < NC; ++m) x[m] = acc[ci + m * cs];
+  }
+}
+
 // ---------------------------------------------------------------- recon policies
 // RV: vertex reconstruction radius; H = RV + 1: stencil radius of a cell.
 // CONS: the reconstructed variables are conserved (converted at the face).
@@ -107,6 +147,15 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
   template <int NC>
   __device__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
     double qL[NC], qR[NC];
+    // the shared-indicator pair costs registers the 3-D stage (five-plane
+    // k-queue) does not have: there it would spill, so it is 2-D only
+    if (kFast && NC == 4 && P.weno_textbook) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) weno_js_pair(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], qL[c], qR[c]);
+      cons_to_prim<NC>(qL, L, P.gamma);
+      cons_to_prim<NC>(qR, R, P.gamma);
+      return;
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       qL[c] = weno_js(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], P.weno_textbook);
@@ -688,6 +737,14 @@ void stage3d_kernel(StageArgs a) {
       }
       __syncthreads();
 
+      // operands of the RK combine of vertex (c, r, k), loaded now so their
+      // latency hides behind the face fluxes
+      // (WENO's five-plane queue leaves no registers for it: loaded in D)
+      constexpr bool kPrefetchRK = RC::RV <= 1;
+      double pq0[NC], px[NC];
+      if constexpr (kPrefetchRK)
+        rk_prefetch<NC>(a.stage, owner, gidx(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
+
       // ---- C: face fluxes (into the plane buffer, dead after B)
       for (int e = t; e < FT; e += NT) {
         double L[NC], R[NC], F[NC];
@@ -739,9 +796,13 @@ void stage3d_kernel(StageArgs a) {
             rr = rr - (Fn[m] - Fc[m]) / g.dz;
           }
           if (m > 0 && rvp) rr = rr + rvp[(int64_t)(m - 1) * g.cs + cidx];
-          const double q0 = q0p[cidx + m * g.cs];
-          double* accp = acc ? acc + cidx + m * g.cs : nullptr;
-          out[m] = rk_apply(a.stage, kc, q0, qin[cidx + m * g.cs], accp, rr);
+          if constexpr (!kPrefetchRK) {
+            if (m == 0) rk_prefetch<NC>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
+          }
+          double accv;
+          out[m] = rk_apply_v(a.stage, kc, pq0[m], px[m], rr, accv);
+          if (a.stage == ST_RK4_1 || a.stage == ST_RK4_2 || a.stage == ST_RK4_3)
+            acc[cidx + m * g.cs] = accv;
           nrm[m] += rr * rr;
         }
         store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
