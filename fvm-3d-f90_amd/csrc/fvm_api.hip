@@ -94,6 +94,15 @@ struct fvm_ctx {
   double time = 0.0;                      // solution time after the last step
   double t_end = HUGE_VAL;                // CFL steps stop here (halt), +inf: never
   int64_t steps_done = 0;
+  // hipGraph replay of whole time steps (fvm_set_graph): one graph per
+  // (norm partials wanted) flavour, re-captured when `gen` or the key moves
+  bool use_graph = false;
+  uint64_t gen = 1;                       // bumped by every change of launch configuration
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+    bool cfl_mode = false;
+  } graphs[2];
   double* d_norm = nullptr;               // per-tile partial sums of rhs^2
   size_t norm_cap = 0;                    // doubles
   std::vector<double> h_norm;
@@ -317,6 +326,7 @@ int sync_desc(fvm_ctx* ctx) {
     base += b->ntiles;
   }
   ctx->total_tiles = base;
+  ++ctx->gen;
   if (h.size() > ctx->desc_cap) {
     if (ctx->d_desc) HIP_TRY(hipFree(ctx->d_desc));
     ctx->d_desc = nullptr;
@@ -341,6 +351,7 @@ int ensure_norm_capacity(fvm_ctx* ctx, size_t doubles) {
   if (hipMalloc(&ctx->d_norm, doubles * sizeof(double)) != hipSuccess)
     return fail(FVM_ENOMEM, "cannot allocate %zu norm partials", doubles);
   ctx->norm_cap = doubles;
+  ++ctx->gen;
   return FVM_OK;
 }
 
@@ -462,6 +473,83 @@ int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
   return FVM_OK;
 }
 
+// One time step of every block: the CFL dt update (CFL mode), then the RK
+// stages.  `with_norm`: the last stage leaves its residual norm partials.
+// With timing on, each stage launch is bracketed by events (ev index *nev).
+int launch_step(fvm_ctx* ctx, bool cfl_mode, bool with_norm, size_t* nev) {
+  if (cfl_mode) {
+    hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot);
+    HIP_TRY(hipGetLastError());
+  }
+  const int nstage = ctx->rk == RK4 ? 4 : 3;
+  for (int st = 1; st <= nstage; ++st) {
+    const bool last = st == nstage;
+    StageArgs a{};
+    a.P = ctx->P;
+    a.q0_buf = 0;
+    a.use_acc = 0;
+    if (ctx->rk == RK4) {
+      a.use_acc = 1;
+      switch (st) {
+        case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_RK4_1; break;
+        case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_RK4_2; break;
+        case 3: a.in_buf = 2; a.out_buf = 1; a.stage = ST_RK4_3; break;
+        default: a.in_buf = 1; a.out_buf = 0; a.stage = ST_RK4_4; break;
+      }
+    } else {
+      switch (st) {
+        case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_SSP1; break;
+        case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_SSP2; break;
+        default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
+      }
+    }
+    a.norm_part = (last && with_norm) ? ctx->d_norm : nullptr;
+    a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
+    a.halt = cfl_mode ? &ctx->d_slot->halt : nullptr;
+    const bool ev = ctx->timing && nev;
+    if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
+    if (int rc = launch_stage_all(ctx, a)) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
+  }
+  return FVM_OK;
+}
+
+// The instantiated graph of one step, captured from launch_step on the
+// context's stream the first time it is needed and again whenever the launch
+// configuration changed.  No memset or copy is captured: the step is kernel
+// launches only, reading dt, the CFL maximum and the halt flag from device
+// memory.
+int step_graph(fvm_ctx* ctx, bool cfl_mode, bool with_norm, hipGraphExec_t* out) {
+  fvm_ctx::StepGraph& sg = ctx->graphs[with_norm ? 1 : 0];
+  if (sg.exec && sg.gen == ctx->gen && sg.cfl_mode == cfl_mode) {
+    *out = sg.exec;
+    return FVM_OK;
+  }
+  if (sg.exec) {
+    HIP_TRY(hipGraphExecDestroy(sg.exec));
+    sg.exec = nullptr;
+  }
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = launch_step(ctx, cfl_mode, with_norm, nullptr);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(FVM_EHIP, "step capture: %s", hipGetErrorString(ec));
+  const hipError_t ei = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    sg.exec = nullptr;
+    return fail(FVM_EHIP, "step graph instantiate: %s", hipGetErrorString(ei));
+  }
+  sg.gen = ctx->gen;
+  sg.cfl_mode = cfl_mode;
+  *out = sg.exec;
+  return FVM_OK;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -504,6 +592,8 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
     delete b;
   }
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  for (auto& sg : ctx->graphs)
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   (void)hipFree(ctx->d_desc);
   (void)hipFree(ctx->d_slot);
   (void)hipFree(ctx->d_norm);
@@ -534,6 +624,7 @@ int fvm_scheme_set(fvm_ctx* ctx, int recon, int limiter, int flux, int rk, doubl
   ctx->P.limiter = limiter;
   ctx->P.weno_textbook = ref_compat ? 0 : 1;
   ctx->scheme_set = true;
+  ++ctx->gen;
   return FVM_OK;
 }
 
@@ -542,6 +633,7 @@ int fvm_set_numerics(fvm_ctx* ctx, int mode) {
   if (mode != FVM_NUMERICS_EXACT && mode != FVM_NUMERICS_FAST)
     return fail(FVM_EINVAL, "unknown numerics mode %d", mode);
   ctx->fast = mode == FVM_NUMERICS_FAST;
+  ++ctx->gen;
   return FVM_OK;
 }
 
@@ -550,6 +642,7 @@ int fvm_set_launch_mode(fvm_ctx* ctx, int mode) {
   if (mode != FVM_LAUNCH_PER_BLOCK && mode != FVM_LAUNCH_BATCHED && mode != FVM_LAUNCH_PERSISTENT)
     return fail(FVM_EINVAL, "unknown launch mode %d", mode);
   ctx->launch_mode = mode;
+  ++ctx->gen;
   return FVM_OK;
 }
 
@@ -809,37 +902,13 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     }
   }
   for (int s = 0; s < nsteps; ++s) {
-    if (cfl_mode) {
-      hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot);
-      HIP_TRY(hipGetLastError());
-    }
-    for (int st = 1; st <= nstage; ++st) {
-      const bool last = st == nstage;
-      StageArgs a{};
-      a.P = ctx->P;
-      a.q0_buf = 0;
-      a.use_acc = 0;
-      if (ctx->rk == RK4) {
-        a.use_acc = 1;
-        switch (st) {
-          case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_RK4_1; break;
-          case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_RK4_2; break;
-          case 3: a.in_buf = 2; a.out_buf = 1; a.stage = ST_RK4_3; break;
-          default: a.in_buf = 1; a.out_buf = 0; a.stage = ST_RK4_4; break;
-        }
-      } else {
-        switch (st) {
-          case 1: a.in_buf = 0; a.out_buf = 1; a.stage = ST_SSP1; break;
-          case 2: a.in_buf = 1; a.out_buf = 2; a.stage = ST_SSP2; break;
-          default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
-        }
-      }
-      a.norm_part = (last && (s == nsteps - 1 || may_halt)) ? ctx->d_norm : nullptr;
-      a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
-      a.halt = cfl_mode ? &ctx->d_slot->halt : nullptr;
-      if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
-      if (int rc = launch_stage_all(ctx, a)) return rc;
-      if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[nev++], ctx->stream));
+    const bool with_norm = s == nsteps - 1 || may_halt;
+    if (ctx->use_graph && !ctx->timing) {
+      hipGraphExec_t exec = nullptr;
+      if (int rc = step_graph(ctx, cfl_mode, with_norm, &exec)) return rc;
+      HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+    } else {
+      if (int rc = launch_step(ctx, cfl_mode, with_norm, &nev)) return rc;
     }
   }
 
@@ -915,6 +984,7 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
   if (mu_ref <= 0.0) {
     ctx->visc_mu_ref = 0.0;
     ctx->desc_dirty = true;
+    ++ctx->gen;
     return FVM_OK;
   }
   if (!(t_ref > 0.0) || !(prandtl > 0.0) || !(r_gas > 0.0) || !(sutherland_s >= 0.0) ||
@@ -933,6 +1003,13 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
   ctx->visc_pr = prandtl;
   ctx->visc_r = r_gas;
   ctx->desc_dirty = true;
+  ++ctx->gen;
+  return FVM_OK;
+}
+
+int fvm_set_graph(fvm_ctx* ctx, int enable) {
+  if (int rc = check_ctx(ctx)) return rc;
+  ctx->use_graph = enable != 0;
   return FVM_OK;
 }
 

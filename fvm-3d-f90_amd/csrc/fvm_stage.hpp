@@ -94,26 +94,32 @@ __device__ __forceinline__ double rk_apply_v(int stage, const RKCoef& k, double 
   }
 }
 
-// Loads the operands rk_apply_v needs at one vertex (uniform stage).
-template <int NC>
+// Loads the operands rk_apply_v needs at one vertex (uniform stage): PARTS
+// bit 0 the RK base state q0, bit 1 the stage input or accumulator x.
+template <int NC, int PARTS>
 __device__ __forceinline__ void rk_prefetch(int stage, bool owner, int64_t ci, int64_t cs,
                                             const double* __restrict__ q0p,
                                             const double* __restrict__ qin,
                                             const double* __restrict__ acc, double q0[NC],
                                             double x[NC]) {
+  if constexpr (PARTS & 1) {
 #pragma unroll
-  for (int m = 0; m < NC; ++m) { q0[m] = 0.0; x[m] = 0.0; }
-  if (!owner) return;
-  if (stage != ST_RK4_4) {
+    for (int m = 0; m < NC; ++m) q0[m] = 0.0;
+    if (owner && stage != ST_RK4_4) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) q0[m] = q0p[ci + m * cs];
+      for (int m = 0; m < NC; ++m) q0[m] = q0p[ci + m * cs];
+    }
   }
-  if (stage == ST_SSP2 || stage == ST_SSP3) {
+  if constexpr (PARTS & 2) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) x[m] = qin[ci + m * cs];
-  } else if (stage == ST_RK4_2 || stage == ST_RK4_3 || stage == ST_RK4_4) {
+    for (int m = 0; m < NC; ++m) x[m] = 0.0;
+    if (owner && (stage == ST_SSP2 || stage == ST_SSP3)) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) x[m] = acc[ci + m * cs];
+      for (int m = 0; m < NC; ++m) x[m] = qin[ci + m * cs];
+    } else if (owner && (stage == ST_RK4_2 || stage == ST_RK4_3 || stage == ST_RK4_4)) {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) x[m] = acc[ci + m * cs];
+    }
   }
 }
 
@@ -560,6 +566,14 @@ void stage3d_kernel(StageArgs a) {
   constexpr int NHL = (NHALO + NT - 1) / NT;       // halo loads per thread
   constexpr int RT = 2 * NT + 2 * TY + 2 * TX;     // reconstruction tasks
   constexpr int FT = 2 * NT + TY + TX;             // face tasks
+  // Balanced schedule (256-thread tiles): the halo reconstructions go to the
+  // first waves while the others reconstruct along k, and the halo faces to
+  // the last wave while the first ones reconstruct along k, so no wave runs a
+  // third round of i/j work while the rest wait at the barrier.
+  constexpr int kHaloWavesB = (RT - 2 * NT + 63) / 64;
+  // (WENO's five-plane queue has no registers to spare for it.)
+  constexpr bool kBalanced = RV <= 1 && NT == 256 && kHaloWavesB < NT / 64 &&
+                             FT - 2 * NT <= 64 && kHaloWavesB < NT / 64 - 1;
   __shared__ double Vs[NC * AREA];
   __shared__ double LRx[NLR][NC * LRX];
   __shared__ double LRy[NLR][NC * LRY];
@@ -570,6 +584,7 @@ void stage3d_kernel(StageArgs a) {
   const Params& P = a.P;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
+  const int wave = t / 64;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
   if (a.halt && *a.halt) return;
@@ -696,8 +711,8 @@ void stage3d_kernel(StageArgs a) {
       }
       __syncthreads();
 
-      // ---- B: vertex reconstructions
-      for (int e = t; e < RT; e += NT) {
+      // ---- B: vertex reconstructions (task e: own i, own j, then halo)
+      auto recon_task = [&](int e) {
         int src, stride, dst;
         bool xdir;
         if (e < NT) {                       // own vertex, i-direction
@@ -734,19 +749,40 @@ void stage3d_kernel(StageArgs a) {
             if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + dst] = R[m];
           }
         }
+      };
+      // k-direction vertex k+1 and face k+1 of the owned column (registers
+      // only); afterwards Lc holds L(k+1) for the next plane's face
+      double Fn[NC];
+      auto zstep = [&]() {
+        RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+        face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+      };
+      if constexpr (kBalanced) {
+        recon_task(t);
+        recon_task(t + NT);
+        if (wave < kHaloWavesB) {
+          if (t < RT - 2 * NT) recon_task(2 * NT + t);
+        } else {
+          zstep();
+        }
+      } else {
+        for (int e = t; e < RT; e += NT) recon_task(e);
       }
       __syncthreads();
 
       // operands of the RK combine of vertex (c, r, k), loaded now so their
       // latency hides behind the face fluxes
-      // (WENO's five-plane queue leaves no registers for it: loaded in D)
-      constexpr bool kPrefetchRK = RC::RV <= 1;
+      // (all of them where registers allow; q0 only in the balanced schedule;
+      // none with WENO's five-plane queue: the rest is loaded in D)
+      constexpr int kPrefetchRK = RC::RV > 1 ? 0 : (kBalanced ? 1 : 3);
       double pq0[NC], px[NC];
-      if constexpr (kPrefetchRK)
-        rk_prefetch<NC>(a.stage, owner, gidx(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
+      if constexpr (kPrefetchRK != 0)
+        rk_prefetch<NC, kPrefetchRK>(a.stage, owner, gidx(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
 
       // ---- C: face fluxes (into the plane buffer, dead after B)
-      for (int e = t; e < FT; e += NT) {
+      auto face_task = [&](int e) {
         double L[NC], R[NC], F[NC];
         if (e < NT || (e >= 2 * NT && e < 2 * NT + TY)) {   // i-face s of row y
           int y, s;
@@ -772,11 +808,19 @@ void stage3d_kernel(StageArgs a) {
 #pragma unroll
           for (int m = 0; m < NC; ++m) Fy[m * FYN + s * TX + x] = F[m];
         }
+      };
+      if constexpr (kBalanced) {
+        face_task(t);
+        face_task(t + NT);
+        if (wave < kHaloWavesB) {
+          zstep();
+        } else if (wave == NT / 64 - 1) {
+          if (t - (NT - 64) < FT - 2 * NT) face_task(2 * NT + t - (NT - 64));
+        }
+      } else {
+        for (int e = t; e < FT; e += NT) face_task(e);
+        zstep();
       }
-      // k-face k+1 of the owned column
-      double Fn[NC];
-      RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
-      face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
       __syncthreads();
 
       // ---- D: update
@@ -796,8 +840,8 @@ void stage3d_kernel(StageArgs a) {
             rr = rr - (Fn[m] - Fc[m]) / g.dz;
           }
           if (m > 0 && rvp) rr = rr + rvp[(int64_t)(m - 1) * g.cs + cidx];
-          if constexpr (!kPrefetchRK) {
-            if (m == 0) rk_prefetch<NC>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
+          if constexpr (kPrefetchRK != 3) {
+            if (m == 0) rk_prefetch<NC, 3 - kPrefetchRK>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
           }
           double accv;
           out[m] = rk_apply_v(a.stage, kc, pq0[m], px[m], rr, accv);
@@ -809,7 +853,7 @@ void stage3d_kernel(StageArgs a) {
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
       }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) { Fc[m] = Fn[m]; Lc[m] = Lt[m]; }
+      for (int m = 0; m < NC; ++m) Fc[m] = Fn[m];
       __syncthreads();
     }
 

@@ -41,7 +41,8 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_timing_enable", "fvm_timing_read", "fvm_set_numerics", "fvm_autotune",
            "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
-           "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set")
+           "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
+           "fvm_set_graph")
 
 
 class FvmError(RuntimeError):
@@ -94,6 +95,7 @@ def load_library(path=LIB_PATH):
         "fvm_set_launch_mode": ([vp, ci], ci),
         "fvm_set_time": ([vp, cd, cd], ci),
         "fvm_viscous_set": ([vp, cd, cd, cd, cd, cd], ci),
+        "fvm_set_graph": ([vp, ci], ci),
         "fvm_time": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
@@ -226,6 +228,10 @@ class Context:
         """Central viscous fluxes with Sutherland's law; mu_ref <= 0: off."""
         _check(_lib.fvm_viscous_set(self._h, float(mu_ref), float(t_ref), float(sutherland_s),
                                     float(prandtl), float(r_gas)))
+
+    def graph(self, enable=True):
+        """Replay whole time steps from a captured hipGraph."""
+        _check(_lib.fvm_set_graph(self._h, int(bool(enable))))
 
     def set_time(self, t=0.0, t_end=0.0):
         """Current time and end time (t_end <= t: none).  CFL steps clip to

@@ -160,6 +160,14 @@ int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
 int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland_s,
                     double prandtl, double r_gas);
 
+/* Replay each time step of fvm_step from a captured hipGraph (no reference
+ * counterpart): one graph per context holds the CFL dt update and the stage
+ * launches of a step; it is captured on first use and again after any
+ * change of scheme, numerics, launch mode, blocks or launch shapes.  Results
+ * are identical to eager launches.  Steps run with stage timing on
+ * (fvm_timing) are launched eagerly.  Off by default. */
+int fvm_set_graph(fvm_ctx *ctx, int enable);
+
 /* Solution time and end time (the time loop's time/total_time,
  * src/process/solver_sub.f90:196-206).  With an end time, CFL steps clip
  * the last dt to t_end - t and the steps of an fvm_step call that would
