@@ -527,12 +527,13 @@ def bench_serving(args):
         if act == 0 and pend == 0 and nxt < len(reqs):
             time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.001)))
     makespan = time.perf_counter() - t0
-    lat, cell_updates, steps = [], 0.0, 0
+    lat, cell_updates, steps, infos = [], 0.0, 0, []
     for i, rid in zip(range(len(reqs)), ids):
         try:
             _, info = srv.result(rid)
         except fvmgpu.FvmError as e:
             raise SystemExit(f"bench.py: request {i} ({reqs[i]['label']}) failed: {e}")
+        infos.append((rid, info))
         lat.append(info["t_done"] - arrivals[i])
         cell_updates += info["cell_updates"]
         steps += int(info["steps"])
@@ -541,7 +542,9 @@ def bench_serving(args):
 
     # sequential plain solves: one context per request, run to t_end
     ts = time.perf_counter()
-    for r in reqs:
+    per_kind = {}
+    for r, (_, info) in zip(reqs, infos):
+        t_req = time.perf_counter()
         _, nk, nj, ni = r["q0"].shape
         with fvmgpu.Context(0) as ctx:
             ctx.scheme(recon=r["recon"], limiter=r["limiter"], flux=r["flux"], rk=r["rk"],
@@ -560,6 +563,11 @@ def bench_serving(args):
                     break
                 chunk = int(min(64, max(1, np.ceil((r["t_end"] - t) / dt) + 1)))
             b.download()
+        kind = r["label"].split()[0] + ("-weno" if r["recon"] == fvmgpu.WENO else "-muscl")
+        k = per_kind.setdefault(kind, [0, 0.0, 0.0])
+        k[0] += 1
+        k[1] += info["cell_updates"]
+        k[2] += time.perf_counter() - t_req
     seq_sec = time.perf_counter() - ts
     seq_value = cell_updates / seq_sec
 
@@ -573,7 +581,9 @@ def bench_serving(args):
               "p99_time_to_solution_s": float(np.percentile(lat, 99)),
               "sequential_plain_value": seq_value, "sequential_plain_s": seq_sec,
               "speedup_vs_sequential": served_value / seq_value,
-              "steps_per_sync": args.steps_per_sync, "numerics": args.numerics}
+              "steps_per_sync": args.steps_per_sync, "numerics": args.numerics,
+              "sequential_by_kind": {k: {"requests": v[0], "cell_updates_per_s": v[1] / v[2]}
+                                     for k, v in sorted(per_kind.items())}}
     return line(args, served_value, steps, makespan, config, None, None)
 
 

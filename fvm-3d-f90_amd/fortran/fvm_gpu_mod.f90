@@ -25,10 +25,14 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_AUSMPLUS = 1303
   integer(c_int), parameter, public :: FVM_BC_EXTRAP = 0, FVM_BC_PERIODIC = 1
   integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
+  integer(c_int), parameter, public :: FVM_LAUNCH_PER_BLOCK = 0, FVM_LAUNCH_BATCHED = 1
+  integer(c_int), parameter, public :: FVM_LAUNCH_PERSISTENT = 2
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
+  public :: fvm_block_connect, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
+  public :: fvm_set_launch_mode, fvm_set_graph
   public :: fvm_check, fvm_error_text, scheme_index
 
   interface
@@ -127,6 +131,57 @@ module fvm_gpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function fvm_sync
+
+    function fvm_block_connect(a, face, b, shared) bind(C, name="fvm_block_connect") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, b
+      integer(c_int), value :: face, shared
+      integer(c_int) :: rc
+    end function fvm_block_connect
+
+    function fvm_viscous_set(ctx, mu_ref, t_ref, sutherland_s, prandtl, r_gas) &
+        bind(C, name="fvm_viscous_set") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: mu_ref, t_ref, sutherland_s, prandtl, r_gas
+      integer(c_int) :: rc
+    end function fvm_viscous_set
+
+    function fvm_set_time(ctx, t, t_end) bind(C, name="fvm_set_time") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t, t_end
+      integer(c_int) :: rc
+    end function fvm_set_time
+
+    function fvm_time(ctx, t, steps) bind(C, name="fvm_time") result(rc)
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: t
+      integer(c_int64_t), intent(out) :: steps
+      integer(c_int) :: rc
+    end function fvm_time
+
+    function fvm_autotune(ctx, reps) bind(C, name="fvm_autotune") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int) :: rc
+    end function fvm_autotune
+
+    function fvm_set_launch_mode(ctx, mode) bind(C, name="fvm_set_launch_mode") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      integer(c_int) :: rc
+    end function fvm_set_launch_mode
+
+    function fvm_set_graph(ctx, enable) bind(C, name="fvm_set_graph") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      integer(c_int) :: rc
+    end function fvm_set_graph
 
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t

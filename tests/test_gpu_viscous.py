@@ -87,3 +87,26 @@ def test_tgv_fast_one_step_and_conservation():
         assert abs(q20[c].sum() - q0[c].sum()) <= 1e-11 * np.abs(q0[c]).sum() + 1e-12
     # viscosity dissipates kinetic energy
     assert cases.kinetic_energy(q20) < cases.kinetic_energy(q0)
+
+
+def test_tgv_initial_dissipation_rate():
+    """Early kinetic-energy decay of the TGV equals the closed form
+    -dE/dt = nu <|omega|^2> = 0.75 nu of the initial field (incompressible
+    limit; Ma = 0.1 here), within 5 %."""
+    p, h = cases.tgv(32)
+    re = 100.0
+    print("case tgv32 dissipation", flush=True)
+    with fvmgpu.Context() as ctx:
+        ctx.scheme(**TGV_SCHEME)
+        ctx.numerics(1)
+        ctx.viscous(*cases.tgv_params(re=re))
+        b = ctx.block((32, 32, 32), (h, h, h))
+        b.set_periodic()
+        b.upload(cases.to_conserved(p))
+        e0 = cases.kinetic_energy(b.download())
+        ctx.step(10, cfl=0.5)
+        t, _ = ctx.time()
+        e1 = cases.kinetic_energy(b.download())
+    rate = (e0 - e1) / t
+    print("dissipation rate", rate, "closed form", 0.75 / re)
+    assert abs(rate / (0.75 / re) - 1.0) < 0.05
