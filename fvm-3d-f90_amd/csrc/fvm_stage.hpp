@@ -578,6 +578,7 @@ void stage3d_kernel(StageArgs a) {
   __shared__ double LRx[NLR][NC * LRX];
   __shared__ double LRy[NLR][NC * LRY];
   __shared__ double red[NT / 64];
+  __shared__ double Nrm[NC * NT];     // per-thread rhs^2 sums (norm stage only)
   double* Fx = Vs;                 // [NC][FXN] after phase B
   double* Fy = Vs + NC * FXN;      // [NC][FYN]
 
@@ -675,9 +676,13 @@ void stage3d_kernel(StageArgs a) {
       }
     }
 
-    double nrm[NC];
+    const bool want_norm = a.norm_part != nullptr;
+    if (want_norm) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
+      for (int m = 0; m < NC; ++m) Nrm[m * NT + t] = 0.0;
+    }
+#pragma unroll
+
 
     for (int k = k0; k < k1; ++k) {
       // ---- A: queue shift (centre k+1), plane k into LDS, next loads
@@ -776,7 +781,7 @@ void stage3d_kernel(StageArgs a) {
       // latency hides behind the face fluxes
       // (all of them where registers allow; q0 only in the balanced schedule;
       // none with WENO's five-plane queue: the rest is loaded in D)
-      constexpr int kPrefetchRK = RC::RV > 1 ? 0 : (kBalanced ? 1 : 3);
+      constexpr int kPrefetchRK = RC::RV > 1 ? 0 : 3;
       double pq0[NC], px[NC];
       if constexpr (kPrefetchRK != 0)
         rk_prefetch<NC, kPrefetchRK>(a.stage, owner, gidx(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
@@ -847,7 +852,7 @@ void stage3d_kernel(StageArgs a) {
           out[m] = rk_apply_v(a.stage, kc, pq0[m], px[m], rr, accv);
           if (a.stage == ST_RK4_1 || a.stage == ST_RK4_2 || a.stage == ST_RK4_3)
             acc[cidx + m * g.cs] = accv;
-          nrm[m] += rr * rr;
+          if (want_norm) Nrm[m * NT + t] += rr * rr;
         }
         store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
@@ -860,7 +865,7 @@ void stage3d_kernel(StageArgs a) {
     if (a.norm_part) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        const double s = wg_sum<NT>(nrm[m], red);
+        const double s = wg_sum<NT>(Nrm[m * NT + t], red);
         if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
       }
     }
