@@ -310,8 +310,10 @@ class Server:
 
     def result(self, rid):
         """(final state, info dict) of a finished request."""
-        q = np.zeros(self.shapes[rid])
         info = (ctypes.c_double * 8)()
+        if rid not in self.shapes:            # the library reports unknown ids
+            _check(_lib.fvm_server_result(self._h, rid, None, info))
+        q = np.zeros(self.shapes[rid])
         _check(_lib.fvm_server_result(self._h, rid, q.ctypes.data, info))
         keys = ("t_submit", "t_admit", "t_done", "steps", "cell_updates", "t_final", "status",
                 "bytes")
