@@ -14,8 +14,9 @@ algorithmic bytes per cell-update of 160 B for RK4 (it reads/writes 3, 5,
 SSP-RK3 (2, 3, 3 arrays) and 85.3 B for 2-D SSP-RK3 (four components),
 divided by the mean stage launch duration measured with HIP events on the
 library's stream.
-CPU baseline: the oracle (a scalar C restatement of the reference
-algorithm) on one host core, a bounded sample of the same workload.
+CPU baseline: the oracle (a C restatement of the reference algorithm,
+OpenMP over independent pencils as the reference has no parallel path) on
+the host cores, and on one core, over a bounded sample of the same workload.
 
 Other workloads (--workload):
   riemann400  2-D four-quadrant Riemann problem, 400 x 400 vertices, the
@@ -106,6 +107,26 @@ def oracle():
     return O
 
 
+def cpu_measure(run, cells, sample):
+    """The oracle on the host cores (OpenMP over pencils, OMP_NUM_THREADS or
+    every core) and on one core, the same bounded sample both times."""
+    O = oracle()
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    used = O.set_threads(n)
+    t0 = time.perf_counter()
+    run(O)
+    sec_n = time.perf_counter() - t0
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    run(O)
+    sec_1 = time.perf_counter() - t0
+    O.set_threads(used)
+    return {"value": cells / sec_n, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": f"{sample} ({cells:.3g} cell-updates; {sec_n:.2f} s on {used} threads, "
+                      f"{sec_1:.2f} s on 1; oracle/fvm_oracle.c)",
+            "value_1core": cells / sec_1}
+
+
 def numerics_code(fvmgpu, args):
     return fvmgpu.NUMERICS_FAST if args.numerics == "fast" else fvmgpu.NUMERICS_EXACT
 
@@ -182,16 +203,12 @@ def bench_vortex(args):
               "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)}
     cpu = None
     if not args.no_cpu:
-        O = oracle()
-        t0 = time.perf_counter()
-        O.oracle_solve(p, 3, (h, h, h), recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE,
-                       rk=O.RK4, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
-                       nsteps=args.cpu_steps, p_from_q=1, bc=(O.BC_PERIODIC,) * 6)
-        csec = time.perf_counter() - t0
-        cells = ncell * 4 * args.cpu_steps
-        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload "
-                         f"({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+        cpu = cpu_measure(
+            lambda O: O.oracle_solve(p, 3, (h, h, h), recon=O.MUSCL, limiter=O.LIM_VANALBADA,
+                                     flux=O.ROE, rk=O.RK4, kappa=0.33333, entfix=0.2, dt=0.0,
+                                     cfl=cfl, nsteps=args.cpu_steps, p_from_q=1,
+                                     bc=(O.BC_PERIODIC,) * 6),
+            ncell * 4 * args.cpu_steps, f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
@@ -253,17 +270,12 @@ def bench_riemann(args):
               "pct_hbm_roofline": 100.0 * value * B_ALG["ssp3_2d"] / (HBM_PEAK_GBS * 1e9)}
     cpu = None
     if not args.no_cpu:
-        O = oracle()
         cs = max(args.cpu_steps, 10)
-        t0 = time.perf_counter()
-        O.oracle_solve(p, 2, (dx, dx, 1.0), recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE,
-                       rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl, nsteps=cs,
-                       p_from_q=1)
-        csec = time.perf_counter() - t0
-        cells = ncell * 3 * cs
-        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{cs} SSP-RK3 steps of the {n}x{n} Riemann workload "
-                         f"({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+        cpu = cpu_measure(
+            lambda O: O.oracle_solve(p, 2, (dx, dx, 1.0), recon=O.MUSCL, limiter=O.LIM_MINMOD,
+                                     flux=O.ROE, rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0,
+                                     cfl=cfl, nsteps=cs, p_from_q=1),
+            ncell * 3 * cs, f"{cs} SSP-RK3 steps of the {n}x{n} Riemann workload")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
@@ -342,18 +354,14 @@ def bench_blob27(args):
               "pct_hbm_roofline": 100.0 * value * B_ALG["ssp3_3d"] / (HBM_PEAK_GBS * 1e9)}
     cpu = None
     if not args.no_cpu:
-        O = oracle()
         key = max(states, key=lambda k: states[k].size)
         pb = states[key]
-        t0 = time.perf_counter()
-        O.oracle_solve(pb, 3, sp, recon=O.WENO, limiter=O.LIM_NONE, flux=O.AUSMPLUS,
-                       rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
-                       nsteps=args.cpu_steps, p_from_q=1, weno_textbook=1)
-        csec = time.perf_counter() - t0
         cells = float(pb.shape[0] * pb.shape[1] * pb.shape[2]) * 3 * args.cpu_steps
-        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} SSP-RK3 steps of the largest block {pb.shape[:3]} "
-                         f"alone ({cells:.3g} cell-updates, {csec:.1f} s, oracle/fvm_oracle.c)"}
+        cpu = cpu_measure(
+            lambda O: O.oracle_solve(pb, 3, sp, recon=O.WENO, limiter=O.LIM_NONE, flux=O.AUSMPLUS,
+                                     rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
+                                     nsteps=args.cpu_steps, p_from_q=1, weno_textbook=1),
+            cells, f"{args.cpu_steps} SSP-RK3 steps of the largest block {pb.shape[:3]} alone")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
@@ -422,16 +430,12 @@ def bench_tgv(args):
               "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)}
     cpu = None
     if not args.no_cpu:
-        O = oracle()
-        t0 = time.perf_counter()
-        O.oracle_solve(p, 3, (h, h, h), recon=O.WENO, limiter=O.LIM_NONE, flux=O.VANLEER,
-                       rk=O.RK4, dt=0.0, cfl=cfl, nsteps=1, p_from_q=1, bc=(O.BC_PERIODIC,) * 6,
-                       weno_textbook=1, visc=visc)
-        csec = time.perf_counter() - t0
-        cells = ncell * 4
-        cpu = {"value": cells / csec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"1 RK4 step of the {n}^3 TGV workload ({cells:.3g} cell-updates, "
-                         f"{csec:.1f} s, oracle/fvm_oracle.c)"}
+        cpu = cpu_measure(
+            lambda O: O.oracle_solve(p, 3, (h, h, h), recon=O.WENO, limiter=O.LIM_NONE,
+                                     flux=O.VANLEER, rk=O.RK4, dt=0.0, cfl=cfl, nsteps=1,
+                                     p_from_q=1, bc=(O.BC_PERIODIC,) * 6, weno_textbook=1,
+                                     visc=visc),
+            ncell * 4, f"1 RK4 step of the {n}^3 TGV workload")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
@@ -543,6 +547,7 @@ def bench_serving(args):
     # sequential plain solves: one context per request, run to t_end
     ts = time.perf_counter()
     per_kind = {}
+    service = []
     for r, (_, info) in zip(reqs, infos):
         t_req = time.perf_counter()
         _, nk, nj, ni = r["q0"].shape
@@ -563,15 +568,23 @@ def bench_serving(args):
                     break
                 chunk = int(min(64, max(1, np.ceil((r["t_end"] - t) / dt) + 1)))
             b.download()
+        service.append(time.perf_counter() - t_req)
         kind = r["label"].split()[0] + ("-weno" if r["recon"] == fvmgpu.WENO else "-muscl")
         k = per_kind.setdefault(kind, [0, 0.0, 0.0])
         k[0] += 1
         k[1] += info["cell_updates"]
-        k[2] += time.perf_counter() - t_req
+        k[2] += service[-1]
     seq_sec = time.perf_counter() - ts
     seq_value = cell_updates / seq_sec
 
     lat = np.array(lat)
+    # the same arrivals served first-come first-served by one plain solve at a
+    # time, with the service times just measured
+    fifo, free_at = [], 0.0
+    for a, sv in zip(arrivals, service):
+        free_at = max(free_at, a) + sv
+        fifo.append(free_at - a)
+    fifo = np.array(fifo)
     config = {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.rate}/s (seed {seed}); "
                           "riemann2d 64^2..512^2, vortex3d / blob3d 32^3..128^3, log-uniform "
                           f"sizes, end times {SERVE_T_END}, MUSCL-minmod + Roe or WENO5 + "
@@ -580,6 +593,8 @@ def bench_serving(args):
               "makespan_s": makespan, "p50_time_to_solution_s": float(np.percentile(lat, 50)),
               "p99_time_to_solution_s": float(np.percentile(lat, 99)),
               "sequential_plain_value": seq_value, "sequential_plain_s": seq_sec,
+              "sequential_fifo_p50_s": float(np.percentile(fifo, 50)),
+              "sequential_fifo_p99_s": float(np.percentile(fifo, 99)),
               "speedup_vs_sequential": served_value / seq_value,
               "steps_per_sync": args.steps_per_sync, "numerics": args.numerics,
               "sequential_by_kind": {k: {"requests": v[0], "cell_updates_per_s": v[1] / v[2]}

@@ -17,6 +17,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NEQ 5
 #define NG  3   /* ghost layers per side, src/base/defs_ghost_cell.f90:42 */
@@ -544,6 +547,11 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
     for (int d = 0; d < c->dim; ++d) {
         int64_t nv = b->n[d], nface = nv + 1, g = goff(b, d), npg = b->ng[d];
         int a_dim = d == 0 ? 1 : 0, b_dim = d == 2 ? 1 : 2;
+        int64_t na = b->n[a_dim], nb = b->n[b_dim];
+        /* pencils of one direction write disjoint rhs entries: they run in
+         * parallel (OpenMP), each thread with its own pencil buffers */
+#pragma omp parallel
+        {
         double *pg = malloc(sizeof(double) * 5 * npg);
         double *qg = malloc(sizeof(double) * NEQ * npg);
         double *pl = calloc(5 * nface, sizeof(double));
@@ -552,8 +560,9 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
         double *fl = calloc(NEQ * nface, sizeof(double));
         double *norm = calloc(3 * nface, sizeof(double));
         for (int64_t f = 0; f < nface; ++f) norm[d * nface + f] = 1.0;
-        for (int64_t bb = 0; bb < b->n[b_dim]; ++bb)
-            for (int64_t aa = 0; aa < b->n[a_dim]; ++aa) {
+#pragma omp for schedule(static)
+        for (int64_t pe = 0; pe < na * nb; ++pe) {
+                int64_t aa = pe % na, bb = pe / na;
                 int64_t base[3];
                 base[a_dim] = aa + goff(b, a_dim);
                 base[b_dim] = bb + goff(b, b_dim);
@@ -596,6 +605,7 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
                 }
             }
         free(pg); free(qg); free(pl); free(pr); free(qr); free(fl); free(norm);
+        }
     }
 }
 
@@ -618,6 +628,18 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
  *     F_d = (0, tau_d1, tau_d2, tau_d3, u_i tau_di + k dT/dx_d);
  *   - rv = sum_d (F_d(a + e_d/2) - F_d(a - e_d/2)) / dr_d, added to the
  *     inviscid rhs after its three directions. */
+
+/* Threads the pencil loops use (OpenMP); n <= 0 leaves the default. */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 double orc_sutherland(double t, double mu_ref, double t_ref, double s)
 {
@@ -702,6 +724,7 @@ static void visc_face(const vprim_t *P, const int64_t n[3], const orc_config *c,
 static void visc_rhs(const int64_t n[3], const orc_config *c, const vprim_t *P, double *rv)
 {
     int64_t nreal = n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < n[2]; ++k)
         for (int64_t j = 0; j < n[1]; ++j)
             for (int64_t i = 0; i < n[0]; ++i) {

@@ -69,6 +69,10 @@ typedef struct orc_config {
     double visc_r;        /* gas constant: T = p / (rho R), cp = gamma R/(gamma-1) */
 } orc_config;
 
+/* Threads of the pencil loops (OpenMP; results do not depend on it);
+ * n <= 0 keeps the current setting.  Returns the thread count in use. */
+int orc_set_threads(int n);
+
 /* Sutherland's law mu(T) = mu_ref (T/T_ref)^(3/2) (T_ref + S) / (T + S). */
 double orc_sutherland(double t, double mu_ref, double t_ref, double s);
 

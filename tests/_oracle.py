@@ -63,6 +63,8 @@ def oracle():
         lib.orc_cfl_dt.restype = cd
         lib.orc_rk4_stage.argtypes = [ci, cd, i64, P, P, P, P]
         lib.orc_rk4_stage.restype = None
+        lib.orc_set_threads.argtypes = [ci]
+        lib.orc_set_threads.restype = ci
         lib.orc_sutherland.argtypes = [cd, cd, cd, cd]
         lib.orc_sutherland.restype = cd
         lib.orc_viscous_residual.argtypes = [P, ctypes.c_void_p, P, P]
@@ -213,3 +215,8 @@ def viscous_residual(p_real, dim, dr, visc, gamma=1.4, bc=(BC_EXTRAP,) * 6):
     rv = np.zeros((5, nk, nj, ni))
     oracle().orc_viscous_residual(_ptr(n), ctypes.byref(cfg), _ptr(p_real), _ptr(rv))
     return rv
+
+
+def set_threads(n):
+    """Threads of the oracle's pencil loops; returns the count in use."""
+    return oracle().orc_set_threads(int(n))
