@@ -81,6 +81,8 @@ def parse():
                          "one workgroup per tile")
     ap.add_argument("--requests", type=int, default=96, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: arrivals per second")
+    ap.add_argument("--max-active", type=int, default=256,
+                    help="serving: solves resident on the GPU at once")
     ap.add_argument("--steps-per-sync", type=int, default=4,
                     help="serving: time steps per host synchronisation")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -517,7 +519,7 @@ def bench_serving(args):
     while srv.poll(100, args.steps_per_sync)[1]:
         pass
     srv.close()
-    srv = fvmgpu.Server(max_active=256)
+    srv = fvmgpu.Server(max_active=args.max_active)
     t0 = time.perf_counter()
     ids = []
     nxt = 0
@@ -596,7 +598,8 @@ def bench_serving(args):
               "sequential_fifo_p50_s": float(np.percentile(fifo, 50)),
               "sequential_fifo_p99_s": float(np.percentile(fifo, 99)),
               "speedup_vs_sequential": served_value / seq_value,
-              "steps_per_sync": args.steps_per_sync, "numerics": args.numerics,
+              "steps_per_sync": args.steps_per_sync, "max_active": args.max_active,
+              "numerics": args.numerics,
               "sequential_by_kind": {k: {"requests": v[0], "cell_updates_per_s": v[1] / v[2]}
                                      for k, v in sorted(per_kind.items())}}
     return line(args, served_value, steps, makespan, config, None, None)

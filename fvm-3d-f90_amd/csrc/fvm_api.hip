@@ -1179,6 +1179,7 @@ struct fvm_server {
   std::vector<SolveSlot> h_slots;
   std::vector<int> free_slots;
   int* d_ids = nullptr;     // slot ids of a group's active solves
+  int chunk3 = 16, chunk2 = 8;    // k-chunk (3-D) and row chunk (2-D) of served blocks
   double total_cell_updates = 0;
 };
 
@@ -1216,8 +1217,10 @@ int server_admit(fvm_server* s) {
     const fvm_request& r = j->req;
     if (int rc = fvm_block_create(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2], &b))
       return rc;
-    // one tile shape per group so its blocks share a launch
-    if (b->dim == 3) plan_launch(b, 1, -1);
+    // one tile shape per group so its blocks share a launch; long k-marches
+    // (the group's launch has tiles enough without splitting each block)
+    if (b->dim == 3) plan_launch(b, 1, s->chunk3);
+    else plan_launch(b, -1, s->chunk2);
     for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
       if (r.bc[f] == FVM_BC_PERIODIC)
         if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
@@ -1344,6 +1347,8 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
   s->budget = hbm_budget_bytes > 0 ? std::min(hbm_budget_bytes, 0.9 * (double)free_b) : 0.8 * (double)free_b;
   s->max_active = max_active;
   s->t0 = Clock::now();
+  if (const char* e = getenv("FVM_SERVE_CHUNK3")) s->chunk3 = std::max(1, atoi(e));
+  if (const char* e = getenv("FVM_SERVE_CHUNK2")) s->chunk2 = std::max(1, atoi(e));
   s->h_slots.assign(max_active, SolveSlot{});
   for (int i = max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
   HIP_TRY(hipMalloc(&s->d_slots, max_active * sizeof(SolveSlot)));
