@@ -83,6 +83,8 @@ struct fvm_ctx {
   size_t desc_cap = 0;
   bool desc_dirty = true;                // table must be rebuilt before a launch
   bool ghosts_dirty = true;              // u^n ghost layers must be refilled
+  bool ghosts_stale = false;             // steps ran: own ghost layers (not read by the
+                                         // stages) are refilled before a download with ghosts
   int total_tiles = 0;
   bool has_conn = false;                 // some block face is connected to another block
   // central viscous terms (fvm_viscous_set); mu_ref <= 0: inviscid
@@ -807,12 +809,14 @@ int fvm_download(fvm_block* b, double* q, int host_ng) {
   if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
   fvm_ctx* ctx = b->ctx;
   if (int rc = set_device(ctx)) return rc;
-  if (ctx->ghosts_dirty && b->uploaded) {
+  if ((ctx->ghosts_dirty || (ctx->ghosts_stale && host_ng > 0)) && b->uploaded) {
     bool all = true;
     for (fvm_block* o : ctx->blocks) all = all && o->uploaded;
     if (all) {
+      if (int rc = sync_desc(ctx)) return rc;
       if (int rc = fill_ghosts_all(ctx, 0)) return rc;
       ctx->ghosts_dirty = false;
+      ctx->ghosts_stale = false;
     }
   }
   const BlockGeom& g = b->g;
@@ -901,6 +905,7 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
       ctx->ev.push_back(e);
     }
   }
+  ctx->ghosts_stale = true;
   for (int s = 0; s < nsteps; ++s) {
     const bool with_norm = s == nsteps - 1 || may_halt;
     if (ctx->use_graph && !ctx->timing) {
@@ -1275,6 +1280,7 @@ int server_step(fvm_server* s, int nsteps) {
   for (size_t gi = 0; gi < offs.size(); ++gi) {
     fvm_ctx* g = offs[gi].first;
     const int n = (int)ids[g].size();
+    g->ghosts_stale = true;
     if (int rc = sync_desc(g)) return rc;
     const int nstage = g->rk == RK4 ? 4 : 3;
     for (int step = 0; step < nsteps; ++step) {

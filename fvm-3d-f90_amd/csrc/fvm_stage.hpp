@@ -208,6 +208,20 @@ __device__ __forceinline__ void to_recon_vars(const double raw[NC], double v[NC]
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// Index of a stencil point along one direction.  Real vertices map to
+// themselves; outside the block, to the vertex its ghost copy holds (the
+// periodic image, or the boundary vertex under extrapolation,
+// src/base/defs_block_data.f90:226-256), except across a connected face,
+// whose ghost layer the neighbour block writes.  Reading the source vertex
+// instead of a ghost copy gives the same value, so the stage writes no
+// ghost images of its own.
+__device__ __forceinline__ int bmap(int x, int n, int lo, int hi, int h) {
+  x = clampi(x, -h, n + h - 1);
+  if (x < 0) return lo == BC_PERIODIC ? x + n : (lo == BC_CONN ? x : 0);
+  if (x >= n) return hi == BC_PERIODIC ? x - n : (hi == BC_CONN ? x : n - 1);
+  return x;
+}
+
 // max(a, b) where a NaN operand wins, so a diverged state reaches the CFL
 // accumulator (fmax would drop it).
 __device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
@@ -231,11 +245,10 @@ __device__ __forceinline__ int64_t gidx_of(Tab tab, int blk, int i, int j, int k
   return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
 }
 
-// Write one output vertex and its ghost images along every swept direction:
-// its own ghosts (extrapolation: copies of the boundary vertex; periodic: the
-// vertex n periods away) and, across connected faces, the neighbour block's
+// Write one output vertex and, across connected faces, the neighbour block's
 // ghosts it donates to (shared interface vertex: vertex n-1-l of this block is
-// ghost -l of the next; abutting blocks: vertex n-l is ghost -l).
+// ghost -l of the next; abutting blocks: vertex n-l is ghost -l).  The block's
+// own ghosts are not written: readers map such points with bmap.
 template <int NC, int NDIR, bool CONN, class Tab, class Desc>
 __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, const Desc& B, int out_buf,
                                                   double* __restrict__ out, int i, int j, int k,
@@ -243,38 +256,19 @@ __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, c
   const int64_t base = gidx(g, i, j, k);
 #pragma unroll
   for (int c = 0; c < NC; ++c) out[base + c * g.cs] = v[c];
-  const int n[3] = {g.nx, g.ny, g.nz};
-  const int x[3] = {i, j, k};
-  // only vertices within ng + 1 of a face have ghost images; whole waves of
-  // interior vertices skip the rest
-  bool near = false;
+  if constexpr (CONN) {
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const int x[3] = {i, j, k};
+    // only vertices within ng + 1 of a face donate; whole waves of interior
+    // vertices skip the rest
+    bool near = false;
 #pragma unroll
-  for (int d = 0; d < NDIR; ++d) near = near || x[d] <= g.ng || x[d] >= n[d] - 1 - g.ng;
-  if (!near) return;
+    for (int d = 0; d < NDIR; ++d) near = near || x[d] <= g.ng || x[d] >= n[d] - 1 - g.ng;
+    if (!near) return;
 #pragma unroll
-  for (int d = 0; d < NDIR; ++d) {
-    const int64_t stride = d == 0 ? 1 : (d == 1 ? g.px : g.pxy);
-    const int blo = g.bc[2 * d], bhi = g.bc[2 * d + 1];
-    for (int l = 1; l <= g.ng; ++l) {
-      // own ghosts, low side
-      int tgt = -1000000;
-      if (blo == BC_PERIODIC) { if (x[d] == n[d] - l) tgt = -l; }
-      else if (blo == BC_EXTRAP && x[d] == 0) tgt = -l;
-      if (tgt != -1000000) {
-        const int64_t o = base + (int64_t)(tgt - x[d]) * stride;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[o + c * g.cs] = v[c];
-      }
-      // own ghosts, high side
-      tgt = -1000000;
-      if (bhi == BC_PERIODIC) { if (x[d] == l - 1) tgt = n[d] - 1 + l; }
-      else if (bhi == BC_EXTRAP && x[d] == n[d] - 1) tgt = n[d] - 1 + l;
-      if (tgt != -1000000) {
-        const int64_t o = base + (int64_t)(tgt - x[d]) * stride;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[o + c * g.cs] = v[c];
-      }
-      if constexpr (CONN) {
+    for (int d = 0; d < NDIR; ++d) {
+      const int blo = g.bc[2 * d], bhi = g.bc[2 * d + 1];
+      for (int l = 1; l <= g.ng; ++l) {
         // neighbour across the low face: its ghost n_nb - 1 + l
         if (blo == BC_CONN && x[d] == (B.nbr_shared[2 * d] ? l : l - 1)) {
           const int nb = B.nbr[2 * d];
@@ -303,7 +297,6 @@ __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, c
   }
 }
 
-// Block of a tile in the launch's tile list (uniform per workgroup).
 // The descriptor table is read through the constant address space: the
 // stage writes only state arrays, never the table, and loads marked
 // invariant let the compiler re-load descriptor fields instead of keeping
@@ -408,14 +401,14 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
     const int j0 = tj * B.chunk;
     const int j1 = min(j0 + B.chunk, g.ny);
     const int c = i0 - 1 + t;                       // owned column
-    const int cl = clampi(c, -H, g.nx + H - 1);
+    const int cl = bmap(c, g.nx, g.bc[0], g.bc[1], H);
     const bool owner = (t >= 1) && (t <= TX) && (c < g.nx);
 
     // j-queue of column c: Vq[w] = V(c, jc - QOFF + w), jc = vertex being reconstructed
     double Vq[NW][NC];
     auto loadV = [&](int j, double v[NC]) {
       double raw[NC];
-      load_raw<RC, NC>(qin, g.cs, gidx(g, cl, clampi(j, -H, g.ny + H - 1), 0), raw);
+      load_raw<RC, NC>(qin, g.cs, gidx(g, cl, bmap(j, g.ny, g.bc[2], g.bc[3], H), 0), raw);
       to_recon_vars<RC, NC>(raw, v, P.gamma);
     };
 #pragma unroll
@@ -453,7 +446,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
       if (t < 2 * RV) {
         const int hc = t < RV ? i0 - 1 - RV + t : i0 + TX + 1 + (t - RV);
         double raw[NC], v[NC];
-        load_raw<RC, NC>(qin, g.cs, gidx(g, clampi(hc, -H, g.nx + H - 1), j, 0), raw);
+        load_raw<RC, NC>(qin, g.cs, gidx(g, bmap(hc, g.nx, g.bc[0], g.bc[1], H), j, 0), raw);
         to_recon_vars<RC, NC>(raw, v, P.gamma);
         const int slot = t < RV ? t : NT + RV + (t - RV);
 #pragma unroll
@@ -612,11 +605,11 @@ void stage3d_kernel(StageArgs a) {
     const int k0 = tk * B.chunk, k1 = min(k0 + B.chunk, g.nz);
     const int c = i0 + tx, r = j0 + ty;
     const bool owner = c < g.nx && r < g.ny;
-    const int ci_ = clampi(c, -H, g.nx + H - 1), rj_ = clampi(r, -H, g.ny + H - 1);
+    const int ci_ = bmap(c, g.nx, g.bc[0], g.bc[1], H), rj_ = bmap(r, g.ny, g.bc[2], g.bc[3], H);
 
     auto gaddr = [&](int i, int j, int k) {
-      return gidx(g, clampi(i, -H, g.nx + H - 1), clampi(j, -H, g.ny + H - 1),
-                  clampi(k, -H, g.nz + H - 1));
+      return gidx(g, bmap(i, g.nx, g.bc[0], g.bc[1], H), bmap(j, g.ny, g.bc[2], g.bc[3], H),
+                  bmap(k, g.nz, g.bc[4], g.bc[5], H));
     };
     // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
     auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
