@@ -663,6 +663,13 @@ int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx
     return fail(FVM_EINVAL, "vertex spacing must be positive");
   if (!ctx->blocks.empty() && ctx->blocks[0]->dim != dim)
     return fail(FVM_EINVAL, "all blocks of a context must have the same dimension");
+  {
+    // kernels address a component with a 32-bit byte offset
+    const int64_t cs = (ni + 2 * kGhost) * (nj + 2 * kGhost) * (dim == 3 ? nk + 2 * kGhost : 1);
+    if (cs > (int64_t(1) << 29) - 64)
+      return fail(FVM_EINVAL, "block of %lld vertices per component (with ghosts) exceeds 2^29; "
+                  "split the grid into connected blocks", (long long)cs);
+  }
   if (int rc = set_device(ctx)) return rc;
   fvm_block* b = new fvm_block();
   b->ctx = ctx;

@@ -103,22 +103,26 @@ __device__ __forceinline__ void cons_to_prim(const double q[NC], double p[NC], d
 // Per-vertex form: both face values of vertex p2 share the limited slopes.
 //   L = value at the vertex's right face (left state of that face, wind -1)
 //   R = value at the vertex's left face  (right state of that face, wind +1)
-__device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3,
-                                             double kappa, int limiter,
+// LIM is the limiter code (a compile-time constant: the caller switches on it
+// once per vertex); codes other than minmod, van Leer and van Albada leave the
+// slopes unlimited.
+template <int LIM>
+__device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3, double kappa,
                                              double& L, double& R) {
   const double D_m = p2 - p1;
   const double D_p = p3 - p2;
   double Db_m, Db_p;
-  if (limiter == LIM_MINMOD) {
+  constexpr bool kLimited = LIM == LIM_MINMOD || LIM == LIM_VANLEER || LIM == LIM_VANALBADA;
+  if constexpr (LIM == LIM_MINMOD) {
     // minmod(x, y, 1) is symmetric in x and y: one evaluation serves both.
     Db_m = 0.5 * (fsign1(D_m) + fsign1(D_p)) * fmin2(fabs(D_m), fabs(1.0 * D_p));
     Db_p = Db_m;
-  } else if (limiter == LIM_VANLEER) {
+  } else if constexpr (LIM == LIM_VANLEER) {
     const double eps = 1.0e-6;
     if constexpr (kFast) Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p * frcp(fabs(D_m) + fabs(D_p) + eps);
     else Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p / (fabs(D_m) + fabs(D_p) + eps);
     Db_p = Db_m;
-  } else if (limiter == LIM_VANALBADA) {
+  } else if constexpr (LIM == LIM_VANALBADA) {
     const double eps = 1.0e-6;
     if constexpr (kFast) Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) * frcp(D_m * D_m + D_p * D_p + eps);
     else Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) / (D_m * D_m + D_p * D_p + eps);
@@ -127,7 +131,7 @@ __device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3,
     Db_m = D_m;
     Db_p = D_p;
   }
-  if (kFast && limiter != LIM_NONE && limiter != -1) {
+  if constexpr (kFast && kLimited) {
     // limited slopes are equal (Db_m == Db_p): (1-k)Db + (1+k)Db = 2 Db
     L = p2 + 0.5 * Db_m;
     R = p2 - 0.5 * Db_m;

@@ -35,6 +35,19 @@ __device__ __forceinline__ int64_t gidx(const BlockGeom& g, int i, int j, int k)
   return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
 }
 
+// 32-bit element offset of (i, j, k) within one component array (blocks are
+// limited to 2^29 elements per component, so byte offsets fit 32 bits and the
+// loads address a uniform component base plus a 32-bit lane offset).
+__device__ __forceinline__ uint32_t gidx32(const BlockGeom& g, int i, int j, int k) {
+  return (uint32_t)((k + g.oz) * (int)g.pxy + (j + g.oy) * (int)g.px + (i + g.ox));
+}
+__device__ __forceinline__ double ld_comp(const double* __restrict__ q, int64_t cs, int c, uint32_t idx) {
+  return *(const double*)((const char*)(q + c * cs) + (idx << 3));
+}
+__device__ __forceinline__ void st_comp(double* __restrict__ q, int64_t cs, int c, uint32_t idx, double v) {
+  *(double*)((char*)(q + c * cs) + (idx << 3)) = v;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of tiles so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
@@ -97,7 +110,7 @@ __device__ __forceinline__ double rk_apply_v(int stage, const RKCoef& k, double 
 // Loads the operands rk_apply_v needs at one vertex (uniform stage): PARTS
 // bit 0 the RK base state q0, bit 1 the stage input or accumulator x.
 template <int NC, int PARTS>
-__device__ __forceinline__ void rk_prefetch(int stage, bool owner, int64_t ci, int64_t cs,
+__device__ __forceinline__ void rk_prefetch(int stage, bool owner, uint32_t ci, int64_t cs,
                                             const double* __restrict__ q0p,
                                             const double* __restrict__ qin,
                                             const double* __restrict__ acc, double q0[NC],
@@ -107,7 +120,7 @@ __device__ __forceinline__ void rk_prefetch(int stage, bool owner, int64_t ci, i
     for (int m = 0; m < NC; ++m) q0[m] = 0.0;
     if (owner && stage != ST_RK4_4) {
 #pragma unroll
-      for (int m = 0; m < NC; ++m) q0[m] = q0p[ci + m * cs];
+      for (int m = 0; m < NC; ++m) q0[m] = ld_comp(q0p, cs, m, ci);
     }
   }
   if constexpr (PARTS & 2) {
@@ -115,10 +128,10 @@ __device__ __forceinline__ void rk_prefetch(int stage, bool owner, int64_t ci, i
     for (int m = 0; m < NC; ++m) x[m] = 0.0;
     if (owner && (stage == ST_SSP2 || stage == ST_SSP3)) {
 #pragma unroll
-      for (int m = 0; m < NC; ++m) x[m] = qin[ci + m * cs];
+      for (int m = 0; m < NC; ++m) x[m] = ld_comp(qin, cs, m, ci);
     } else if (owner && (stage == ST_RK4_2 || stage == ST_RK4_3 || stage == ST_RK4_4)) {
 #pragma unroll
-      for (int m = 0; m < NC; ++m) x[m] = acc[ci + m * cs];
+      for (int m = 0; m < NC; ++m) x[m] = ld_comp(acc, cs, m, ci);
     }
   }
 }
@@ -140,10 +153,20 @@ struct ReconSimple {   // scheme_reconstruct_simple.f90:42-85
 struct ReconMuscl {    // scheme_reconstruct_MUSCL.f90:47-226
   static constexpr int RV = 1, H = 2;
   static constexpr bool CONS = false, SAME = false;
+  template <int NC, int LIM>
+  __device__ static void vertex_lim(const double (*V)[NC], double kappa, double L[NC], double R[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) muscl_vertex<LIM>(V[0][c], V[1][c], V[2][c], kappa, L[c], R[c]);
+  }
+  // one uniform branch on the limiter per vertex, none per component
   template <int NC>
   __device__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) muscl_vertex(V[0][c], V[1][c], V[2][c], P.kappa, P.limiter, L[c], R[c]);
+    switch (P.limiter) {
+      case LIM_MINMOD: vertex_lim<NC, LIM_MINMOD>(V, P.kappa, L, R); break;
+      case LIM_VANLEER: vertex_lim<NC, LIM_VANLEER>(V, P.kappa, L, R); break;
+      case LIM_VANALBADA: vertex_lim<NC, LIM_VANALBADA>(V, P.kappa, L, R); break;
+      default: vertex_lim<NC, LIM_NONE>(V, P.kappa, L, R); break;
+    }
   }
 };
 
@@ -177,7 +200,6 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
     }
   }
 };
-
 template <int FLUX, int NC, int D>
 __device__ __forceinline__ void face_flux(const double L[NC], const double R[NC], const Params& P,
                                           double F[NC]) {
@@ -190,10 +212,10 @@ __device__ __forceinline__ void face_flux(const double L[NC], const double R[NC]
 // Load the reconstruction variables of one vertex (clamped indices must be
 // inside the allocation; the caller clamps).
 template <class RC, int NC>
-__device__ __forceinline__ void load_raw(const double* __restrict__ q, int64_t cs, int64_t idx,
+__device__ __forceinline__ void load_raw(const double* __restrict__ q, int64_t cs, uint32_t idx,
                                          double v[NC]) {
 #pragma unroll
-  for (int c = 0; c < NC; ++c) v[c] = q[idx + c * cs];
+  for (int c = 0; c < NC; ++c) v[c] = ld_comp(q, cs, c, idx);
 }
 
 template <class RC, int NC>
@@ -253,9 +275,9 @@ template <int NC, int NDIR, bool CONN, class Tab, class Desc>
 __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, const Desc& B, int out_buf,
                                                   double* __restrict__ out, int i, int j, int k,
                                                   const double v[NC]) {
-  const int64_t base = gidx(g, i, j, k);
+  const uint32_t base = gidx32(g, i, j, k);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) out[base + c * g.cs] = v[c];
+  for (int c = 0; c < NC; ++c) st_comp(out, g.cs, c, base, v[c]);
   if constexpr (CONN) {
     const int n[3] = {g.nx, g.ny, g.nz};
     const int x[3] = {i, j, k};
@@ -408,7 +430,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
     double Vq[NW][NC];
     auto loadV = [&](int j, double v[NC]) {
       double raw[NC];
-      load_raw<RC, NC>(qin, g.cs, gidx(g, cl, bmap(j, g.ny, g.bc[2], g.bc[3], H), 0), raw);
+      load_raw<RC, NC>(qin, g.cs, gidx32(g, cl, bmap(j, g.ny, g.bc[2], g.bc[3], H), 0), raw);
       to_recon_vars<RC, NC>(raw, v, P.gamma);
     };
 #pragma unroll
@@ -446,7 +468,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
       if (t < 2 * RV) {
         const int hc = t < RV ? i0 - 1 - RV + t : i0 + TX + 1 + (t - RV);
         double raw[NC], v[NC];
-        load_raw<RC, NC>(qin, g.cs, gidx(g, bmap(hc, g.nx, g.bc[0], g.bc[1], H), j, 0), raw);
+        load_raw<RC, NC>(qin, g.cs, gidx32(g, bmap(hc, g.nx, g.bc[0], g.bc[1], H), j, 0), raw);
         to_recon_vars<RC, NC>(raw, v, P.gamma);
         const int slot = t < RV ? t : NT + RV + (t - RV);
 #pragma unroll
@@ -608,8 +630,8 @@ void stage3d_kernel(StageArgs a) {
     const int ci_ = bmap(c, g.nx, g.bc[0], g.bc[1], H), rj_ = bmap(r, g.ny, g.bc[2], g.bc[3], H);
 
     auto gaddr = [&](int i, int j, int k) {
-      return gidx(g, bmap(i, g.nx, g.bc[0], g.bc[1], H), bmap(j, g.ny, g.bc[2], g.bc[3], H),
-                  bmap(k, g.nz, g.bc[4], g.bc[5], H));
+      return gidx32(g, bmap(i, g.nx, g.bc[0], g.bc[1], H), bmap(j, g.ny, g.bc[2], g.bc[3], H),
+                    bmap(k, g.nz, g.bc[4], g.bc[5], H));
     };
     // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
     auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
@@ -777,7 +799,7 @@ void stage3d_kernel(StageArgs a) {
       constexpr int kPrefetchRK = RC::RV > 1 ? 0 : 3;
       double pq0[NC], px[NC];
       if constexpr (kPrefetchRK != 0)
-        rk_prefetch<NC, kPrefetchRK>(a.stage, owner, gidx(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
+        rk_prefetch<NC, kPrefetchRK>(a.stage, owner, gidx32(g, c, r, k), g.cs, q0p, qin, acc, pq0, px);
 
       // ---- C: face fluxes (into the plane buffer, dead after B)
       auto face_task = [&](int e) {
@@ -823,7 +845,7 @@ void stage3d_kernel(StageArgs a) {
 
       // ---- D: update
       if (owner) {
-        const int64_t cidx = gidx(g, c, r, k);
+        const uint32_t cidx = gidx32(g, c, r, k);
         double out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
@@ -837,14 +859,14 @@ void stage3d_kernel(StageArgs a) {
             rr = rr - fyd / g.dy;
             rr = rr - (Fn[m] - Fc[m]) / g.dz;
           }
-          if (m > 0 && rvp) rr = rr + rvp[(int64_t)(m - 1) * g.cs + cidx];
+          if (m > 0 && rvp) rr = rr + ld_comp(rvp, g.cs, m - 1, cidx);
           if constexpr (kPrefetchRK != 3) {
             if (m == 0) rk_prefetch<NC, 3 - kPrefetchRK>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
           }
           double accv;
           out[m] = rk_apply_v(a.stage, kc, pq0[m], px[m], rr, accv);
           if (a.stage == ST_RK4_1 || a.stage == ST_RK4_2 || a.stage == ST_RK4_3)
-            acc[cidx + m * g.cs] = accv;
+            st_comp(acc, g.cs, m, cidx, accv);
           if (want_norm) Nrm[m * NT + t] += rr * rr;
         }
         store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
