@@ -90,24 +90,47 @@ __device__ __forceinline__ double rk_apply(int stage, const RKCoef& k, double q0
   }
 }
 
-// The same combine on values loaded ahead: x is the stage input value (SSP
-// stages 2, 3) or the accumulator (RK4 stages 2-4); acc_out receives the new
-// accumulator of RK4 stages 1-3.
-__device__ __forceinline__ double rk_apply_v(int stage, const RKCoef& k, double q0, double x, double r,
-                                             double& acc_out) {
-  acc_out = 0.0;
+// The combine of rk_apply over the NC components of one vertex, on operands
+// loaded ahead: x is the stage input (SSP stages 2, 3) or the accumulator
+// (RK4 stages 2-4); RK4 stages 1-3 store the new accumulator at ci.  One
+// branch on the (uniform) stage per vertex.
+template <int NC>
+__device__ __forceinline__ void rk_apply_all(int stage, const RKCoef& k, const double q0[NC],
+                                             const double x[NC], const double r[NC], double out[NC],
+                                             double* __restrict__ acc, int64_t cs, uint32_t ci) {
   switch (stage) {
-    case ST_SSP1: return q0 + k.c * r;
+    case ST_SSP1:
+#pragma unroll
+      for (int m = 0; m < NC; ++m) out[m] = q0[m] + k.c * r[m];
+      break;
     case ST_SSP2:
-    case ST_SSP3: return k.a * q0 + k.b * x + k.c * r;
-    case ST_RK4_1: acc_out = q0 + k.c * r; return q0 + k.d * r;
+    case ST_SSP3:
+#pragma unroll
+      for (int m = 0; m < NC; ++m) out[m] = k.a * q0[m] + k.b * x[m] + k.c * r[m];
+      break;
+    case ST_RK4_1:
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        st_comp(acc, cs, m, ci, q0[m] + k.c * r[m]);
+        out[m] = q0[m] + k.d * r[m];
+      }
+      break;
     case ST_RK4_2:
-    case ST_RK4_3: acc_out = x + k.c * r; return q0 + k.d * r;
-    default: return x + k.c * r;
+    case ST_RK4_3:
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        st_comp(acc, cs, m, ci, x[m] + k.c * r[m]);
+        out[m] = q0[m] + k.d * r[m];
+      }
+      break;
+    default:
+#pragma unroll
+      for (int m = 0; m < NC; ++m) out[m] = x[m] + k.c * r[m];
+      break;
   }
 }
 
-// Loads the operands rk_apply_v needs at one vertex (uniform stage): PARTS
+// Loads the operands rk_apply_all needs at one vertex (uniform stage): PARTS
 // bit 0 the RK base state q0, bit 1 the stage input or accumulator x.
 template <int NC, int PARTS>
 __device__ __forceinline__ void rk_prefetch(int stage, bool owner, uint32_t ci, int64_t cs,
@@ -846,28 +869,30 @@ void stage3d_kernel(StageArgs a) {
       // ---- D: update
       if (owner) {
         const uint32_t cidx = gidx32(g, c, r, k);
-        double out[NC];
+        double rr[NC], out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
           const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
           const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
-          double rr;
           if constexpr (kFast) {
-            rr = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
+            rr[m] = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
           } else {
-            rr = 0.0 - fxd / g.dx;
-            rr = rr - fyd / g.dy;
-            rr = rr - (Fn[m] - Fc[m]) / g.dz;
+            rr[m] = 0.0 - fxd / g.dx;
+            rr[m] = rr[m] - fyd / g.dy;
+            rr[m] = rr[m] - (Fn[m] - Fc[m]) / g.dz;
           }
-          if (m > 0 && rvp) rr = rr + ld_comp(rvp, g.cs, m - 1, cidx);
           if constexpr (kPrefetchRK != 3) {
             if (m == 0) rk_prefetch<NC, 3 - kPrefetchRK>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
           }
-          double accv;
-          out[m] = rk_apply_v(a.stage, kc, pq0[m], px[m], rr, accv);
-          if (a.stage == ST_RK4_1 || a.stage == ST_RK4_2 || a.stage == ST_RK4_3)
-            st_comp(acc, g.cs, m, cidx, accv);
-          if (want_norm) Nrm[m * NT + t] += rr * rr;
+        }
+        if (rvp) {
+#pragma unroll
+          for (int m = 1; m < NC; ++m) rr[m] = rr[m] + ld_comp(rvp, g.cs, m - 1, cidx);
+        }
+        rk_apply_all<NC>(a.stage, kc, pq0, px, rr, out, acc, g.cs, cidx);
+        if (want_norm) {
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Nrm[m * NT + t] += rr[m] * rr[m];
         }
         store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
