@@ -178,8 +178,7 @@ struct ReconMuscl {    // scheme_reconstruct_MUSCL.f90:47-226
   static constexpr bool CONS = false, SAME = false;
   template <int NC, int LIM>
   __device__ static void vertex_lim(const double (*V)[NC], double kappa, double L[NC], double R[NC]) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) muscl_vertex<LIM>(V[0][c], V[1][c], V[2][c], kappa, L[c], R[c]);
+    muscl_vertex_all<LIM, NC>(V, kappa, L, R);
   }
   // one uniform branch on the limiter per vertex, none per component
   template <int NC>
@@ -276,6 +275,12 @@ template <int NC>
 __device__ __forceinline__ double wave_speed(const double q[NC], const BlockGeom& g, double gamma) {
   double p[NC];
   cons_to_prim<NC>(q, p, gamma);
+  if constexpr (kFast) {
+    const double a = fsqrt(gamma * p[NC - 1] * frcp(p[0]));
+    double s = (fabs(p[1]) + a) * g.rdx + (fabs(p[2]) + a) * g.rdy;
+    if constexpr (NC == 5) s = s + (fabs(p[3]) + a) * g.rdz;
+    return s;
+  }
   const double a = sqrt(gamma * p[NC - 1] / p[0]);
   double s = (fabs(p[1]) + a) / g.dx + (fabs(p[2]) + a) / g.dy;
   if constexpr (NC == 5) s = s + (fabs(p[3]) + a) / g.dz;
