@@ -83,9 +83,11 @@ __device__ __forceinline__ void fsqrt_rsqrt(double x, double& s, double& rs) {
 
 // Reciprocals of N positive values from one frcp (fast family only): prefix
 // products, one reciprocal, then back-substitution (Montgomery's trick).  The
-// product of the N values must stay a normal double; callers pass limiter
-// denominators bounded below by 1e-6, so this holds while every value stays
-// below ~1e60.  Relative error below 2.4e-15 (N = 5, measured as for frcp).
+// product of the N values must stay a normal double.  Callers pass MUSCL
+// limiter denominators, at least 1e-6 each, so this holds while every slope
+// stays below ~1e30 in magnitude (van Albada's squared slopes overflow near
+// 1e154 in either family).  Relative error below 2.4e-15 (N = 5, measured as
+// for frcp).
 template <int N>
 __device__ __forceinline__ void batch_rcp(const double d[N], double r[N]) {
   double p[N];
@@ -519,15 +521,23 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
   else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
   double H_l, H_r, c12, M_l, M_r;
   if constexpr (kFast) {
-    H_l = P.g_gm1 * p_l * frcp(rho_l) + 0.5 * ke_l;
-    H_r = P.g_gm1 * p_r * frcp(rho_r) + 0.5 * ke_r;
+    // 1/rho_l, 1/rho_r from one reciprocal; c*, 1/c*^2 from one reciprocal
+    // square root per side; 1/c~ = max(c*, |V|) / c*^2, so the two
+    // max(c*, |V|) share the last reciprocal
+    const double rlr = frcp(rho_l * rho_r);
+    H_l = P.g_gm1 * p_l * (rho_r * rlr) + 0.5 * ke_l;
+    H_r = P.g_gm1 * p_r * (rho_l * rlr) + 0.5 * ke_r;
     const double k2 = 2.0 * (gamma - 1.0) * frcp(gamma + 1.0);
     const double cs2_l = k2 * H_l, cs2_r = k2 * H_r;
-    const double cs_l = fsqrt(cs2_l), cs_r = fsqrt(cs2_r);
-    const double ct_l = cs2_l * frcp(fmax(cs_l, Vc_l));
-    const double ct_r = cs2_r * frcp(fmax(cs_r, -Vc_r));
-    c12 = fmin(ct_l, ct_r);
-    const double rc = frcp(c12);
+    double cs_l, cs_r, rcs_l, rcs_r;
+    fsqrt_rsqrt(cs2_l, cs_l, rcs_l);
+    fsqrt_rsqrt(cs2_r, cs_r, rcs_r);
+    const double m_l = fmax(cs_l, Vc_l), m_r = fmax(cs_r, -Vc_r);
+    const double rm = frcp(m_l * m_r);
+    const double ct_l = cs2_l * (m_r * rm), ct_r = cs2_r * (m_l * rm);
+    const bool left = !(ct_r < ct_l);
+    c12 = left ? ct_l : ct_r;
+    const double rc = left ? m_l * (rcs_l * rcs_l) : m_r * (rcs_r * rcs_r);
     M_l = Vc_l * rc;
     M_r = Vc_r * rc;
   } else {
