@@ -1066,9 +1066,27 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     if (int rc = fill_ghosts_all(ctx, 0)) return rc;
     ctx->ghosts_dirty = false;
   }
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  // scratch word and events, released on every return path
+  struct TuneRes {
+    hipStream_t s;
+    unsigned long long* smax = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~TuneRes() {
+      (void)hipStreamSynchronize(s);
+      if (smax) (void)hipFree(smax);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } res{ctx->stream};
+  if (hipMalloc(&res.smax, sizeof(unsigned long long)) != hipSuccess) {
+    res.smax = nullptr;
+    return fail(FVM_ENOMEM, "cannot allocate the autotuner's scratch word");
+  }
+  unsigned long long* tune_smax = res.smax;
+  HIP_TRY(hipMemsetAsync(tune_smax, 0, sizeof(unsigned long long), ctx->stream));
+  HIP_TRY(hipEventCreate(&res.e0));
+  HIP_TRY(hipEventCreate(&res.e1));
+  hipEvent_t e0 = res.e0, e1 = res.e1;
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot, 1.0e-6);
   HIP_TRY(hipGetLastError());
   const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
@@ -1086,19 +1104,31 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
         const int chunk = dim == 3 ? chunks3[ci] : chunks2[ci];
         for (fvm_block* b : sel) plan_launch(b, dim == 3 ? tile : -1, chunk);
         if (int rc = sync_desc(ctx)) return rc;
-        StageArgs a{};
-        a.P = ctx->P;
-        a.in_buf = 0; a.q0_buf = 0; a.out_buf = 1;
-        a.use_acc = ctx->rk == RK4 ? 1 : 0;
-        a.stage = ctx->rk == RK4 ? ST_RK4_1 : ST_SSP1;
-        a.blocks = ctx->d_desc; a.nblocks = (int)ctx->blocks.size();
-        a.tile0 = together ? 0 : sel[0]->tile_base;
-        a.ntiles = together ? ctx->total_tiles : sel[0]->ntiles;
-        const int grid = grid_for(ctx, a.ntiles);
-        if (int rc = launch_one(ctx, dim, sel[0]->tile, a, grid)) return rc;   // warm
+        // every stage kind of the scheme, the last one with the CFL maximum
+        // (into a scratch word); outputs alternate between buffers 1 and 2
+        // so the state u^n (buffer 0) is left as it was
+        const int nstage = ctx->rk == RK4 ? 4 : 3;
+        auto sweep = [&]() -> int {
+          for (int st = 1; st <= nstage; ++st) {
+            StageArgs a{};
+            a.P = ctx->P;
+            a.q0_buf = 0;
+            a.in_buf = st == 1 ? 0 : (st % 2 == 0 ? 1 : 2);
+            a.out_buf = st % 2 == 1 ? 1 : 2;
+            a.use_acc = ctx->rk == RK4 ? 1 : 0;
+            a.stage = ctx->rk == RK4 ? ST_RK4_1 + (st - 1) : ST_SSP1 + (st - 1);
+            a.smax = st == nstage ? tune_smax : nullptr;
+            a.blocks = ctx->d_desc; a.nblocks = (int)ctx->blocks.size();
+            a.tile0 = together ? 0 : sel[0]->tile_base;
+            a.ntiles = together ? ctx->total_tiles : sel[0]->ntiles;
+            if (int rc = launch_one(ctx, dim, sel[0]->tile, a, grid_for(ctx, a.ntiles))) return rc;
+          }
+          return FVM_OK;
+        };
+        if (int rc = sweep()) return rc;   // warm
         HIP_TRY(hipEventRecord(e0, ctx->stream));
         for (int r = 0; r < reps; ++r)
-          if (int rc = launch_one(ctx, dim, sel[0]->tile, a, grid)) return rc;
+          if (int rc = sweep()) return rc;
         HIP_TRY(hipEventRecord(e1, ctx->stream));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1108,8 +1138,6 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     for (fvm_block* b : sel) plan_launch(b, dim == 3 ? best_tile : -1, best_chunk);
   }
   if (int rc = sync_desc(ctx)) return rc;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return FVM_OK;
 }
 

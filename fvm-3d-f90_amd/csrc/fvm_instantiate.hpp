@@ -32,6 +32,7 @@ struct Launcher {
       case 0: return go3d<FLUX, kTile3d[0][0], kTile3d[0][1], CONN>(a, grid, s);
       case 1: return go3d<FLUX, kTile3d[1][0], kTile3d[1][1], CONN>(a, grid, s);
       case 2: return go3d<FLUX, kTile3d[2][0], kTile3d[2][1], CONN>(a, grid, s);
+      case 3: return go3d<FLUX, kTile3d[3][0], kTile3d[3][1], CONN>(a, grid, s);
       default: return hipErrorInvalidValue;
     }
   }

@@ -10,8 +10,8 @@
 namespace fvm {
 
 // 3-D tile shapes the library is built with: id -> (TX, TY)
-constexpr int kNumTiles3d = 3;
-constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}};
+constexpr int kNumTiles3d = 4;
+constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}, {16, 16}};
 // viscous-residual tile (TX, TY); 3-D launches march over k-chunks
 constexpr int kViscTile[2] = {32, 8};
 

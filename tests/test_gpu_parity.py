@@ -48,7 +48,7 @@ def test_smooth3d_bitwise(recon, lim, flux):
     np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("tile", ["0", "1", "2"])
+@pytest.mark.parametrize("tile", ["0", "1", "2", "3"])
 @pytest.mark.parametrize("chunk", ["3", "8", "32"])
 def test_tile_shapes_bitwise(tile, chunk, monkeypatch):
     """Every compiled tile shape and k-chunk gives the same bits (partial
