@@ -198,9 +198,10 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
   template <int NC>
   __device__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
     double qL[NC], qR[NC];
-    // the shared-indicator pair costs registers the 3-D stage (five-plane
-    // k-queue) does not have: there it would spill, so it is 2-D only
-    if (kFast && NC == 4 && P.weno_textbook) {
+    // textbook WENO: both faces of the vertex share the smoothness
+    // indicators (in the 3-D stage this spills a few registers, and is still
+    // faster than two independent evaluations)
+    if (kFast && P.weno_textbook) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) weno_js_pair(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], qL[c], qR[c]);
       cons_to_prim<NC>(qL, L, P.gamma);
