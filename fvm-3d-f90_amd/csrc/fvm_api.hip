@@ -67,6 +67,9 @@ struct fvm_block {
   unsigned long long* smax_dev = nullptr;
   const int* halt_dev = nullptr;
   double* rv = nullptr;                                  // viscous residual (nc - 1 components)
+  // the four state arrays are one stream-ordered allocation starting at q[0]
+  // (serving); otherwise each is its own hipMalloc
+  bool pooled = false;
 };
 
 struct fvm_ctx {
@@ -222,10 +225,16 @@ __global__ void cfl_dt_kernel(SolveSlot* slot) { advance_slot(*slot); }
 // Fixed time step: the stage kernels read dt from the slot in either mode.
 __global__ void set_dt_kernel(SolveSlot* slot, double dt) { slot->dt = dt; }
 
-// The same for the solves of a serving group (slot ids listed in ids).
+// The same for the solves of a serving group (slot ids listed in ids).  A
+// solve whose dt is NaN, zero or infinite halts at once: its state stays the
+// last one computed, and no later launch writes it while it is retired.
 __global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) advance_slot(slots[ids[i]]);
+  if (i < n) {
+    SolveSlot& sl = slots[ids[i]];
+    advance_slot(sl);
+    if (sl.nan) sl.halt = 1;
+  }
 }
 
 }  // namespace
@@ -292,6 +301,26 @@ void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   if (b->ctx) b->ctx->desc_dirty = true;
 }
 
+// Release a block's device arrays: stream-ordered on `stream` for a pooled
+// block (null: free and wait), hipFree otherwise.
+void free_block_state(fvm_block* b, hipStream_t stream) {
+  if (b->pooled) {
+    if (b->q[0]) {
+      if (stream) {
+        (void)hipFreeAsync(b->q[0], stream);
+      } else {
+        (void)hipFreeAsync(b->q[0], b->ctx->stream);
+        (void)hipStreamSynchronize(b->ctx->stream);
+      }
+    }
+  } else {
+    for (double* p : b->q) if (p) (void)hipFree(p);
+  }
+  for (double*& p : b->q) p = nullptr;
+  if (b->rv) (void)hipFree(b->rv);
+  b->rv = nullptr;
+}
+
 // Viscous residual buffer of a block: nc - 1 components (momentum, energy).
 int alloc_rv(fvm_block* b) {
   if (b->rv) return FVM_OK;
@@ -304,16 +333,148 @@ int alloc_rv(fvm_block* b) {
   return FVM_OK;
 }
 
+// Block creation.  `alloc_stream` null: four hipMalloc'd state arrays, zeroed
+// on the context's stream.  Otherwise (serving) one stream-ordered allocation
+// on `alloc_stream` holds the four arrays, zeroed on that stream: no device-wide
+// synchronisation, and the memory returns to the device pool when freed.
+int block_create_impl(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
+                      double dz, hipStream_t alloc_stream, fvm_block** out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out) return fail(FVM_EINVAL, "null output pointer");
+  if (ni < 4 || nj < 4 || nk < 1)
+    return fail(FVM_EINVAL, "block needs ni, nj >= 4 and nk >= 1 (got %lld %lld %lld)",
+                (long long)ni, (long long)nj, (long long)nk);
+  if (ni > (1 << 24) || nj > (1 << 24) || nk > (1 << 24))
+    return fail(FVM_EINVAL, "block extent too large");
+  const int dim = nk == 1 ? 2 : 3;
+  if (dim == 3 && nk < 4) return fail(FVM_EINVAL, "3-D block needs nk >= 4");
+  if (!(dx > 0) || !(dy > 0) || (dim == 3 && !(dz > 0)))
+    return fail(FVM_EINVAL, "vertex spacing must be positive");
+  if (!ctx->blocks.empty() && ctx->blocks[0]->dim != dim)
+    return fail(FVM_EINVAL, "all blocks of a context must have the same dimension");
+  {
+    // kernels address a component with a 32-bit byte offset
+    const int64_t cs = (ni + 2 * kGhost) * (nj + 2 * kGhost) * (dim == 3 ? nk + 2 * kGhost : 1);
+    if (cs > (int64_t(1) << 29) - 64)
+      return fail(FVM_EINVAL, "block of %lld vertices per component (with ghosts) exceeds 2^29; "
+                  "split the grid into connected blocks", (long long)cs);
+  }
+  if (int rc = set_device(ctx)) return rc;
+  fvm_block* b = new fvm_block();
+  b->ctx = ctx;
+  b->dim = dim;
+  b->nc = dim == 3 ? 5 : 4;
+  BlockGeom& g = b->g;
+  g.nx = (int)ni; g.ny = (int)nj; g.nz = (int)nk;
+  g.ox = kGhost; g.oy = kGhost; g.oz = dim == 3 ? kGhost : 0;
+  g.px = ni + 2 * kGhost;
+  g.pxy = g.px * (nj + 2 * kGhost);
+  const int64_t planes = dim == 3 ? nk + 2 * kGhost : 1;
+  g.cs = (g.pxy * planes + 63) / 64 * 64;   // 512-byte aligned components
+  g.dx = dx; g.dy = dy; g.dz = dim == 3 ? dz : 1.0;
+  g.rdx = 1.0 / g.dx; g.rdy = 1.0 / g.dy; g.rdz = 1.0 / g.dz;
+  for (int f = 0; f < 6; ++f) g.bc[f] = BC_EXTRAP;
+  g.ng = kGhost;
+  b->state_elems = (size_t)b->nc * g.cs;
+  const size_t bytes = b->state_elems * sizeof(double);
+  if (alloc_stream) {
+    void* base = nullptr;
+    if (hipMallocAsync(&base, 4 * bytes, alloc_stream) != hipSuccess) {
+      delete b;
+      return fail(FVM_ENOMEM, "cannot allocate %zu bytes of block state", 4 * bytes);
+    }
+    b->pooled = true;
+    for (int k = 0; k < 4; ++k) b->q[k] = (double*)base + k * b->state_elems;
+    HIP_TRY(hipMemsetAsync(base, 0, 4 * bytes, alloc_stream));
+  } else {
+    for (int k = 0; k < 4; ++k) {
+      if (hipMalloc(&b->q[k], bytes) != hipSuccess) {
+        for (double* p : b->q) if (p) (void)hipFree(p);
+        delete b;
+        return fail(FVM_ENOMEM, "cannot allocate %zu bytes of block state", bytes);
+      }
+      HIP_TRY(hipMemsetAsync(b->q[k], 0, bytes, ctx->stream));
+    }
+  }
+  if (ctx->visc_mu_ref > 0.0)
+    if (int rc = alloc_rv(b)) { free_block_state(b, nullptr); delete b; return rc; }
+  plan_launch(b);
+  ctx->blocks.push_back(b);
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
+  *out = b;
+  return FVM_OK;
+}
+
+// Host layout (5 components, host_ng ghost layers) -> the block's device
+// layout in `stage` (state_elems doubles).
+int pack_state(const fvm_block* b, const double* q, int host_ng, std::vector<double>& stage) {
+  const BlockGeom& g = b->g;
+  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
+  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const int64_t hcs = hx * hy * hz;
+  stage.assign((size_t)b->state_elems, 0.0);
+  for (int c = 0; c < 5; ++c) {
+    int dc = c;
+    if (b->dim == 2) {
+      if (c == 3) {
+        for (int64_t j = 0; j < g.ny; ++j)
+          for (int64_t i = 0; i < g.nx; ++i) {
+            const double v = q[c * hcs + (j + host_ng) * hx + i + host_ng];
+            if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
+          }
+        continue;
+      }
+      if (c == 4) dc = 3;
+    }
+    for (int64_t k = 0; k < g.nz; ++k)
+      for (int64_t j = 0; j < g.ny; ++j) {
+        const double* src = q + c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + host_ng;
+        double* dst = stage.data() + dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + g.ox;
+        std::memcpy(dst, src, g.nx * sizeof(double));
+      }
+  }
+  return FVM_OK;
+}
+
+// The block's device layout in `stage` -> host layout (5 components, host_ng
+// ghost layers; face ghosts only, edges and corners zero).
+void unpack_state(const fvm_block* b, const double* stage, double* q, int host_ng) {
+  const BlockGeom& g = b->g;
+  const int gl = std::min(host_ng, g.ng);
+  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
+  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const int64_t hcs = hx * hy * hz;
+  const int gzl = b->dim == 3 ? gl : 0;
+  std::memset(q, 0, sizeof(double) * 5 * hcs);
+  for (int c = 0; c < 5; ++c) {
+    if (b->dim == 2 && c == 3) continue;   // rho*w == 0
+    const int dc = (b->dim == 2 && c == 4) ? 3 : c;
+    for (int64_t k = -gzl; k < g.nz + gzl; ++k)
+      for (int64_t j = -gl; j < g.ny + gl; ++j)
+        for (int64_t i = -gl; i < g.nx + gl; ++i) {
+          // faces only: at most one index outside the real range
+          const int outside = (i < 0 || i >= g.nx) + (j < 0 || j >= g.ny) + (k < 0 || k >= g.nz);
+          if (outside > 1) continue;
+          q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + (i + host_ng)] =
+              stage[dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + (i + g.ox)];
+        }
+  }
+}
+
 // Upload the descriptor table (blocks in context order, tiles numbered
 // block after block).
-int sync_desc(fvm_ctx* ctx) {
-  if (!ctx->desc_dirty) return FVM_OK;
-  std::vector<BlockDesc> h(ctx->blocks.size());
+// Descriptor entries of the context's blocks into h[0 .. blocks), tile bases
+// and the tile total.
+void fill_desc(fvm_ctx* ctx, BlockDesc* h) {
   int base = 0;
   ctx->has_conn = false;
   for (size_t i = 0; i < ctx->blocks.size(); ++i) {
     fvm_block* b = ctx->blocks[i];
     BlockDesc& d = h[i];
+    d = BlockDesc{};
     for (int f = 0; f < 6; ++f) ctx->has_conn = ctx->has_conn || b->g.bc[f] == BC_CONN;
     for (int k = 0; k < 4; ++k) d.q[k] = b->q[k];
     d.g = b->g;
@@ -329,6 +490,12 @@ int sync_desc(fvm_ctx* ctx) {
   }
   ctx->total_tiles = base;
   ++ctx->gen;
+}
+
+int sync_desc(fvm_ctx* ctx) {
+  if (!ctx->desc_dirty) return FVM_OK;
+  std::vector<BlockDesc> h(ctx->blocks.size());
+  fill_desc(ctx, h.data());
   if (h.size() > ctx->desc_cap) {
     if (ctx->d_desc) HIP_TRY(hipFree(ctx->d_desc));
     ctx->d_desc = nullptr;
@@ -589,8 +756,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (fvm_block* b : ctx->blocks) {
-    for (double* p : b->q) (void)hipFree(p);
-    if (b->rv) (void)hipFree(b->rv);
+    free_block_state(b, nullptr);
     delete b;
   }
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
@@ -650,60 +816,7 @@ int fvm_set_launch_mode(fvm_ctx* ctx, int mode) {
 
 int fvm_block_create(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
                      double dz, fvm_block** out) {
-  if (int rc = check_ctx(ctx)) return rc;
-  if (!out) return fail(FVM_EINVAL, "null output pointer");
-  if (ni < 4 || nj < 4 || nk < 1)
-    return fail(FVM_EINVAL, "block needs ni, nj >= 4 and nk >= 1 (got %lld %lld %lld)",
-                (long long)ni, (long long)nj, (long long)nk);
-  if (ni > (1 << 24) || nj > (1 << 24) || nk > (1 << 24))
-    return fail(FVM_EINVAL, "block extent too large");
-  const int dim = nk == 1 ? 2 : 3;
-  if (dim == 3 && nk < 4) return fail(FVM_EINVAL, "3-D block needs nk >= 4");
-  if (!(dx > 0) || !(dy > 0) || (dim == 3 && !(dz > 0)))
-    return fail(FVM_EINVAL, "vertex spacing must be positive");
-  if (!ctx->blocks.empty() && ctx->blocks[0]->dim != dim)
-    return fail(FVM_EINVAL, "all blocks of a context must have the same dimension");
-  {
-    // kernels address a component with a 32-bit byte offset
-    const int64_t cs = (ni + 2 * kGhost) * (nj + 2 * kGhost) * (dim == 3 ? nk + 2 * kGhost : 1);
-    if (cs > (int64_t(1) << 29) - 64)
-      return fail(FVM_EINVAL, "block of %lld vertices per component (with ghosts) exceeds 2^29; "
-                  "split the grid into connected blocks", (long long)cs);
-  }
-  if (int rc = set_device(ctx)) return rc;
-  fvm_block* b = new fvm_block();
-  b->ctx = ctx;
-  b->dim = dim;
-  b->nc = dim == 3 ? 5 : 4;
-  BlockGeom& g = b->g;
-  g.nx = (int)ni; g.ny = (int)nj; g.nz = (int)nk;
-  g.ox = kGhost; g.oy = kGhost; g.oz = dim == 3 ? kGhost : 0;
-  g.px = ni + 2 * kGhost;
-  g.pxy = g.px * (nj + 2 * kGhost);
-  const int64_t planes = dim == 3 ? nk + 2 * kGhost : 1;
-  g.cs = (g.pxy * planes + 63) / 64 * 64;   // 512-byte aligned components
-  g.dx = dx; g.dy = dy; g.dz = dim == 3 ? dz : 1.0;
-  g.rdx = 1.0 / g.dx; g.rdy = 1.0 / g.dy; g.rdz = 1.0 / g.dz;
-  for (int f = 0; f < 6; ++f) g.bc[f] = BC_EXTRAP;
-  g.ng = kGhost;
-  b->state_elems = (size_t)b->nc * g.cs;
-  const size_t bytes = b->state_elems * sizeof(double);
-  for (int k = 0; k < 4; ++k) {
-    if (hipMalloc(&b->q[k], bytes) != hipSuccess) {
-      for (double* p : b->q) if (p) (void)hipFree(p);
-      delete b;
-      return fail(FVM_ENOMEM, "cannot allocate %zu bytes of block state", bytes);
-    }
-    HIP_TRY(hipMemsetAsync(b->q[k], 0, bytes, ctx->stream));
-  }
-  if (ctx->visc_mu_ref > 0.0)
-    if (int rc = alloc_rv(b)) { for (double* p : b->q) (void)hipFree(p); delete b; return rc; }
-  plan_launch(b);
-  ctx->blocks.push_back(b);
-  ctx->desc_dirty = true;
-  ctx->ghosts_dirty = true;
-  *out = b;
-  return FVM_OK;
+  return block_create_impl(ctx, ni, nj, nk, dx, dy, dz, nullptr, out);
 }
 
 int fvm_block_destroy(fvm_block* b) {
@@ -718,8 +831,7 @@ int fvm_block_destroy(fvm_block* b) {
       if (o->nbr[f] == idx) { o->nbr[f] = -1; o->g.bc[f] = BC_EXTRAP; }
       else if (o->nbr[f] > idx) --o->nbr[f];
     }
-  for (double* p : b->q) (void)hipFree(p);
-  if (b->rv) (void)hipFree(b->rv);
+  free_block_state(b, nullptr);
   ctx->blocks.erase(std::remove(ctx->blocks.begin(), ctx->blocks.end(), b), ctx->blocks.end());
   delete b;
   ctx->desc_dirty = true;
@@ -777,32 +889,8 @@ int fvm_upload(fvm_block* b, const double* q, int host_ng) {
   if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
   fvm_ctx* ctx = b->ctx;
   if (int rc = set_device(ctx)) return rc;
-  const BlockGeom& g = b->g;
-  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
-  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
-  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
-  const int64_t hcs = hx * hy * hz;
-  std::vector<double> stage((size_t)b->state_elems, 0.0);
-  for (int c = 0; c < 5; ++c) {
-    int dc = c;
-    if (b->dim == 2) {
-      if (c == 3) {
-        for (int64_t j = 0; j < g.ny; ++j)
-          for (int64_t i = 0; i < g.nx; ++i) {
-            const double v = q[c * hcs + (j + host_ng) * hx + i + host_ng];
-            if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
-          }
-        continue;
-      }
-      if (c == 4) dc = 3;
-    }
-    for (int64_t k = 0; k < g.nz; ++k)
-      for (int64_t j = 0; j < g.ny; ++j) {
-        const double* src = q + c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + host_ng;
-        double* dst = stage.data() + dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + g.ox;
-        std::memcpy(dst, src, g.nx * sizeof(double));
-      }
-  }
+  std::vector<double> stage;
+  if (int rc = pack_state(b, q, host_ng, stage)) return rc;
   HIP_TRY(hipMemcpyAsync(b->q[0], stage.data(), b->state_elems * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -826,31 +914,11 @@ int fvm_download(fvm_block* b, double* q, int host_ng) {
       ctx->ghosts_stale = false;
     }
   }
-  const BlockGeom& g = b->g;
   std::vector<double> stage((size_t)b->state_elems);
   HIP_TRY(hipMemcpyAsync(stage.data(), b->q[0], b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  const int gl = std::min(host_ng, g.ng);
-  const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
-  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
-  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
-  const int64_t hcs = hx * hy * hz;
-  const int gzl = b->dim == 3 ? gl : 0;
-  std::memset(q, 0, sizeof(double) * 5 * hcs);
-  for (int c = 0; c < 5; ++c) {
-    if (b->dim == 2 && c == 3) continue;   // rho*w == 0
-    const int dc = (b->dim == 2 && c == 4) ? 3 : c;
-    for (int64_t k = -gzl; k < g.nz + gzl; ++k)
-      for (int64_t j = -gl; j < g.ny + gl; ++j)
-        for (int64_t i = -gl; i < g.nx + gl; ++i) {
-          // faces only: at most one index outside the real range
-          const int outside = (i < 0 || i >= g.nx) + (j < 0 || j >= g.ny) + (k < 0 || k >= g.nz);
-          if (outside > 1) continue;
-          q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + (i + host_ng)] =
-              stage[dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + (i + g.ox)];
-        }
-  }
+  unpack_state(b, stage.data(), q, host_ng);
   return FVM_OK;
 }
 
@@ -1186,6 +1254,7 @@ struct Job {
   int64_t id = 0;
   fvm_request req{};
   std::vector<double> q0;          // initial state (host layout of fvm_upload, host_ng 0)
+  std::vector<double> stage;       // device-layout upload staging, kept until the upload is known done
   std::vector<double> result;      // final state once done
   double t_submit = 0, t_admit = -1, t_done = -1;
   int steps = 0;
@@ -1193,6 +1262,7 @@ struct Job {
   double cell_updates = 0;
   int status = 0;                  // 0 pending, 1 active, 2 done, -1 failed
   int slot = -1;
+  int64_t first_batch = 0;         // first batch that advanced this solve (0: none yet)
   fvm_ctx* group = nullptr;
   fvm_block* blk = nullptr;
   size_t bytes = 0;
@@ -1201,6 +1271,23 @@ struct Job {
 // Kernel instantiation and scheme parameters: solves that agree on all of
 // them share one launch per stage.
 using GroupKey = std::tuple<int, int, int, int, int, int, double, double, double, int>;
+
+// Per group: end-of-batch event, pinned descriptor staging and pinned slot
+// snapshots, one table of max_active entries each per batch parity.  The
+// snapshot is copied on the group's stream between its batches, so it holds
+// the slots exactly as the batch left them.
+struct GroupRes {
+  hipEvent_t ev = nullptr;
+  BlockDesc* h_desc = nullptr;
+  SolveSlot* h_snap = nullptr;
+};
+
+// A retired solve's slot: reusable once every batch that may still reference
+// it (up to release_after) has completed.
+struct SlotHold {
+  int slot;
+  int64_t release_after;
+};
 
 }  // namespace
 
@@ -1215,10 +1302,19 @@ struct fvm_server {
   std::vector<Job*> active;
   std::map<int64_t, Job*> jobs;
   std::map<GroupKey, fvm_ctx*> groups;
+  std::map<fvm_ctx*, GroupRes> gres;
   SolveSlot* d_slots = nullptr;
-  std::vector<SolveSlot> h_slots;
   std::vector<int> free_slots;
-  int* d_ids = nullptr;     // slot ids of a group's active solves
+  std::vector<SlotHold> held;
+  int* d_ids = nullptr;           // [2][max_active] slot ids of a batch, per group segment
+  int* h_ids = nullptr;           // pinned staging of d_ids
+  SolveSlot* h_init = nullptr;    // pinned [max_active]: initial slot contents
+  hipStream_t cstream = nullptr;  // uploads, downloads, allocations of solve state
+  hipStream_t sstream = nullptr;  // joins the groups' streams at the end of a batch
+  hipEvent_t up_ev = nullptr;     // uploads of the latest admissions
+  hipEvent_t bev[2] = {nullptr, nullptr};   // every group's part of a batch done (per parity)
+  std::deque<int64_t> inflight;   // batches queued and not yet completed (at most 2)
+  int64_t batch_seq = 0;
   int chunk3 = 16, chunk2 = 8;    // k-chunk (3-D) and row chunk (2-D) of served blocks
   double total_cell_updates = 0;
 };
@@ -1240,13 +1336,42 @@ int server_group(fvm_server* s, const fvm_request& r, fvm_ctx** out) {
   if (int rc = fvm_scheme_set(ctx, r.recon, r.limiter, r.flux, r.rk, r.kappa, r.entropy_fix, r.gamma,
                               r.ref_compat)) { fvm_ctx_destroy(ctx); return rc; }
   if (int rc = fvm_set_numerics(ctx, r.numerics)) { fvm_ctx_destroy(ctx); return rc; }
+  // a table for every slot, so it never grows while batches are queued
+  GroupRes gr;
+  if (hipMalloc(&ctx->d_desc, (size_t)s->max_active * sizeof(BlockDesc)) != hipSuccess ||
+      hipHostMalloc((void**)&gr.h_desc, 2 * (size_t)s->max_active * sizeof(BlockDesc)) != hipSuccess ||
+      hipHostMalloc((void**)&gr.h_snap, 2 * (size_t)s->max_active * sizeof(SolveSlot)) != hipSuccess ||
+      hipEventCreateWithFlags(&gr.ev, hipEventDisableTiming) != hipSuccess) {
+    if (gr.h_desc) (void)hipHostFree(gr.h_desc);
+    if (gr.h_snap) (void)hipHostFree(gr.h_snap);
+    fvm_ctx_destroy(ctx);
+    return fail(FVM_ENOMEM, "cannot allocate a serving group's block table");
+  }
+  ctx->desc_cap = (size_t)s->max_active;
   s->groups[key] = ctx;
+  s->gres[ctx] = gr;
   *out = ctx;
   return FVM_OK;
 }
 
+// Upload a group's block table for the next batch: staged in the pinned table
+// of this batch's parity, copied on the group's stream behind the batches
+// already queued there (which still read the previous table).
+int server_sync_desc(fvm_server* s, fvm_ctx* g, int parity) {
+  BlockDesc* h = s->gres[g].h_desc + (size_t)parity * s->max_active;
+  fill_desc(g, h);
+  if (!g->blocks.empty())
+    HIP_TRY(hipMemcpyAsync(g->d_desc, h, g->blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                           g->stream));
+  g->desc_dirty = false;
+  return FVM_OK;
+}
+
 // Admit pending jobs in arrival order while a slot and the HBM budget allow.
+// Allocation, zeroing and upload of a solve's state run on the copy stream;
+// its group's stream waits for them before the solve's first step.
 int server_admit(fvm_server* s) {
+  std::vector<Job*> admitted;
   while (!s->pending.empty()) {
     Job* j = s->pending.front();
     if (s->free_slots.empty() || s->used + (double)j->bytes > s->budget) break;
@@ -1255,7 +1380,8 @@ int server_admit(fvm_server* s) {
     if (int rc = server_group(s, j->req, &g)) return rc;
     fvm_block* b = nullptr;
     const fvm_request& r = j->req;
-    if (int rc = fvm_block_create(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2], &b))
+    if (int rc = block_create_impl(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2],
+                                   s->cstream, &b))
       return rc;
     // one tile shape per group so its blocks share a launch; long k-marches
     // (the group's launch has tiles enough without splitting each block)
@@ -1264,28 +1390,22 @@ int server_admit(fvm_server* s) {
     for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
       if (r.bc[f] == FVM_BC_PERIODIC)
         if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
-    if (int rc = fvm_upload(b, j->q0.data(), 0)) return rc;
+    if (int rc = pack_state(b, j->q0.data(), 0, j->stage)) return rc;
     j->q0.clear();
     j->q0.shrink_to_fit();
+    HIP_TRY(hipMemcpyAsync(b->q[0], j->stage.data(), b->state_elems * sizeof(double),
+                           hipMemcpyHostToDevice, s->cstream));
+    b->uploaded = true;
     const int slot = s->free_slots.back();
     s->free_slots.pop_back();
-    SolveSlot& init = s->h_slots[slot];   // staging that outlives the async copy
+    SolveSlot& init = s->h_init[slot];
     init = SolveSlot{};
     init.t_end = r.t_end;
     init.cfl = r.cfl;
-    HIP_TRY(hipMemcpyAsync(s->d_slots + slot, &init, sizeof(SolveSlot), hipMemcpyHostToDevice, g->stream));
     b->dt_dev = &s->d_slots[slot].dt;
     b->smax_dev = &s->d_slots[slot].smax;
     b->halt_dev = &s->d_slots[slot].halt;
     g->desc_dirty = true;
-    if (int rc = sync_desc(g)) return rc;
-    if (int rc = fill_ghosts_all(g, 0)) return rc;
-    g->ghosts_dirty = false;
-    if (b->nc == 5)
-      hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
-    else
-      hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
-    HIP_TRY(hipGetLastError());
     j->slot = slot;
     j->group = g;
     j->blk = b;
@@ -1293,34 +1413,54 @@ int server_admit(fvm_server* s) {
     j->t_admit = now_s(s);
     s->used += (double)j->bytes;
     s->active.push_back(j);
+    admitted.push_back(j);
+  }
+  if (admitted.empty()) return FVM_OK;
+  HIP_TRY(hipEventRecord(s->up_ev, s->cstream));
+  std::vector<fvm_ctx*> waited;
+  for (Job* j : admitted) {
+    fvm_ctx* g = j->group;
+    if (std::find(waited.begin(), waited.end(), g) == waited.end()) {
+      HIP_TRY(hipStreamWaitEvent(g->stream, s->up_ev, 0));
+      waited.push_back(g);
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_slots + j->slot, s->h_init + j->slot, sizeof(SolveSlot),
+                           hipMemcpyHostToDevice, g->stream));
+    fvm_block* b = j->blk;
+    if (b->nc == 5)
+      hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
+    else
+      hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
+    HIP_TRY(hipGetLastError());
   }
   return FVM_OK;
 }
 
-// `nsteps` time steps of every active solve with one host synchronisation:
-// per group, the per-solve dt update and one batched launch per RK stage;
-// groups run on their own streams.  A solve that reaches its end time within
-// the run halts on the device and its tiles are skipped until it is retired.
-int server_step(fvm_server* s, int nsteps) {
+// Queue one batch: `nsteps` time steps of every active solve -- per group,
+// the per-solve dt update and one batched launch per RK stage, on the group's
+// stream -- then a copy of the slot table to pinned memory once every group's
+// part has run.  The host does not wait; a solve that reaches its end time
+// halts on the device and its tiles are skipped until it is retired.
+int server_enqueue(fvm_server* s, int nsteps) {
   std::map<fvm_ctx*, std::vector<int>> ids;
   for (Job* j : s->active) ids[j->group].push_back(j->slot);
-  std::vector<int> all_ids;
-  std::vector<std::pair<fvm_ctx*, int>> offs;
+  if (ids.empty()) return FVM_OK;
+  const int64_t bid = ++s->batch_seq;
+  const int par = (int)(bid & 1);
+  int* hid = s->h_ids + (size_t)par * s->max_active;
+  int* did = s->d_ids + (size_t)par * s->max_active;
+  int off = 0;
   for (auto& kv : ids) {
-    offs.push_back({kv.first, (int)all_ids.size()});
-    all_ids.insert(all_ids.end(), kv.second.begin(), kv.second.end());
-  }
-  if (all_ids.empty()) return FVM_OK;
-  HIP_TRY(hipMemcpy(s->d_ids, all_ids.data(), all_ids.size() * sizeof(int), hipMemcpyHostToDevice));
-  for (size_t gi = 0; gi < offs.size(); ++gi) {
-    fvm_ctx* g = offs[gi].first;
-    const int n = (int)ids[g].size();
-    g->ghosts_stale = true;
-    if (int rc = sync_desc(g)) return rc;
+    fvm_ctx* g = kv.first;
+    const int n = (int)kv.second.size();
+    if (g->desc_dirty)
+      if (int rc = server_sync_desc(s, g, par)) return rc;
+    std::memcpy(hid + off, kv.second.data(), n * sizeof(int));
+    HIP_TRY(hipMemcpyAsync(did + off, hid + off, n * sizeof(int), hipMemcpyHostToDevice, g->stream));
     const int nstage = g->rk == RK4 ? 4 : 3;
     for (int step = 0; step < nsteps; ++step) {
       hipLaunchKernelGGL(solve_dt_kernel, dim3((n + 63) / 64), dim3(64), 0, g->stream, s->d_slots,
-                         s->d_ids + offs[gi].second, n);
+                         did + off, n);
       HIP_TRY(hipGetLastError());
       for (int st = 1; st <= nstage; ++st) {
         StageArgs a{};
@@ -1339,18 +1479,46 @@ int server_step(fvm_server* s, int nsteps) {
         if (int rc = launch_stage_all(g, a)) return rc;
       }
     }
+    // the slots as this batch leaves them, before the next batch (already
+    // queued behind it on this stream) moves them on
+    GroupRes& gr = s->gres[g];
+    HIP_TRY(hipMemcpyAsync(gr.h_snap + (size_t)par * s->max_active, s->d_slots,
+                           (size_t)s->max_active * sizeof(SolveSlot), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipEventRecord(gr.ev, g->stream));
+    HIP_TRY(hipStreamWaitEvent(s->sstream, gr.ev, 0));
+    off += n;
   }
-  for (auto& kv : offs) HIP_TRY(hipStreamSynchronize(kv.first->stream));
+  HIP_TRY(hipEventRecord(s->bev[par], s->sstream));
+  for (Job* j : s->active)
+    if (!j->first_batch) j->first_batch = bid;
+  s->inflight.push_back(bid);
   return FVM_OK;
 }
 
-// Retire finished solves: download the state, release memory and slot.
-int server_retire(fvm_server* s) {
-  HIP_TRY(hipMemcpy(s->h_slots.data(), s->d_slots, s->h_slots.size() * sizeof(SolveSlot),
-                    hipMemcpyDeviceToHost));
+// Complete the oldest queued batch: wait for its slot snapshot, free the
+// slots no queued batch references any more, and retire the solves it
+// finished -- download the state (the solve is halted, so the batches still
+// queued leave it alone) and hand its arrays back to the device pool behind
+// those batches on the group's stream.
+int server_complete(fvm_server* s) {
+  const int64_t bid = s->inflight.front();
+  s->inflight.pop_front();
+  const int par = (int)(bid & 1);
+  HIP_TRY(hipEventSynchronize(s->bev[par]));
+  std::vector<SlotHold> still;
+  for (const SlotHold& h : s->held) {
+    if (h.release_after <= bid) s->free_slots.push_back(h.slot);
+    else still.push_back(h);
+  }
+  s->held.swap(still);
   std::vector<Job*> keep;
   for (Job* j : s->active) {
-    const SolveSlot& sl = s->h_slots[j->slot];
+    if (!j->first_batch || j->first_batch > bid) { keep.push_back(j); continue; }
+    if (!j->stage.empty()) {   // the batch ran behind the upload
+      j->stage.clear();
+      j->stage.shrink_to_fit();
+    }
+    const SolveSlot& sl = s->gres[j->group].h_snap[(size_t)par * s->max_active + j->slot];
     const bool done = sl.halt || sl.nan || !(sl.t < j->req.t_end - kEndTol * std::fabs(j->req.t_end));
     if (!done) { keep.push_back(j); continue; }
     const fvm_request& r = j->req;
@@ -1360,11 +1528,20 @@ int server_retire(fvm_server* s) {
     j->t_final = sl.t;
     j->cell_updates = (double)ncell * nstage * sl.steps;
     s->total_cell_updates += j->cell_updates;
+    fvm_block* b = j->blk;
+    std::vector<double> stage(b->state_elems);
+    HIP_TRY(hipMemcpyAsync(stage.data(), b->q[0], b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
+                           s->cstream));
+    HIP_TRY(hipStreamSynchronize(s->cstream));
     j->result.assign((size_t)5 * ncell, 0.0);
-    if (int rc = fvm_download(j->blk, j->result.data(), 0)) return rc;
-    fvm_block_destroy(j->blk);
+    unpack_state(b, stage.data(), j->result.data(), 0);
+    fvm_ctx* g = j->group;
+    g->blocks.erase(std::remove(g->blocks.begin(), g->blocks.end(), b), g->blocks.end());
+    g->desc_dirty = true;
+    free_block_state(b, g->stream);
+    delete b;
     j->blk = nullptr;
-    s->free_slots.push_back(j->slot);
+    s->held.push_back({j->slot, s->batch_seq});
     s->used -= (double)j->bytes;
     j->status = sl.nan ? -1 : 2;
     j->t_done = now_s(s);
@@ -1390,11 +1567,31 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
   s->t0 = Clock::now();
   if (const char* e = getenv("FVM_SERVE_CHUNK3")) s->chunk3 = std::max(1, atoi(e));
   if (const char* e = getenv("FVM_SERVE_CHUNK2")) s->chunk2 = std::max(1, atoi(e));
-  s->h_slots.assign(max_active, SolveSlot{});
   for (int i = max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
-  HIP_TRY(hipMalloc(&s->d_slots, max_active * sizeof(SolveSlot)));
-  HIP_TRY(hipMemset(s->d_slots, 0, max_active * sizeof(SolveSlot)));
-  HIP_TRY(hipMalloc(&s->d_ids, max_active * sizeof(int)));
+  // keep freed solve state in the device pool for the next admission
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t keep_all = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep_all);
+    // no hidden waits: memory freed behind queued batches is reused only
+    // once those batches have run
+    int no = 0;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no);
+  }
+  const size_t ns = (size_t)max_active;
+  if (hipMalloc(&s->d_slots, ns * sizeof(SolveSlot)) != hipSuccess ||
+      hipMalloc(&s->d_ids, 2 * ns * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_ids, 2 * ns * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_init, ns * sizeof(SolveSlot)) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->sstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->bev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->bev[1], hipEventDisableTiming) != hipSuccess) {
+    fvm_server_destroy(s);
+    return fail(FVM_ENOMEM, "cannot allocate the server's slot tables, streams or events");
+  }
+  HIP_TRY(hipMemset(s->d_slots, 0, ns * sizeof(SolveSlot)));
   *out = s;
   return FVM_OK;
 }
@@ -1402,10 +1599,24 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
 int fvm_server_destroy(fvm_server* s) {
   if (!s) return FVM_OK;
   (void)hipSetDevice(s->device);
+  (void)hipDeviceSynchronize();
   for (auto& kv : s->groups) fvm_ctx_destroy(kv.second);
+  for (auto& kv : s->gres) {
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+    if (kv.second.h_desc) (void)hipHostFree(kv.second.h_desc);
+    if (kv.second.h_snap) (void)hipHostFree(kv.second.h_snap);
+  }
   for (auto& kv : s->jobs) delete kv.second;
   (void)hipFree(s->d_slots);
   (void)hipFree(s->d_ids);
+  if (s->h_ids) (void)hipHostFree(s->h_ids);
+  if (s->h_init) (void)hipHostFree(s->h_init);
+  if (s->up_ev) (void)hipEventDestroy(s->up_ev);
+  for (hipEvent_t e : s->bev) if (e) (void)hipEventDestroy(e);
+  if (s->cstream) (void)hipStreamDestroy(s->cstream);
+  if (s->sstream) (void)hipStreamDestroy(s->sstream);
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, s->device) == hipSuccess && pool) (void)hipMemPoolTrimTo(pool, 0);
   delete s;
   return FVM_OK;
 }
@@ -1417,11 +1628,14 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   const int dim = r->n[2] == 1 ? 2 : 3;
   const int nc = dim == 3 ? 5 : 4;
   const double cs = (double)(r->n[0] + 6) * (r->n[1] + 6) * (dim == 3 ? r->n[2] + 6 : 1);
+  const size_t ncell = (size_t)r->n[0] * r->n[1] * r->n[2];
+  if (dim == 2)
+    for (size_t e = 0; e < ncell; ++e)
+      if (r->q0[3 * ncell + e] != 0.0) return fail(FVM_EINVAL, "2-D request: rho*w must be zero");
   Job* j = new Job();
   j->id = s->next_id++;
   j->req = *r;
   j->req.q0 = nullptr;
-  const size_t ncell = (size_t)r->n[0] * r->n[1] * r->n[2];
   j->q0.assign(r->q0, r->q0 + 5 * ncell);
   j->bytes = (size_t)(4.0 * nc * cs * sizeof(double));
   if ((double)j->bytes > s->budget) {
@@ -1443,9 +1657,13 @@ int fvm_server_poll(fvm_server* s, int iterations, int steps_per_sync, int* n_pe
   HIP_TRY(hipSetDevice(s->device));
   for (int it = 0; it < iterations; ++it) {
     if (int rc = server_admit(s)) return rc;
-    if (s->active.empty()) break;
-    if (int rc = server_step(s, steps_per_sync)) return rc;
-    if (int rc = server_retire(s)) return rc;
+    if (!s->active.empty())
+      if (int rc = server_enqueue(s, steps_per_sync)) return rc;
+    // keep the newest batch queued while the host works; with nothing left
+    // to step, finish everything
+    while (s->inflight.size() > 1 || (s->active.empty() && !s->inflight.empty()))
+      if (int rc = server_complete(s)) return rc;
+    if (s->active.empty() && s->inflight.empty()) break;
   }
   int done = 0;
   for (auto& kv : s->jobs) done += kv.second->status == 2 || kv.second->status == -1;

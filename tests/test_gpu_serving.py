@@ -101,6 +101,27 @@ def test_served_equals_plain_context():
         assert np.array_equal(q, qp)
 
 
+def test_arrivals_during_work():
+    """Requests submitted one by one between polls join while others run
+    (batches queued ahead of the host, slots and device memory reused once
+    the queued batches are done); each result equals its alone run."""
+    reqs = mixed_requests()
+    print("case serving arrivals between polls", flush=True)
+    alone = [run([r])[0] for r in reqs]
+    with fvmgpu.Server(max_active=3) as srv:
+        ids = []
+        for r in reqs:
+            ids.append(srv.submit(**r))
+            srv.poll(1, 2)
+        while srv.poll(1, 2)[2] < len(ids):
+            pass
+        got = [srv.result(i) for i in ids]
+    for (qa, ia), (qb, ib) in zip(alone, got):
+        assert ib["status"] == 2
+        assert ia["steps"] == ib["steps"]
+        assert np.array_equal(qa, qb)
+
+
 def test_steps_per_sync():
     """Several steps per host synchronisation: solves that finish mid-run
     halt on the device; results are unchanged."""
