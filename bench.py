@@ -444,8 +444,9 @@ def bench_tgv(args):
 # ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
-# end times per case kind: long enough that the offered load (50 / s)
-# exceeds what the GPU finishes, so the served rate measures capacity
+# end times per case kind.  At 50 arrivals / s the served run ends soon after
+# the last arrival (the rate is then set by the arrivals); the capacity run
+# queues every request at t = 0 and measures what the GPU sustains.
 SERVE_T_END = {"riemann2d": 0.1, "vortex3d": 1.0, "blob3d": 0.1}
 
 
@@ -546,6 +547,22 @@ def bench_serving(args):
     srv.close()
     served_value = cell_updates / makespan
 
+    # capacity: the same requests all queued at t = 0
+    srv = fvmgpu.Server(max_active=args.max_active)
+    t0 = time.perf_counter()
+    cap_ids = [srv.submit(**submit_args(r)) for r in reqs]
+    while srv.poll(1, args.steps_per_sync)[2] < len(reqs):
+        pass
+    cap_makespan = time.perf_counter() - t0
+    cap_lat = []
+    for i, rid in enumerate(cap_ids):
+        _, info = srv.result(rid)
+        if info["status"] != 2:
+            raise SystemExit(f"bench.py: request {i} ({reqs[i]['label']}) failed in the capacity run")
+        cap_lat.append(info["t_done"])
+    srv.close()
+    cap_value = cell_updates / cap_makespan
+
     # sequential plain solves: one context per request, run to t_end
     ts = time.perf_counter()
     per_kind = {}
@@ -598,6 +615,10 @@ def bench_serving(args):
               "sequential_fifo_p50_s": float(np.percentile(fifo, 50)),
               "sequential_fifo_p99_s": float(np.percentile(fifo, 99)),
               "speedup_vs_sequential": served_value / seq_value,
+              "capacity_value": cap_value, "capacity_makespan_s": cap_makespan,
+              "capacity_speedup_vs_sequential": cap_value / seq_value,
+              "capacity_p50_time_to_solution_s": float(np.percentile(cap_lat, 50)),
+              "capacity_p99_time_to_solution_s": float(np.percentile(cap_lat, 99)),
               "steps_per_sync": args.steps_per_sync, "max_active": args.max_active,
               "numerics": args.numerics,
               "sequential_by_kind": {k: {"requests": v[0], "cell_updates_per_s": v[1] / v[2]}

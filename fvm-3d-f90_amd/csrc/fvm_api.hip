@@ -9,11 +9,15 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -171,6 +175,25 @@ __global__ void ghost_fill_kernel(const BlockDesc* __restrict__ tab, int bi, int
     }
     const int64_t di = gidx(g, x[0], x[1], x[2]);
     for (int c = 0; c < nc; ++c) q[di + c * g.cs] = src[si + c * cs];
+  }
+}
+
+// Compact state (nc components of nx*ny*nz real vertices, i fastest) <->
+// the ghost-padded block layout.  `to_block`: src compact, dst block arrays;
+// otherwise the reverse.
+__global__ __launch_bounds__(256) void compact_kernel(const double* __restrict__ src,
+                                                     double* __restrict__ dst, BlockGeom g, int nc,
+                                                     int to_block) {
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nc * ncell;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / ncell, r = e % ncell;
+    const int i = (int)(r % g.nx);
+    const int j = (int)((r / g.nx) % g.ny);
+    const int k = (int)(r / ((int64_t)g.nx * g.ny));
+    const int64_t bi = c * g.cs + gidx(g, i, j, k);
+    if (to_block) dst[bi] = src[e];
+    else dst[e] = src[bi];
   }
 }
 
@@ -1253,9 +1276,14 @@ using Clock = std::chrono::steady_clock;
 struct Job {
   int64_t id = 0;
   fvm_request req{};
-  std::vector<double> q0;          // initial state (host layout of fvm_upload, host_ng 0)
-  std::vector<double> stage;       // device-layout upload staging, kept until the upload is known done
-  std::vector<double> result;      // final state once done
+  const double* src = nullptr;     // borrowed initial state, copied by the staging thread
+  double* q0 = nullptr;            // pinned: initial state (layout of q0 in fvm_request), kept
+  size_t q0_cap = 0;               // until a batch behind its upload has completed
+  std::atomic<int> staged{0};      // 1: q0 holds the initial state; -1: staging failed
+  double* res = nullptr;           // pinned: final state as downloaded (rho w of 2-D not written)
+  size_t res_cap = 0;
+  hipEvent_t res_ev = nullptr;     // download of res done
+  std::vector<double> result;      // final state, filled from res on the first fvm_server_result
   double t_submit = 0, t_admit = -1, t_done = -1;
   int steps = 0;
   double t_final = 0;
@@ -1317,12 +1345,90 @@ struct fvm_server {
   int64_t batch_seq = 0;
   int chunk3 = 16, chunk2 = 8;    // k-chunk (3-D) and row chunk (2-D) of served blocks
   double total_cell_updates = 0;
+  // pinned host buffers (capacity in doubles) and events kept for reuse, so
+  // that uploads and downloads are asynchronous without pinning memory per
+  // request
+  std::multimap<size_t, double*> pin_free;
+  size_t pin_free_doubles = 0;
+  std::mutex pin_mu;              // pin_free is shared with the staging thread
+  std::vector<hipEvent_t> ev_free;
+  // staging thread: copies borrowed initial states into pinned buffers
+  std::thread stager;
+  std::mutex st_mu;
+  std::condition_variable st_cv;
+  std::deque<Job*> to_stage;
+  bool st_stop = false;
 };
 
 namespace {
 
+// A pinned buffer of at least n doubles: a free one no more than twice that
+// large, else a new one.
+double* pin_get(fvm_server* s, size_t n, size_t* cap) {
+  std::lock_guard<std::mutex> lk(s->pin_mu);
+  auto it = s->pin_free.lower_bound(n);
+  if (it != s->pin_free.end() && it->first <= 2 * n) {
+    double* p = it->second;
+    *cap = it->first;
+    s->pin_free_doubles -= it->first;
+    s->pin_free.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  if (hipHostMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+  *cap = n;
+  return (double*)p;
+}
+
+// Keep a pinned buffer for reuse (up to 2^28 doubles, 2 GiB, in the free list).
+void pin_put(fvm_server* s, double* p, size_t cap) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(s->pin_mu);
+  s->pin_free.insert({cap, p});
+  s->pin_free_doubles += cap;
+  while (s->pin_free_doubles > (size_t(1) << 28) && !s->pin_free.empty()) {
+    auto last = std::prev(s->pin_free.end());
+    s->pin_free_doubles -= last->first;
+    (void)hipHostFree(last->second);
+    s->pin_free.erase(last);
+  }
+}
+
+hipEvent_t ev_get(fvm_server* s) {
+  if (!s->ev_free.empty()) {
+    hipEvent_t e = s->ev_free.back();
+    s->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+  return e;
+}
+
 double now_s(fvm_server* s) {
   return std::chrono::duration<double>(Clock::now() - s->t0).count();
+}
+
+// Staging thread: copy each borrowed initial state into a pinned buffer, in
+// submission order, so admission finds it ready and uploads it asynchronously.
+void stager_main(fvm_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Job* j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->st_mu);
+      s->st_cv.wait(lk, [s] { return s->st_stop || !s->to_stage.empty(); });
+      if (s->st_stop) return;
+      j = s->to_stage.front();
+      s->to_stage.pop_front();
+    }
+    const fvm_request& r = j->req;
+    const size_t n = (size_t)5 * r.n[0] * r.n[1] * r.n[2];
+    double* p = pin_get(s, n, &j->q0_cap);
+    if (p) std::memcpy(p, j->src, n * sizeof(double));
+    j->q0 = p;
+    j->staged.store(p ? 1 : -1, std::memory_order_release);
+  }
 }
 
 int server_group(fvm_server* s, const fvm_request& r, fvm_ctx** out) {
@@ -1374,6 +1480,14 @@ int server_admit(fvm_server* s) {
   std::vector<Job*> admitted;
   while (!s->pending.empty()) {
     Job* j = s->pending.front();
+    const int st = j->staged.load(std::memory_order_acquire);
+    if (st == 0) break;   // still being copied: later requests wait their turn
+    if (st < 0) {         // no pinned memory for it
+      s->pending.pop_front();
+      j->status = -1;
+      j->t_done = now_s(s);
+      continue;
+    }
     if (s->free_slots.empty() || s->used + (double)j->bytes > s->budget) break;
     s->pending.pop_front();
     fvm_ctx* g = nullptr;
@@ -1390,11 +1504,21 @@ int server_admit(fvm_server* s) {
     for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
       if (r.bc[f] == FVM_BC_PERIODIC)
         if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
-    if (int rc = pack_state(b, j->q0.data(), 0, j->stage)) return rc;
-    j->q0.clear();
-    j->q0.shrink_to_fit();
-    HIP_TRY(hipMemcpyAsync(b->q[0], j->stage.data(), b->state_elems * sizeof(double),
-                           hipMemcpyHostToDevice, s->cstream));
+    // the compact state goes to the stage array q[1] (unused until the first
+    // step) and a kernel scatters it into q[0]; the host copies nothing
+    const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
+    if (b->dim == 3) {
+      HIP_TRY(hipMemcpyAsync(b->q[1], j->q0, 5 * ncell * sizeof(double), hipMemcpyHostToDevice,
+                             s->cstream));
+    } else {   // 2-D: rho, rho u, rho v, E (rho w is zero)
+      HIP_TRY(hipMemcpyAsync(b->q[1], j->q0, 3 * ncell * sizeof(double), hipMemcpyHostToDevice,
+                             s->cstream));
+      HIP_TRY(hipMemcpyAsync(b->q[1] + 3 * ncell, j->q0 + 4 * ncell, ncell * sizeof(double),
+                             hipMemcpyHostToDevice, s->cstream));
+    }
+    hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[1], b->q[0], b->g,
+                       b->nc, 1);
+    HIP_TRY(hipGetLastError());
     b->uploaded = true;
     const int slot = s->free_slots.back();
     s->free_slots.pop_back();
@@ -1514,9 +1638,9 @@ int server_complete(fvm_server* s) {
   std::vector<Job*> keep;
   for (Job* j : s->active) {
     if (!j->first_batch || j->first_batch > bid) { keep.push_back(j); continue; }
-    if (!j->stage.empty()) {   // the batch ran behind the upload
-      j->stage.clear();
-      j->stage.shrink_to_fit();
+    if (j->q0) {   // the batch ran behind the upload
+      pin_put(s, j->q0, j->q0_cap);
+      j->q0 = nullptr;
     }
     const SolveSlot& sl = s->gres[j->group].h_snap[(size_t)par * s->max_active + j->slot];
     const bool done = sl.halt || sl.nan || !(sl.t < j->req.t_end - kEndTol * std::fabs(j->req.t_end));
@@ -1528,16 +1652,30 @@ int server_complete(fvm_server* s) {
     j->t_final = sl.t;
     j->cell_updates = (double)ncell * nstage * sl.steps;
     s->total_cell_updates += j->cell_updates;
+    // gather the real vertices into the (idle) stage array q[1], then one
+    // asynchronous copy to pinned memory; the block's arrays are released
+    // behind it
     fvm_block* b = j->blk;
-    std::vector<double> stage(b->state_elems);
-    HIP_TRY(hipMemcpyAsync(stage.data(), b->q[0], b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
-                           s->cstream));
-    HIP_TRY(hipStreamSynchronize(s->cstream));
-    j->result.assign((size_t)5 * ncell, 0.0);
-    unpack_state(b, stage.data(), j->result.data(), 0);
+    j->res = pin_get(s, (size_t)5 * ncell, &j->res_cap);
+    j->res_ev = ev_get(s);
+    if (!j->res || !j->res_ev) return fail(FVM_ENOMEM, "cannot allocate pinned result memory");
+    hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[0], b->q[1], b->g,
+                       b->nc, 0);
+    HIP_TRY(hipGetLastError());
+    if (b->dim == 3) {
+      HIP_TRY(hipMemcpyAsync(j->res, b->q[1], 5 * ncell * sizeof(double), hipMemcpyDeviceToHost,
+                             s->cstream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(j->res, b->q[1], 3 * ncell * sizeof(double), hipMemcpyDeviceToHost,
+                             s->cstream));
+      HIP_TRY(hipMemcpyAsync(j->res + 4 * ncell, b->q[1] + 3 * ncell, ncell * sizeof(double),
+                             hipMemcpyDeviceToHost, s->cstream));
+    }
+    HIP_TRY(hipEventRecord(j->res_ev, s->cstream));
     fvm_ctx* g = j->group;
     g->blocks.erase(std::remove(g->blocks.begin(), g->blocks.end(), b), g->blocks.end());
     g->desc_dirty = true;
+    HIP_TRY(hipStreamWaitEvent(g->stream, j->res_ev, 0));
     free_block_state(b, g->stream);
     delete b;
     j->blk = nullptr;
@@ -1592,12 +1730,21 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
     return fail(FVM_ENOMEM, "cannot allocate the server's slot tables, streams or events");
   }
   HIP_TRY(hipMemset(s->d_slots, 0, ns * sizeof(SolveSlot)));
+  s->stager = std::thread(stager_main, s);
   *out = s;
   return FVM_OK;
 }
 
 int fvm_server_destroy(fvm_server* s) {
   if (!s) return FVM_OK;
+  if (s->stager.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->st_mu);
+      s->st_stop = true;
+    }
+    s->st_cv.notify_all();
+    s->stager.join();
+  }
   (void)hipSetDevice(s->device);
   (void)hipDeviceSynchronize();
   for (auto& kv : s->groups) fvm_ctx_destroy(kv.second);
@@ -1606,7 +1753,15 @@ int fvm_server_destroy(fvm_server* s) {
     if (kv.second.h_desc) (void)hipHostFree(kv.second.h_desc);
     if (kv.second.h_snap) (void)hipHostFree(kv.second.h_snap);
   }
-  for (auto& kv : s->jobs) delete kv.second;
+  for (auto& kv : s->jobs) {
+    Job* j = kv.second;
+    if (j->q0) (void)hipHostFree(j->q0);
+    if (j->res) (void)hipHostFree(j->res);
+    if (j->res_ev) (void)hipEventDestroy(j->res_ev);
+    delete j;
+  }
+  for (auto& kv : s->pin_free) (void)hipHostFree(kv.second);
+  for (hipEvent_t e : s->ev_free) (void)hipEventDestroy(e);
   (void)hipFree(s->d_slots);
   (void)hipFree(s->d_ids);
   if (s->h_ids) (void)hipHostFree(s->h_ids);
@@ -1636,7 +1791,17 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   j->id = s->next_id++;
   j->req = *r;
   j->req.q0 = nullptr;
-  j->q0.assign(r->q0, r->q0 + 5 * ncell);
+  if (r->borrow) {
+    j->src = r->q0;
+  } else {
+    j->q0 = pin_get(s, 5 * ncell, &j->q0_cap);
+    if (!j->q0) {
+      delete j;
+      return fail(FVM_ENOMEM, "cannot allocate pinned memory for the request");
+    }
+    std::memcpy(j->q0, r->q0, 5 * ncell * sizeof(double));
+    j->staged.store(1, std::memory_order_release);
+  }
   j->bytes = (size_t)(4.0 * nc * cs * sizeof(double));
   if ((double)j->bytes > s->budget) {
     delete j;
@@ -1645,6 +1810,13 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   j->t_submit = now_s(s);
   s->jobs[j->id] = j;
   s->pending.push_back(j);
+  if (r->borrow) {
+    {
+      std::lock_guard<std::mutex> lk(s->st_mu);
+      s->to_stage.push_back(j);
+    }
+    s->st_cv.notify_one();
+  }
   if (id) *id = j->id;
   return FVM_OK;
 }
@@ -1685,6 +1857,17 @@ int fvm_server_result(fvm_server* s, int64_t id, double* q_out, double info[8]) 
   }
   if (j->status == 0 || j->status == 1) return fail(FVM_ESTATE, "request %lld not finished", (long long)id);
   if (j->status < 0) return fail(FVM_ENAN, "request %lld diverged (NaN or non-positive dt)", (long long)id);
+  if (j->res) {   // first call: wait for the download, keep the state, release the pinned copy
+    HIP_TRY(hipEventSynchronize(j->res_ev));
+    const fvm_request& r = j->req;
+    const size_t ncell = (size_t)(r.n[0] * r.n[1] * r.n[2]);
+    j->result.assign(j->res, j->res + 5 * ncell);
+    if (r.n[2] == 1) std::fill(j->result.begin() + 3 * ncell, j->result.begin() + 4 * ncell, 0.0);
+    pin_put(s, j->res, j->res_cap);
+    j->res = nullptr;
+    s->ev_free.push_back(j->res_ev);
+    j->res_ev = nullptr;
+  }
   if (q_out) std::memcpy(q_out, j->result.data(), j->result.size() * sizeof(double));
   return FVM_OK;
 }

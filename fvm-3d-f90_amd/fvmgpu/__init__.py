@@ -57,7 +57,7 @@ class Request(ctypes.Structure):
                 ("entropy_fix", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("ref_compat", ctypes.c_int), ("numerics", ctypes.c_int),
                 ("bc", ctypes.c_int * 6), ("t_end", ctypes.c_double), ("cfl", ctypes.c_double),
-                ("q0", ctypes.c_void_p)]
+                ("q0", ctypes.c_void_p), ("borrow", ctypes.c_int)]
 
 
 _lib = None
@@ -271,11 +271,13 @@ class Server:
         _check(_lib.fvm_server_create(device, float(hbm_budget), int(max_active), ctypes.byref(h)))
         self._h = h
         self.shapes = {}
+        self._hold = {}     # borrowed initial states, kept until the request is done
 
     def close(self):
         if self._h:
             _lib.fvm_server_destroy(self._h)
             self._h = None
+        self._hold.clear()
 
     def __enter__(self):
         return self
@@ -285,8 +287,11 @@ class Server:
 
     def submit(self, q0, spacing, t_end, cfl, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE,
                rk=RK_SSP3, kappa=0.33333, entropy_fix=0.2, gamma=1.4, ref_compat=1,
-               numerics=NUMERICS_EXACT, periodic=(False, False, False)):
-        """q0: (5, nk, nj, ni) conserved state.  Returns the request id."""
+               numerics=NUMERICS_EXACT, periodic=(False, False, False), borrow=True):
+        """q0: (5, nk, nj, ni) conserved state.  Returns the request id.
+        borrow: the server's staging thread copies q0 after submit returns
+        (the array is held here until the result is taken); otherwise submit
+        copies it."""
         q0 = np.ascontiguousarray(q0, dtype=np.float64)
         _, nk, nj, ni = q0.shape
         bc = [0] * 6
@@ -295,10 +300,12 @@ class Server:
                 bc[2 * d] = bc[2 * d + 1] = BC_PERIODIC
         r = Request((ni, nj, nk), tuple(float(x) for x in spacing), recon, limiter, flux, rk,
                     kappa, entropy_fix, gamma, ref_compat, numerics, tuple(bc), float(t_end),
-                    float(cfl), q0.ctypes.data)
+                    float(cfl), q0.ctypes.data, 1 if borrow else 0)
         rid = ctypes.c_int64()
         _check(_lib.fvm_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
         self.shapes[rid.value] = q0.shape
+        if borrow:
+            self._hold[rid.value] = q0
         return rid.value
 
     def poll(self, iterations=1, steps_per_sync=1):
@@ -315,6 +322,7 @@ class Server:
             _check(_lib.fvm_server_result(self._h, rid, None, info))
         q = np.zeros(self.shapes[rid])
         _check(_lib.fvm_server_result(self._h, rid, q.ctypes.data, info))
+        self._hold.pop(rid, None)
         keys = ("t_submit", "t_admit", "t_done", "steps", "cell_updates", "t_final", "status",
                 "bytes")
         return q, dict(zip(keys, info[:]))

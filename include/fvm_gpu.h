@@ -218,7 +218,11 @@ typedef struct fvm_request {
   int ref_compat, numerics;
   int bc[6];                /* FVM_BC_EXTRAP / FVM_BC_PERIODIC per face         */
   double t_end, cfl;
-  const double *q0;         /* (5, nk, nj, ni) conserved state, copied at submit */
+  const double *q0;         /* (5, nk, nj, ni) conserved state                  */
+  int borrow;               /* 0: q0 is copied by fvm_server_submit.  1: q0 is
+                               copied later by the server's staging thread; it
+                               must stay valid and unchanged until the request
+                               is done (fvm_server_poll counts it done)         */
 } fvm_request;
 
 /* hbm_budget_bytes <= 0: 80 % of the free device memory. */
