@@ -4,8 +4,10 @@
 // residual of all swept directions and the Runge-Kutta combine of that stage:
 //   rhs = ((0 - dFx/dx) - dFy/dy) - dFz/dz          (class_space_discret.f90:678-732)
 //   out = RK combine(q0, qin, acc, rhs)             (scheme_Runge_Kutta.f90:30-63)
-// and scatters the new values into the output's ghost layers, so the next
-// stage reads valid ghosts without a separate launch.
+// and writes the new values of the owned vertices.  Readers map a point
+// outside the block to the vertex its ghost would copy (bmap), so no ghost
+// layer is written or read, except across a connected face: there the stage
+// writes its boundary vertices into the neighbour's ghost layers.
 //
 // Data: structure-of-arrays FP64, i fastest, each component a separate
 // ghost-inclusive array; loads along i are coalesced.
