@@ -1343,7 +1343,7 @@ struct fvm_server {
   hipEvent_t bev[2] = {nullptr, nullptr};   // every group's part of a batch done (per parity)
   std::deque<int64_t> inflight;   // batches queued and not yet completed (at most 2)
   int64_t batch_seq = 0;
-  int chunk3 = 16, chunk2 = 8;    // k-chunk (3-D) and row chunk (2-D) of served blocks
+  int chunk3 = 8, chunk2 = 8;     // k-chunk (3-D) and row chunk (2-D) of served blocks
   double total_cell_updates = 0;
   // pinned host buffers (capacity in doubles) and events kept for reuse, so
   // that uploads and downloads are asynchronous without pinning memory per
@@ -1497,9 +1497,11 @@ int server_admit(fvm_server* s) {
     if (int rc = block_create_impl(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2],
                                    s->cstream, &b))
       return rc;
-    // one tile shape per group so its blocks share a launch; long k-marches
-    // (the group's launch has tiles enough without splitting each block)
-    if (b->dim == 3) plan_launch(b, 1, s->chunk3);
+    // one tile shape per group so its blocks share a launch (WENO: 16 x 16,
+    // the shape the autotuner picks for it on 32-128 vertex blocks; otherwise
+    // 32 x 8); long k-marches (the group's launch has tiles enough without
+    // splitting each block)
+    if (b->dim == 3) plan_launch(b, r.recon == RECON_WENO ? 3 : 1, s->chunk3);
     else plan_launch(b, -1, s->chunk2);
     for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
       if (r.bc[f] == FVM_BC_PERIODIC)
