@@ -53,6 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fvm-3d-f90_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (published)
 B_ALG = {"rk4_3d": 160.0, "ssp3_3d": 8.0 / 3.0 * 40.0, "ssp3_2d": 8.0 / 3.0 * 32.0}
 METRIC = "FP64 cell-updates/s per RK stage"
 
@@ -191,13 +192,22 @@ def bench_vortex(args):
     sec = t1 - t0
     ncell = float(n) ** 3
     value = ncell * 4 * steps / sec
-    traffic = None
+    traffic, flops = None, None
     tpath = os.path.join(ROOT, "profiles", "stage3d_muscl_roe_vortex128_traffic.json")
     if os.path.exists(tpath) and n == 128:
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        if args.numerics == "fast":
+            flops = prof.get("fp64_flops_per_cell_update")
     roof = roofline(stage_ms, launches, ncell, B_ALG["rk4_3d"], traffic,
                     f"fvm::{args.numerics}::stage3d_kernel<ReconMuscl, 300, {tile[0]}, {tile[1]}>")
+    if flops:
+        # the co-limiting roof: counted FP64 VALU work against the vector FP64 peak
+        tflops = flops * ncell / (roof["avg_launch_us"] * 1e-6) / 1e12
+        roof["fp64_valu"] = {"flops_per_cell_update": flops, "achieved": tflops,
+                             "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tflops / FP64_VECTOR_PEAK_TFLOPS}
     config = {"workload": f"isentropic-vortex-3d {n}^3 periodic, MUSCL-vanAlbada + Roe, "
                           f"RK4, CFL {cfl}, FP64",
               "grid": [n, n, n], "stages_per_step": 4, "dt_last": dt_last,

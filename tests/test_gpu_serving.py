@@ -104,14 +104,15 @@ def test_served_equals_plain_context():
 def test_arrivals_during_work():
     """Requests submitted one by one between polls join while others run
     (batches queued ahead of the host, slots and device memory reused once
-    the queued batches are done); each result equals its alone run."""
+    the queued batches are done); each result equals its alone run.  Every
+    other request is copied at submit instead of borrowed."""
     reqs = mixed_requests()
     print("case serving arrivals between polls", flush=True)
     alone = [run([r])[0] for r in reqs]
     with fvmgpu.Server(max_active=3) as srv:
         ids = []
-        for r in reqs:
-            ids.append(srv.submit(**r))
+        for i, r in enumerate(reqs):
+            ids.append(srv.submit(**r, borrow=i % 2 == 0))
             srv.poll(1, 2)
         while srv.poll(1, 2)[2] < len(ids):
             pass
