@@ -111,7 +111,7 @@ struct fvm_ctx {
     hipGraphExec_t exec = nullptr;
     uint64_t gen = 0;
     bool cfl_mode = false;
-  } graphs[2];
+  } graphs[3];   // per norm mode of launch_step
   double* d_norm = nullptr;               // per-tile partial sums of rhs^2
   size_t norm_cap = 0;                    // doubles
   std::vector<double> h_norm;
@@ -220,14 +220,11 @@ __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__
   if (threadIdx.x == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(s));
 }
 
-// Relative tolerance on t_end: a run whose time is this close counts as done
-// (t + (t_end - t) can round one ulp short of t_end).
-constexpr double kEndTol = 1.0e-14;
-
-// Time step of one solve for the step about to start: dt = min(cfl / smax,
-// t_end - t), t += dt; clears the wave-speed accumulator so the step's last
-// stage collects the next maximum.  Once t reaches t_end the slot halts and
-// the step's stage launches return at once.
+// Time step of one served solve for the step about to start: dt =
+// min(cfl / smax, t_end - t), t += dt; clears the wave-speed accumulator so
+// the step's last stage collects the next maximum.  Once t reaches t_end the
+// slot halts and the step's stage launches skip the solve.  (A plain
+// context's stages do the same work themselves: stage_clock.)
 __device__ __forceinline__ void advance_slot(SolveSlot& s) {
   if (s.halt) return;
   // (an infinite t_end never ends: inf - tol * inf would be NaN)
@@ -242,8 +239,6 @@ __device__ __forceinline__ void advance_slot(SolveSlot& s) {
   s.smax = 0ull;
   s.steps = s.steps + 1;
 }
-
-__global__ void cfl_dt_kernel(SolveSlot* slot) { advance_slot(*slot); }
 
 // Fixed time step: the stage kernels read dt from the slot in either mode.
 __global__ void set_dt_kernel(SolveSlot* slot, double dt) { slot->dt = dt; }
@@ -665,14 +660,12 @@ int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
   return FVM_OK;
 }
 
-// One time step of every block: the CFL dt update (CFL mode), then the RK
-// stages.  `with_norm`: the last stage leaves its residual norm partials.
-// With timing on, each stage launch is bracketed by events (ev index *nev).
-int launch_step(fvm_ctx* ctx, bool cfl_mode, bool with_norm, size_t* nev) {
-  if (cfl_mode) {
-    hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot);
-    HIP_TRY(hipGetLastError());
-  }
+// One time step of every block: the RK stages (in CFL mode the first one
+// computes dt from the device time slot and the second commits the step).
+// `norm_mode`: 0 no residual norms, 1 the last stage leaves its norm
+// partials, 2 only if the step reached t_end.  With timing on, each stage
+// launch is bracketed by events (ev index *nev).
+int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   for (int st = 1; st <= nstage; ++st) {
     const bool last = st == nstage;
@@ -695,9 +688,13 @@ int launch_step(fvm_ctx* ctx, bool cfl_mode, bool with_norm, size_t* nev) {
         default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
       }
     }
-    a.norm_part = (last && with_norm) ? ctx->d_norm : nullptr;
+    a.norm_part = (last && norm_mode) ? ctx->d_norm : nullptr;
+    a.norm_if_end = last && norm_mode == 2 ? 1 : 0;
     a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
     a.halt = cfl_mode ? &ctx->d_slot->halt : nullptr;
+    a.slot = cfl_mode ? ctx->d_slot : nullptr;
+    a.cfl_dt = cfl_mode && st == 1 ? 1 : 0;
+    a.commit = cfl_mode && st == 2 ? 1 : 0;
     const bool ev = ctx->timing && nev;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_stage_all(ctx, a)) return rc;
@@ -711,8 +708,8 @@ int launch_step(fvm_ctx* ctx, bool cfl_mode, bool with_norm, size_t* nev) {
 // configuration changed.  No memset or copy is captured: the step is kernel
 // launches only, reading dt, the CFL maximum and the halt flag from device
 // memory.
-int step_graph(fvm_ctx* ctx, bool cfl_mode, bool with_norm, hipGraphExec_t* out) {
-  fvm_ctx::StepGraph& sg = ctx->graphs[with_norm ? 1 : 0];
+int step_graph(fvm_ctx* ctx, bool cfl_mode, int norm_mode, hipGraphExec_t* out) {
+  fvm_ctx::StepGraph& sg = ctx->graphs[norm_mode];
   if (sg.exec && sg.gen == ctx->gen && sg.cfl_mode == cfl_mode) {
     *out = sg.exec;
     return FVM_OK;
@@ -722,7 +719,7 @@ int step_graph(fvm_ctx* ctx, bool cfl_mode, bool with_norm, hipGraphExec_t* out)
     sg.exec = nullptr;
   }
   HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = launch_step(ctx, cfl_mode, with_norm, nullptr);
+  const int rc = launch_step(ctx, cfl_mode, norm_mode, nullptr);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
   if (rc) {
@@ -1005,13 +1002,15 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   }
   ctx->ghosts_stale = true;
   for (int s = 0; s < nsteps; ++s) {
-    const bool with_norm = s == nsteps - 1 || may_halt;
+    // the call's last step reports its norms; an earlier one does if it
+    // ends the run (the steps after it are skipped on the device)
+    const int norm_mode = s == nsteps - 1 ? 1 : (may_halt ? 2 : 0);
     if (ctx->use_graph && !ctx->timing) {
       hipGraphExec_t exec = nullptr;
-      if (int rc = step_graph(ctx, cfl_mode, with_norm, &exec)) return rc;
+      if (int rc = step_graph(ctx, cfl_mode, norm_mode, &exec)) return rc;
       HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     } else {
-      if (int rc = launch_step(ctx, cfl_mode, with_norm, &nev)) return rc;
+      if (int rc = launch_step(ctx, cfl_mode, norm_mode, &nev)) return rc;
     }
   }
 

@@ -132,6 +132,49 @@ __device__ __forceinline__ void rk_apply_all(int stage, const RKCoef& k, const d
   }
 }
 
+// Per-launch time-slot work of a plain context's stage (StageArgs::slot);
+// returns false when the launch has nothing to do (the run reached its end
+// time).  The first stage of a CFL step computes dt with advance_slot's
+// operations in every workgroup (workgroup 0 stores it, and the halt and NaN
+// flags); the second advances t and the step count and clears the CFL maximum
+// the last stage accumulates (no other workgroup reads them meanwhile); the
+// last stage of a step that may be skipped leaves norms only if it reached
+// t_end, read from the committed t.
+__device__ __forceinline__ bool stage_clock(const StageArgs& a, double& dt, bool& want_norm) {
+  want_norm = a.norm_part != nullptr;
+  if (a.halt && *a.halt) return false;
+  SolveSlot* S = a.slot;
+  if (!S) return true;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (a.cfl_dt) {
+    const double t = S->t, t_end = S->t_end;
+    if (isfinite(t_end) && !(t < t_end - kEndTol * fabs(t_end))) {
+      if (lead) S->halt = 1;
+      return false;
+    }
+    const double sm = __longlong_as_double((long long)S->smax);
+    double d = S->cfl / sm;
+    const bool bad = !(d > 0.0) || !(d < 1.0e300);
+    const double rest = t_end - t;
+    if (d > rest) d = rest;
+    dt = d;
+    if (lead) {
+      S->dt = d;
+      if (bad) S->nan = 1;
+    }
+  }
+  if (a.commit && lead) {
+    S->t = S->t + S->dt;
+    S->steps = S->steps + 1;
+    S->smax = 0ull;
+  }
+  if (a.norm_if_end && want_norm) {
+    const double t = S->t, t_end = S->t_end;
+    want_norm = !(t < t_end - kEndTol * fabs(t_end));
+  }
+  return true;
+}
+
 // Loads the operands rk_apply_all needs at one vertex (uniform stage): PARTS
 // bit 0 the RK base state q0, bit 1 the stage input or accumulator x.
 template <int NC, int PARTS>
@@ -432,7 +475,9 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
   const int t = threadIdx.x;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
-  if (a.halt && *a.halt) return;
+  double dt_step = 0.0;
+  bool want_norm = false;
+  if (!stage_clock(a, dt_step, want_norm)) return;
   const DescTable tab = as_const_table(a.blocks);
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
@@ -440,7 +485,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
     const auto& B = tab[bi];
     if (B.halt && *B.halt) continue;   // uniform over the workgroup
     const BlockGeom g = load_geom(B.g);
-    const double dt = *B.dt_dev;
+    const double dt = a.cfl_dt ? dt_step : *B.dt_dev;
     const double* __restrict__ rvp = B.rv;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
@@ -560,7 +605,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
       __syncthreads();
     }
 
-    if (a.norm_part) {
+    if (want_norm) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
         const double s = wg_sum<NT>(nrm[m], red);
@@ -634,7 +679,9 @@ void stage3d_kernel(StageArgs a) {
   const int wave = t / 64;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
-  if (a.halt && *a.halt) return;
+  double dt_step = 0.0;
+  bool want_norm = false;
+  if (!stage_clock(a, dt_step, want_norm)) return;
   const DescTable tab = as_const_table(a.blocks);
   for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
     const int tile = a.tile0 + xcd_remap(it, a.ntiles);
@@ -642,7 +689,7 @@ void stage3d_kernel(StageArgs a) {
     const auto& B = tab[bi];
     if (B.halt && *B.halt) continue;   // uniform over the workgroup
     const BlockGeom g = load_geom(B.g);
-    const double dt = *B.dt_dev;
+    const double dt = a.cfl_dt ? dt_step : *B.dt_dev;
     const double* __restrict__ rvp = B.rv;
     const RKCoef kc = rk_coef(a.stage, dt);
     double smax = 0.0;
@@ -722,7 +769,6 @@ void stage3d_kernel(StageArgs a) {
       }
     }
 
-    const bool want_norm = a.norm_part != nullptr;
     if (want_norm) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) Nrm[m * NT + t] = 0.0;
@@ -910,7 +956,7 @@ void stage3d_kernel(StageArgs a) {
       __syncthreads();
     }
 
-    if (a.norm_part) {
+    if (want_norm) {
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
         const double s = wg_sum<NT>(Nrm[m * NT + t], red);

@@ -81,6 +81,22 @@ enum : int {
   ST_RK4_4 = 6,  // out = acc + dt/6 r
 };
 
+// Relative tolerance on t_end: a run whose time is this close counts as done
+// (t + (t_end - t) can round one ulp short of t_end).
+constexpr double kEndTol = 1.0e-14;
+
+// Time-stepping state of one solve, kept on the device so a run of steps
+// needs no host round trip: dt = min(cfl / smax, t_end - t), t += dt, and
+// halt once t reaches t_end.
+struct SolveSlot {
+  double t, t_end, cfl, dt;
+  unsigned long long smax;        // CFL wave-speed max (bits of a non-negative double)
+  int steps;
+  int nan;                        // dt was NaN, zero or infinite
+  int halt;                       // t reached t_end: later steps are skipped
+  int pad;
+};
+
 struct StageArgs {
   const BlockDesc* blocks;        // descriptor table (device memory)
   int nblocks;                    // blocks in the table
@@ -94,18 +110,12 @@ struct StageArgs {
   unsigned long long* smax;       // CFL wave-speed max of the output, or null
   int block_smax;                 // 1: accumulate into each block's own smax instead
   const int* halt;                // if non-null and set, the launch does nothing (run ended)
-};
-
-// Time-stepping state of one solve, kept on the device so a run of steps
-// needs no host round trip: dt = min(cfl / smax, t_end - t), t += dt, and
-// halt once t reaches t_end.
-struct SolveSlot {
-  double t, t_end, cfl, dt;
-  unsigned long long smax;        // CFL wave-speed max (bits of a non-negative double)
-  int steps;
-  int nan;                        // dt was NaN, zero or infinite
-  int halt;                       // t reached t_end: later steps are skipped
-  int pad;
+  // a plain context's time slot in CFL mode (null otherwise): the first stage
+  // of a step computes dt from it (cfl_dt), the second advances t and the
+  // step count and clears the CFL maximum (commit); norm_if_end: the last
+  // stage leaves norm partials only if the step reached t_end
+  SolveSlot* slot;
+  int cfl_dt, commit, norm_if_end;
 };
 
 }  // namespace fvm
