@@ -118,8 +118,10 @@ struct fvm_ctx {
   double norms[6] = {0, 0, 0, 0, 0, 0};
   int nan_flag = 0;
   bool have_norms = false;
-  // stage-kernel timing (HIP events around every stage launch)
-  bool timing = false;
+  // stage-kernel timing: HIP events around the stage launches of every
+  // timing_every-th step (0: off); timing_step counts steps since enabled
+  int timing_every = 0;
+  int64_t timing_step = 0;
   std::vector<hipEvent_t> ev;
   double stage_ms = 0.0;
   int64_t stage_launches = 0;
@@ -695,7 +697,7 @@ int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
     a.slot = cfl_mode ? ctx->d_slot : nullptr;
     a.cfl_dt = cfl_mode && st == 1 ? 1 : 0;
     a.commit = cfl_mode && st == 2 ? 1 : 0;
-    const bool ev = ctx->timing && nev;
+    const bool ev = nev != nullptr;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_stage_all(ctx, a)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
@@ -992,8 +994,9 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
 
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   size_t nev = 0;
-  if (ctx->timing) {
-    const size_t need_ev = 2 * (size_t)nsteps * nstage;
+  std::vector<int> ev_step;   // step of each timed launch pair
+  if (ctx->timing_every) {
+    const size_t need_ev = 2 * (size_t)((nsteps + ctx->timing_every - 1) / ctx->timing_every + 1) * nstage;
     while (ctx->ev.size() < need_ev) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -1005,12 +1008,15 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     // the call's last step reports its norms; an earlier one does if it
     // ends the run (the steps after it are skipped on the device)
     const int norm_mode = s == nsteps - 1 ? 1 : (may_halt ? 2 : 0);
-    if (ctx->use_graph && !ctx->timing) {
+    const bool timed = ctx->timing_every && (ctx->timing_step + s) % ctx->timing_every == 0;
+    if (ctx->use_graph && !timed) {
       hipGraphExec_t exec = nullptr;
       if (int rc = step_graph(ctx, cfl_mode, norm_mode, &exec)) return rc;
       HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     } else {
-      if (int rc = launch_step(ctx, cfl_mode, norm_mode, &nev)) return rc;
+      const size_t before = nev;
+      if (int rc = launch_step(ctx, cfl_mode, norm_mode, timed ? &nev : nullptr)) return rc;
+      for (size_t e = before; e < nev; e += 2) ev_step.push_back(s);
     }
   }
 
@@ -1035,15 +1041,19 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   }
   ctx->steps_done += ran;
   if (dt_last) *dt_last = dt_host;
-  if (ctx->timing) {
+  if (ctx->timing_every) {
+    // launches of steps skipped on the device (past t_end) did no work
+    double cells = 0.0;
+    for (fvm_block* b : ctx->blocks) cells += (double)b->g.nx * b->g.ny * b->g.nz;
     for (size_t e = 0; e < nev; e += 2) {
+      if (ev_step[e / 2] >= ran) continue;
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[e], ctx->ev[e + 1]));
       ctx->stage_ms += ms;
+      ctx->stage_launches += 1;
+      ctx->cell_updates += cells;
     }
-    ctx->stage_launches += (int64_t)(nev / 2);
-    for (fvm_block* b : ctx->blocks)
-      ctx->cell_updates += (double)b->g.nx * b->g.ny * b->g.nz * nstage * ran;
+    ctx->timing_step += nsteps;
   }
   double res_abs[6] = {0, 0, 0, 0, 0, 0};
   for (fvm_block* b : ctx->blocks) {
@@ -1241,7 +1251,9 @@ int fvm_block_launch_shape(fvm_block* b, int* tile_x, int* tile_y, int* chunk) {
 
 int fvm_timing_enable(fvm_ctx* ctx, int enable) {
   if (int rc = check_ctx(ctx)) return rc;
-  ctx->timing = enable != 0;
+  if (enable < 0) return fail(FVM_EINVAL, "timing interval %d must be >= 0", enable);
+  ctx->timing_every = enable;
+  ctx->timing_step = 0;
   ctx->stage_ms = 0.0;
   ctx->stage_launches = 0;
   ctx->cell_updates = 0.0;

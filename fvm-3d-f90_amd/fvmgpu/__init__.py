@@ -253,8 +253,9 @@ class Context:
     def sync(self):
         _check(_lib.fvm_sync(self._h))
 
-    def timing(self, enable=True):
-        _check(_lib.fvm_timing_enable(self._h, int(enable)))
+    def timing(self, every=1):
+        """Time the stage launches of every `every`-th step (0 / False: off)."""
+        _check(_lib.fvm_timing_enable(self._h, int(every)))
 
     def timing_read(self):
         ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()

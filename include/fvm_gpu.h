@@ -164,8 +164,8 @@ int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland
  * counterpart): one graph per context holds the CFL dt update and the stage
  * launches of a step; it is captured on first use and again after any
  * change of scheme, numerics, launch mode, blocks or launch shapes.  Results
- * are identical to eager launches.  Steps run with stage timing on
- * (fvm_timing) are launched eagerly.  Off by default. */
+ * are identical to eager launches.  Steps whose stages are timed
+ * (fvm_timing_enable) are launched eagerly.  Off by default. */
 int fvm_set_graph(fvm_ctx *ctx, int enable);
 
 /* Solution time and end time (the time loop's time/total_time,
@@ -193,9 +193,12 @@ int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk)
 
 /* Stage-kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only wall-clocks the whole solve,
- * src/solve/class_output.f90:266-334).  Enabling resets the counters;
- * fvm_timing_read returns the summed stage-kernel milliseconds, the number of
- * stage launches and the cell updates they performed. */
+ * src/solve/class_output.f90:266-334).  enable = n >= 1 brackets the stage
+ * launches of every n-th step (events cost a few microseconds per launch,
+ * so a sample keeps them out of short stages' wall time); 0 turns it off.
+ * Enabling resets the counters; fvm_timing_read returns the summed
+ * stage-kernel milliseconds of the timed launches, their number and the cell
+ * updates they performed (launches of steps skipped past t_end excluded). */
 int fvm_timing_enable(fvm_ctx *ctx, int enable);
 int fvm_timing_read(fvm_ctx *ctx, double *stage_ms, int64_t *launches, double *cell_updates);
 
