@@ -284,24 +284,35 @@ int block_index(fvm_ctx* ctx, fvm_block* b) {
 
 // Launch shape of a block.  3-D: tile (TX x TY columns, kTile3d) marching
 // over `chunk` planes -- 32 x 8 unless 64 wastes fewer columns, the chunk is
-// the longest (<= 32 planes) that still gives >= 512 tiles.  2-D: rows of 62
-// columns marching over `chunk` rows.  FVM_TILE / FVM_CHUNK override.
+// the longest (<= 32 planes) that still gives >= 512 tiles.  2-D (kTile2d):
+// shape 0, rows of 62 columns marching over `chunk` rows, or a TX x TY tile of
+// the plane.  FVM_TILE / FVM_TILE2 / FVM_CHUNK override.
 void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   const BlockGeom& g = b->g;
-  const char* et = getenv("FVM_TILE");
+  const char* et = getenv(b->dim == 2 ? "FVM_TILE2" : "FVM_TILE");
   const char* ec = getenv("FVM_CHUNK");
-  int tile_env = et ? std::min(std::max(atoi(et), 0), kNumTiles3d - 1) : -1;
+  const int ntile_shapes = b->dim == 2 ? kNumTiles2d : kNumTiles3d;
+  int tile_env = et ? std::min(std::max(atoi(et), 0), ntile_shapes - 1) : -1;
   int chunk_env = ec ? std::max(1, atoi(ec)) : -1;
   if (force_tile >= 0) tile_env = force_tile;
   if (force_chunk > 0) chunk_env = force_chunk;
   if (b->dim == 2) {
-    b->tiles_x = (g.nx + 61) / 62;
-    b->tiles_y = 1;
-    int chunk = 16;
-    while (chunk > 2 && (int64_t)b->tiles_x * ((g.ny + chunk - 1) / chunk) < 2048) chunk /= 2;
-    if (chunk_env > 0) chunk = chunk_env;
-    b->chunk = chunk;
-    b->tiles_z = (g.ny + chunk - 1) / chunk;
+    const int tile = tile_env >= 0 ? tile_env : 0;
+    b->tile = tile;
+    if (tile == 0) {
+      b->tiles_x = (g.nx + 61) / 62;
+      b->tiles_y = 1;
+      int chunk = 16;
+      while (chunk > 2 && (int64_t)b->tiles_x * ((g.ny + chunk - 1) / chunk) < 2048) chunk /= 2;
+      if (chunk_env > 0) chunk = chunk_env;
+      b->chunk = chunk;
+      b->tiles_z = (g.ny + chunk - 1) / chunk;
+    } else {
+      b->tiles_x = (g.nx + kTile2d[tile][0] - 1) / kTile2d[tile][0];
+      b->tiles_y = (g.ny + kTile2d[tile][1] - 1) / kTile2d[tile][1];
+      b->chunk = 1;
+      b->tiles_z = 1;
+    }
   } else {
     const int w64 = (g.nx + 63) / 64 * 64 - g.nx, w32 = (g.nx + 31) / 32 * 32 - g.nx;
     int tile = w32 <= w64 ? 1 : 0;
@@ -563,9 +574,9 @@ template <class Fam>
 hipError_t launch_family(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
   if (dim == 2) {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_MUSCL: return Fam::m2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_WENO: return Fam::w2(ctx->flux, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_SIMPLE: return Fam::s2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
     }
   } else {
     switch (ctx->recon) {
@@ -605,8 +616,7 @@ int launch_one(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
 // Can all blocks go in one launch (same kernel instantiation)?
 bool batchable(fvm_ctx* ctx) {
   for (fvm_block* b : ctx->blocks)
-    if (b->dim != ctx->blocks[0]->dim || (b->dim == 3 && b->tile != ctx->blocks[0]->tile))
-      return false;
+    if (b->dim != ctx->blocks[0]->dim || b->tile != ctx->blocks[0]->tile) return false;
   return true;
 }
 
@@ -1198,11 +1208,12 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     if (together) sel = ctx->blocks; else sel.push_back(ctx->blocks[gi]);
     float best = 1e30f;
     int best_tile = sel[0]->tile, best_chunk = sel[0]->chunk;
-    const int ntiles = dim == 3 ? kNumTiles3d : 1;
+    const int ntiles = dim == 3 ? kNumTiles3d : kNumTiles2d;
     for (int tile = 0; tile < ntiles; ++tile)
       for (int ci = 0; ci < 4; ++ci) {
+        if (dim == 2 && tile > 0 && ci > 0) break;   // a 2-D tile does not march
         const int chunk = dim == 3 ? chunks3[ci] : chunks2[ci];
-        for (fvm_block* b : sel) plan_launch(b, dim == 3 ? tile : -1, chunk);
+        for (fvm_block* b : sel) plan_launch(b, tile, chunk);
         if (int rc = sync_desc(ctx)) return rc;
         // every stage kind of the scheme, the last one with the CFL maximum
         // (into a scratch word); outputs alternate between buffers 1 and 2
@@ -1235,7 +1246,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         if (ms < best) { best = ms; best_tile = sel[0]->tile; best_chunk = sel[0]->chunk; }
       }
-    for (fvm_block* b : sel) plan_launch(b, dim == 3 ? best_tile : -1, best_chunk);
+    for (fvm_block* b : sel) plan_launch(b, best_tile, best_chunk);
   }
   if (int rc = sync_desc(ctx)) return rc;
   return FVM_OK;
@@ -1243,8 +1254,8 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
 
 int fvm_block_launch_shape(fvm_block* b, int* tile_x, int* tile_y, int* chunk) {
   if (!b) return fail(FVM_EINVAL, "null block");
-  if (tile_x) *tile_x = b->dim == 3 ? kTile3d[b->tile][0] : 62;
-  if (tile_y) *tile_y = b->dim == 3 ? kTile3d[b->tile][1] : 1;
+  if (tile_x) *tile_x = b->dim == 3 ? kTile3d[b->tile][0] : kTile2d[b->tile][0];
+  if (tile_y) *tile_y = b->dim == 3 ? kTile3d[b->tile][1] : kTile2d[b->tile][1];
   if (chunk) *chunk = b->chunk;
   return FVM_OK;
 }

@@ -21,6 +21,22 @@ struct Launcher {
   }
 
   template <int FLUX, int TX, int TY, bool CONN>
+  static hipError_t go2d_tiled(const StageArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((stage2d_tile_kernel<RC, FLUX, TX, TY, CONN>), dim3(grid), dim3(TX * TY), 0, s, a);
+    return hipGetLastError();
+  }
+
+  template <int FLUX, bool CONN>
+  static hipError_t go2d_tile(int tile, const StageArgs& a, int grid, hipStream_t s) {
+    switch (tile) {
+      case 0: return go2d<FLUX, CONN>(a, grid, s);
+      case 1: return go2d_tiled<FLUX, kTile2d[1][0], kTile2d[1][1], CONN>(a, grid, s);
+      case 2: return go2d_tiled<FLUX, kTile2d[2][0], kTile2d[2][1], CONN>(a, grid, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+
+  template <int FLUX, int TX, int TY, bool CONN>
   static hipError_t go3d(const StageArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL((stage3d_kernel<RC, FLUX, TX, TY, CONN>), dim3(grid), dim3(TX * TY), 0, s, a);
     return hipGetLastError();
@@ -38,12 +54,12 @@ struct Launcher {
   }
 
   template <bool CONN>
-  static hipError_t stage2d_c(int flux, const StageArgs& a, int grid, hipStream_t s) {
+  static hipError_t stage2d_c(int flux, int tile, const StageArgs& a, int grid, hipStream_t s) {
     switch (flux) {
-      case FLUX_ROE: return go2d<FLUX_ROE, CONN>(a, grid, s);
-      case FLUX_VANLEER: return go2d<FLUX_VANLEER, CONN>(a, grid, s);
-      case FLUX_AUSM: return go2d<FLUX_AUSM, CONN>(a, grid, s);
-      case FLUX_AUSMPLUS: return go2d<FLUX_AUSMPLUS, CONN>(a, grid, s);
+      case FLUX_ROE: return go2d_tile<FLUX_ROE, CONN>(tile, a, grid, s);
+      case FLUX_VANLEER: return go2d_tile<FLUX_VANLEER, CONN>(tile, a, grid, s);
+      case FLUX_AUSM: return go2d_tile<FLUX_AUSM, CONN>(tile, a, grid, s);
+      case FLUX_AUSMPLUS: return go2d_tile<FLUX_AUSMPLUS, CONN>(tile, a, grid, s);
       default: return hipErrorInvalidValue;
     }
   }
@@ -59,8 +75,8 @@ struct Launcher {
     }
   }
 
-  static hipError_t stage2d(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s) {
-    return conn ? stage2d_c<true>(flux, a, grid, s) : stage2d_c<false>(flux, a, grid, s);
+  static hipError_t stage2d(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s) {
+    return conn ? stage2d_c<true>(flux, tile, a, grid, s) : stage2d_c<false>(flux, tile, a, grid, s);
   }
   static hipError_t stage3d(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s) {
     return conn ? stage3d_c<true>(flux, tile, a, grid, s) : stage3d_c<false>(flux, tile, a, grid, s);

@@ -9,8 +9,8 @@ namespace fast {
 namespace exact {
 #endif
 
-hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s) {
-  return Launcher<ReconSimple>::stage2d(flux, conn, a, grid, s);
+hipError_t launch_stage2d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s) {
+  return Launcher<ReconSimple>::stage2d(flux, tile, conn, a, grid, s);
 }
 
 hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s) {

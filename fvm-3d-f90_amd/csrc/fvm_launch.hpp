@@ -12,13 +12,17 @@ namespace fvm {
 // 3-D tile shapes the library is built with: id -> (TX, TY)
 constexpr int kNumTiles3d = 4;
 constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}, {16, 16}};
+// 2-D stage shapes: id 0 the row kernel (62 columns marching over rows),
+// then the tiled kernel's (TX, TY)
+constexpr int kNumTiles2d = 3;
+constexpr int kTile2d[kNumTiles2d][2] = {{62, 1}, {32, 4}, {16, 8}};
 // viscous-residual tile (TX, TY); 3-D launches march over k-chunks
 constexpr int kViscTile[2] = {32, 8};
 
 namespace exact {
-hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
-hipError_t launch_stage2d_muscl(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
-hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
@@ -26,9 +30,9 @@ hipError_t launch_viscous(int dim, const ViscArgs& a, int grid, hipStream_t s);
 
 }  // namespace exact
 namespace fast {
-hipError_t launch_stage2d_simple(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
-hipError_t launch_stage2d_muscl(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
-hipError_t launch_stage2d_weno(int flux, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
+hipError_t launch_stage2d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_simple(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_muscl(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);
 hipError_t launch_stage3d_weno(int flux, int tile, bool conn, const StageArgs& a, int grid, hipStream_t s);

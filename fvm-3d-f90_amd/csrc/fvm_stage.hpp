@@ -621,6 +621,193 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
 }
 
 // ============================================================================
+// 2-D tiled stage: a workgroup owns a TX x TY tile of vertices of the plane.
+//   A  the tile and its halo (H vertices) into LDS as reconstruction variables
+//   B  vertex reconstructions along i and j (owned vertices, then the halo
+//      column pair and row pair the tile's faces need)
+//   C  face fluxes along i and j into LDS (aliasing A's plane)
+//   D  owner update: residual, RK combine, store
+// Same per-vertex and per-face arithmetic as the row kernel above (same
+// results bit for bit); fewer halo reconstructions per vertex and
+// multi-wave workgroups, for planes too small to fill the GPU with rows.
+// ============================================================================
+template <class RC, int FLUX, int TX, int TY, bool CONN>
+__global__ __launch_bounds__(TX * TY) void stage2d_tile_kernel(StageArgs a) {
+  constexpr int NC = 4, NT = TX * TY;
+  constexpr int RV = RC::RV, H = RC::H;
+  constexpr int VW = TX + 2 * H, VR = TY + 2 * H, VSN = VW * VR;
+  constexpr int NLR = RC::SAME ? 1 : 2;
+  constexpr int XRW = TX + 2, LRX = TY * XRW;      // i-recon columns i0-1 .. i0+TX
+  constexpr int LRY = (TY + 2) * TX;               // j-recon rows j0-1 .. j0+TY
+  constexpr int FXW = TX + 1, FXN = TY * FXW, FYN = (TY + 1) * TX;
+  constexpr int AREA = VSN > FXN + FYN ? VSN : FXN + FYN;
+  constexpr int RT = 2 * NT + 2 * TY + 2 * TX;     // reconstruction tasks
+  constexpr int FT = 2 * NT + TY + TX;             // face tasks
+  __shared__ double Vs[NC * AREA];
+  __shared__ double LRx[NLR][NC * LRX];
+  __shared__ double LRy[NLR][NC * LRY];
+  __shared__ double red[NT / 64];
+  double* Fx = Vs;                 // [NC][FXN] after phase B
+  double* Fy = Vs + NC * FXN;      // [NC][FYN]
+
+  const Params& P = a.P;
+  const int t = threadIdx.x;
+  const int tx = t % TX, ty = t / TX;
+  const bool want_smax = a.smax != nullptr || a.block_smax;
+
+  double dt_step = 0.0;
+  bool want_norm = false;
+  if (!stage_clock(a, dt_step, want_norm)) return;
+  const DescTable tab = as_const_table(a.blocks);
+  for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
+    const int tile = a.tile0 + xcd_remap(it, a.ntiles);
+    const int bi = block_of_tile(tab, a.nblocks, tile);
+    const auto& B = tab[bi];
+    if (B.halt && *B.halt) continue;   // uniform over the workgroup
+    const BlockGeom g = load_geom(B.g);
+    const double dt = a.cfl_dt ? dt_step : *B.dt_dev;
+    const double* __restrict__ rvp = B.rv;
+    const RKCoef kc = rk_coef(a.stage, dt);
+    const double* __restrict__ qin = B.q[a.in_buf];
+    const double* __restrict__ q0p = B.q[a.q0_buf];
+    double* __restrict__ qout = B.q[a.out_buf];
+    double* __restrict__ acc = a.use_acc ? B.q[3] : nullptr;
+    const int lt = tile - B.tile_base;
+    const int i0 = (lt % B.tiles_x) * TX, j0 = (lt / B.tiles_x) * TY;
+    const int c = i0 + tx, r = j0 + ty;
+    const bool owner = c < g.nx && r < g.ny;
+
+    // ---- A: the plane (tile + halo) into LDS
+    for (int e = t; e < VSN; e += NT) {
+      const int x = i0 - H + e % VW, y = j0 - H + e / VW;
+      double raw[NC], v[NC];
+      load_raw<RC, NC>(qin, g.cs, gidx32(g, bmap(x, g.nx, g.bc[0], g.bc[1], H),
+                                         bmap(y, g.ny, g.bc[2], g.bc[3], H), 0), raw);
+      to_recon_vars<RC, NC>(raw, v, P.gamma);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Vs[m * VSN + e] = v[m];
+    }
+    // operands of the RK combine, loaded now so their latency hides behind B and C
+    double pq0[NC], px[NC];
+    rk_prefetch<NC, 3>(a.stage, owner, gidx32(g, c, r, 0), g.cs, q0p, qin, acc, pq0, px);
+    __syncthreads();
+
+    // ---- B: vertex reconstructions (task e: own i, own j, then halo)
+    for (int e = t; e < RT; e += NT) {
+      int src, stride, dst;
+      bool xdir;
+      if (e < NT) {                       // own vertex, i-direction
+        const int x = e % TX, y = e / TX;
+        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+      } else if (e < 2 * NT) {            // own vertex, j-direction
+        const int e1 = e - NT, x = e1 % TX, y = e1 / TX;
+        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
+      } else if (e < 2 * NT + 2 * TY) {   // i-direction at columns i0-1 and i0+TX
+        const int e2 = e - 2 * NT, y = e2 >> 1, side = e2 & 1;
+        const int x = side ? TX : -1;
+        src = (y + H) * VW + x + H; stride = 1; xdir = true; dst = y * XRW + x + 1;
+      } else {                            // j-direction at rows j0-1 and j0+TY
+        const int e3 = e - 2 * NT - 2 * TY, x = e3 % TX, side = e3 / TX;
+        const int y = side ? TY : -1;
+        src = (y + H) * VW + x + H; stride = VW; xdir = false; dst = (y + 1) * TX + x;
+      }
+      double W[2 * RV + 1][NC], L[NC], R[NC];
+#pragma unroll
+      for (int s = 0; s < 2 * RV + 1; ++s)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) W[s][m] = Vs[m * VSN + src + (s - RV) * stride];
+      RC::template vertex<NC>(W, P, L, R);
+      if (xdir) {
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          LRx[0][m * LRX + dst] = L[m];
+          if constexpr (NLR == 2) LRx[NLR - 1][m * LRX + dst] = R[m];
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          LRy[0][m * LRY + dst] = L[m];
+          if constexpr (NLR == 2) LRy[NLR - 1][m * LRY + dst] = R[m];
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- C: face fluxes (into the plane buffer, dead after B)
+    for (int e = t; e < FT; e += NT) {
+      double L[NC], R[NC], F[NC];
+      if (e < NT || (e >= 2 * NT && e < 2 * NT + TY)) {   // i-face s of row y
+        int y, s;
+        if (e < NT) { s = e % TX; y = e / TX; } else { s = TX; y = e - 2 * NT; }
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          L[m] = LRx[0][m * LRX + y * XRW + s];
+          R[m] = LRx[NLR - 1][m * LRX + y * XRW + s + 1];
+        }
+        face_flux<FLUX, NC, 0>(L, R, P, F);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fx[m * FXN + y * FXW + s] = F[m];
+      } else {                                            // j-face s of column x
+        int x, s;
+        if (e < 2 * NT) { const int e1 = e - NT; x = e1 % TX; s = e1 / TX; }
+        else { x = e - 2 * NT - TY; s = TY; }
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          L[m] = LRy[0][m * LRY + s * TX + x];
+          R[m] = LRy[NLR - 1][m * LRY + (s + 1) * TX + x];
+        }
+        face_flux<FLUX, NC, 1>(L, R, P, F);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fy[m * FYN + s * TX + x] = F[m];
+      }
+    }
+    __syncthreads();
+
+    // ---- D: update
+    double nrm[NC], smax = 0.0;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
+    if (owner) {
+      const uint32_t cidx = gidx32(g, c, r, 0);
+      double rr[NC], out[NC];
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
+        const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
+        if constexpr (kFast) {
+          rr[m] = -(fxd * g.rdx) - fyd * g.rdy;
+        } else {
+          rr[m] = 0.0 - fxd / g.dx;
+          rr[m] = rr[m] - fyd / g.dy;
+        }
+      }
+      if (rvp) {
+#pragma unroll
+        for (int m = 1; m < NC; ++m) rr[m] = rr[m] + ld_comp(rvp, g.cs, m - 1, cidx);
+      }
+      rk_apply_all<NC>(a.stage, kc, pq0, px, rr, out, acc, g.cs, cidx);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) nrm[m] = rr[m] * rr[m];
+      store_with_ghosts<NC, 2, CONN>(g, tab, B, a.out_buf, qout, c, r, 0, out);
+      if (want_smax) smax = wave_speed<NC>(out, g, P.gamma);
+    }
+    if (want_norm) {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        const double s = wg_sum<NT>(nrm[m], red);
+        if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
+      }
+    }
+    if (want_smax) {
+      const double s = wg_max<NT>(smax, red);
+      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
+      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+    }
+    __syncthreads();   // the next tile's plane overwrites the fluxes
+  }
+}
+
+// ============================================================================
 // 3-D stage: tile of TX (32 or 64) x TY columns, NT = TX * TY threads, every
 // thread owns the column (i0 + tx, j0 + ty) and marches along k.  A
 // workgroup loops over tiles of the launch's tile list (all blocks of the

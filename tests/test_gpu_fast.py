@@ -31,8 +31,10 @@ def test_fast_one_step_3d(recon, lim, flux):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("tile2", ["0", "1"])
 @pytest.mark.parametrize("recon,lim,flux", COMBOS[:4])
-def test_fast_one_step_riemann2d(recon, lim, flux):
+def test_fast_one_step_riemann2d(recon, lim, flux, tile2, monkeypatch):
+    monkeypatch.setenv("FVM_TILE2", tile2)
     p, dx = cases.riemann2d(101, seed=0)
     print(f"case fast 1-step riemann2d recon={recon} lim={lim} flux={flux}", flush=True)
     with fvmgpu.Context() as ctx:

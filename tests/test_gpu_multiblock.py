@@ -130,10 +130,12 @@ def test_independent_cases_batched_equal_alone():
             assert np.array_equal(b.download(), q)
 
 
-def test_split_2d_equals_single_block():
+@pytest.mark.parametrize("tile2", ["0", "1", "2"])
+def test_split_2d_equals_single_block(tile2, monkeypatch):
+    monkeypatch.setenv("FVM_TILE2", tile2)
     p, dx = cases.riemann2d(61, seed=0)
     ranges = (split_bounds(61, 3), split_bounds(61, 2), [(0, 0)])
-    print("case 2-D split 3x2 riemann61", flush=True)
+    print(f"case 2-D split 3x2 riemann61 tile {tile2}", flush=True)
     q, norms = run_blocks(p, (dx, dx, 1.0), ranges, 5, 2e-4, fvmgpu.LAUNCH_BATCHED)
     qo, r = run_oracle(p, (dx, dx, 1.0), 5, dt=2e-4)
     assert np.array_equal(q, qo)
