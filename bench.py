@@ -290,7 +290,37 @@ def bench_riemann(args):
                                      flux=O.ROE, rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0,
                                      cfl=cfl, nsteps=cs, p_from_q=1),
             ncell * 3 * cs, f"{cs} SSP-RK3 steps of the {n}x{n} Riemann workload")
+        ref = cpu_reference(p, q, dx, cfl, cs, ncell * 3 * cs)
+        if ref is not None:
+            # the reference's own scheme modules on one core (the reference
+            # has no threading of its own): the baseline; the port's figures
+            # stay beside it
+            port = cpu
+            cpu = dict(ref)
+            cpu["port_value"] = port["value"]
+            cpu["port_cores"] = port["cores"]
+            cpu["port_value_1core"] = port["value_1core"]
+            cpu["port_sample"] = port["sample"]
     return line(args, value, steps, sec, config, roof, cpu)
+
+
+def cpu_reference(p, q, dx, cfl, steps, cells):
+    """The reference's Fortran scheme modules (oracle/_ref, compiled from its
+    sources) driven over the same Riemann sample on one core.  The reference
+    steps with a fixed dt: the CFL dt of the initial state.  None where the
+    harness is not built."""
+    O = oracle()
+    if O.ref() is None:
+        return None
+    dt = O.cfl_dt(q, 2, (dx, dx, 1.0), 1.4, cfl)
+    t0 = time.perf_counter()
+    O.ref_solve(p, 2, (dx, dx, 1.0), recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE,
+                kappa=0.33333, entfix=0.2, dt=dt, nsteps=steps, p_from_q=1)
+    sec = time.perf_counter() - t0
+    return {"value": cells / sec, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} SSP-RK3 steps (fixed dt {dt:.4g}) of the Riemann workload, "
+                      f"{cells:.3g} cell-updates in {sec:.2f} s; oracle/_ref/libfvmref.so "
+                      "(the reference's scheme modules)"}
 
 
 # ---------------------------------------------------------------------------
