@@ -1310,7 +1310,7 @@ struct Job {
   int steps = 0;
   double t_final = 0;
   double cell_updates = 0;
-  int status = 0;                  // 0 pending, 1 active, 2 done, -1 failed
+  int status = 0;                  // 0 pending, 1 active, 2 done, -1 diverged, -2 not staged
   int slot = -1;
   int64_t first_batch = 0;         // first batch that advanced this solve (0: none yet)
   fvm_ctx* group = nullptr;
@@ -1506,7 +1506,7 @@ int server_admit(fvm_server* s) {
     if (st == 0) break;   // still being copied: later requests wait their turn
     if (st < 0) {         // no pinned memory for it
       s->pending.pop_front();
-      j->status = -1;
+      j->status = -2;
       j->t_done = now_s(s);
       continue;
     }
@@ -1862,7 +1862,7 @@ int fvm_server_poll(fvm_server* s, int iterations, int steps_per_sync, int* n_pe
     if (s->active.empty() && s->inflight.empty()) break;
   }
   int done = 0;
-  for (auto& kv : s->jobs) done += kv.second->status == 2 || kv.second->status == -1;
+  for (auto& kv : s->jobs) done += kv.second->status == 2 || kv.second->status < 0;
   if (n_pending) *n_pending = (int)s->pending.size();
   if (n_active) *n_active = (int)s->active.size();
   if (n_done) *n_done = done;
@@ -1880,6 +1880,8 @@ int fvm_server_result(fvm_server* s, int64_t id, double* q_out, double info[8]) 
     info[6] = j->status; info[7] = (double)j->bytes;
   }
   if (j->status == 0 || j->status == 1) return fail(FVM_ESTATE, "request %lld not finished", (long long)id);
+  if (j->status == -2)
+    return fail(FVM_ENOMEM, "request %lld: no pinned host memory to stage its state", (long long)id);
   if (j->status < 0) return fail(FVM_ENAN, "request %lld diverged (NaN or non-positive dt)", (long long)id);
   if (j->res) {   // first call: wait for the download, keep the state, release the pinned copy
     HIP_TRY(hipEventSynchronize(j->res_ev));

@@ -240,7 +240,8 @@ int fvm_server_poll(fvm_server *srv, int iterations, int steps_per_sync, int *n_
                     int *n_active, int *n_done);
 /* Final state (same layout as q0) and info[8] = {submit, admit, done (s since
  * server creation), steps, cell-updates, final time, status (2 done, -1
- * diverged), device bytes}. */
+ * diverged, -2 not staged: no pinned host memory), device bytes}.  The first
+ * call for a request waits for its asynchronous download. */
 int fvm_server_result(fvm_server *srv, int64_t id, double *q_out, double info[8]);
 
 /* Wait for all work of the context. */
