@@ -57,3 +57,26 @@ def test_scheme_codes_match_reference_and_oracle():
     for macro, val in (("FVM_MUSCL", 201), ("FVM_WENO", 202), ("FVM_ROE", 300),
                        ("FVM_AUSM", 303), ("FVM_LIM_VANALBADA", 253), ("FVM_RK_SSP3", 101)):
         assert re.search(rf"#define {macro}\s+{val}\b", hdr)
+
+
+def test_request_struct_layout_matches_header(tmp_path):
+    """fvmgpu.Request (ctypes) has the size and field offsets of the C
+    compiler's fvm_request."""
+    fields = [f[0] for f in fvmgpu.Request._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "fvm_gpu.h"\n'
+        "int main(void) {\n"
+        '  printf("size %zu\\n", sizeof(fvm_request));\n'
+        + "".join(f'  printf("{f} %zu\\n", offsetof(fvm_request, {f}));\n' for f in fields)
+        + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True,
+                                                       capture_output=True, text=True)
+               .stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(fvmgpu.Request)
+    for f in fields:
+        assert int(out[f]) == getattr(fvmgpu.Request, f).offset, f
