@@ -86,6 +86,19 @@ struct fvm_ctx {
   int recon = 0, flux = 0, rk = 0;
   Params P{};
   std::vector<fvm_block*> blocks;
+  // 1-to-1 connection patches (fvm_conn_add): host list (block pointers),
+  // device table rebuilt with the descriptor table
+  struct Patch {
+    fvm_block* recv;
+    fvm_block* donor;
+    int face;
+    int64_t rmin[3], rmax[3], dmin[3];
+    int map_dir[3], map_sign[3];
+  };
+  std::vector<Patch> patches;
+  PatchDesc* d_patch = nullptr;
+  size_t patch_cap = 0;
+  int64_t patch_points = 0;        // ghost points over all patches
   BlockDesc* d_desc = nullptr;           // descriptor table on the device
   size_t desc_cap = 0;
   bool desc_dirty = true;                // table must be rebuilt before a launch
@@ -177,6 +190,55 @@ __global__ void ghost_fill_kernel(const BlockDesc* __restrict__ tab, int bi, int
     }
     const int64_t di = gidx(g, x[0], x[1], x[2]);
     for (int c = 0; c < nc; ++c) q[di + c * g.cs] = src[si + c * cs];
+  }
+}
+
+// Ghost layers of the 1-to-1 connection patches, state array `buf`: point e
+// of the launch's list belongs to the patch whose range contains it (patches
+// listed by their first point, `start`); the receiver's ghost index maps
+// linearly onto the donor's interior (src/base/defs_face_geom.f90:361-403:
+// donor ranges and direction transform).
+__global__ void patch_copy_kernel(const BlockDesc* __restrict__ tab, const PatchDesc* __restrict__ pd,
+                                  int npatch, int64_t total, int buf) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int p = 0;
+    while (p + 1 < npatch && pd[p + 1].start <= e) ++p;
+    const PatchDesc& P = pd[p];
+    const int64_t loc = e - P.start;
+    const int ng = tab[P.recv].g.ng;
+    const int a = (int)(loc % P.na);
+    const int b = (int)((loc / P.na) % P.nb);
+    const int l = 1 + (int)(loc / ((int64_t)P.na * P.nb));
+    if (l > ng) continue;
+    int64_t r[3] = {P.rmin[0], P.rmin[1], P.rmin[2]};
+    r[P.ta] += a;
+    r[P.tb] += b;
+    r[P.dn] += P.side ? l : -l;
+    int64_t d[3];
+#pragma unroll
+    for (int ex = 0; ex < 3; ++ex) d[ex] = 0;
+#pragma unroll
+    for (int ex = 0; ex < 3; ++ex)
+      d[P.map_dir[ex]] = P.dmin[P.map_dir[ex]] + P.map_sign[ex] * (r[ex] - P.rmin[ex]);
+    const BlockGeom& gr = tab[P.recv].g;
+    const BlockGeom& gd = tab[P.donor].g;
+    double* dst = tab[P.recv].q[buf];
+    const double* src = tab[P.donor].q[buf];
+    const int64_t ri = gidx(gr, (int)r[0], (int)r[1], (int)r[2]);
+    const int64_t di = gidx(gd, (int)d[0], (int)d[1], (int)d[2]);
+    // momentum is stored along each block's own index directions: the
+    // receiver's component along direction e is the donor's along
+    // map_dir[e], times map_sign[e]
+    for (int c = 0; c < P.nc; ++c) {
+      int sc = c;
+      double sg = 1.0;
+      if (c >= 1 && c <= P.nc - 2) {
+        sc = 1 + P.map_dir[c - 1];
+        sg = P.map_sign[c - 1];
+      }
+      dst[ri + c * gr.cs] = sg * src[di + sc * gd.cs];
+    }
   }
 }
 
@@ -523,8 +585,85 @@ void fill_desc(fvm_ctx* ctx, BlockDesc* h) {
   ++ctx->gen;
 }
 
+// Device table of the connection patches (block indices as in the
+// descriptor table); checks that every BC_PATCH face is covered exactly once.
+int sync_patches(fvm_ctx* ctx) {
+  ctx->patch_points = 0;
+  if (ctx->patches.empty()) return FVM_OK;
+  std::vector<PatchDesc> h;
+  int64_t start = 0;
+  for (const fvm_ctx::Patch& q : ctx->patches) {
+    PatchDesc d{};
+    d.recv = block_index(ctx, q.recv);
+    d.donor = block_index(ctx, q.donor);
+    d.dn = q.face / 2;
+    d.side = q.face % 2;
+    d.ta = d.dn == 0 ? 1 : 0;
+    d.tb = d.dn == 2 ? 1 : 2;
+    d.na = (int)(q.rmax[d.ta] - q.rmin[d.ta] + 1);
+    d.nb = (int)(q.rmax[d.tb] - q.rmin[d.tb] + 1);
+    d.nc = q.recv->nc;
+    for (int e = 0; e < 3; ++e) {
+      d.map_dir[e] = q.map_dir[e];
+      d.map_sign[e] = q.map_sign[e];
+      d.rmin[e] = q.rmin[e];
+      d.dmin[e] = q.dmin[e];
+    }
+    d.start = start;
+    start += (int64_t)d.na * d.nb * q.recv->g.ng;
+    h.push_back(d);
+  }
+  // coverage: the patches of a face tile it without overlap
+  for (fvm_block* b : ctx->blocks)
+    for (int f = 0; f < 2 * b->dim; ++f) {
+      if (b->g.bc[f] != BC_PATCH) continue;
+      const int dn = f / 2, ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
+      const int n[3] = {b->g.nx, b->g.ny, b->g.nz};
+      int64_t area = 0;
+      std::vector<const fvm_ctx::Patch*> on;
+      for (const fvm_ctx::Patch& q : ctx->patches)
+        if (q.recv == b && q.face == f) {
+          for (const fvm_ctx::Patch* o : on) {
+            bool apart = false;
+            for (int e : {ta, tb}) apart = apart || q.rmax[e] < o->rmin[e] || o->rmax[e] < q.rmin[e];
+            if (!apart) return fail(FVM_EINVAL, "block %d face %d: connection patches overlap",
+                                    block_index(ctx, b), f);
+          }
+          on.push_back(&q);
+          area += (q.rmax[ta] - q.rmin[ta] + 1) * (q.rmax[tb] - q.rmin[tb] + 1);
+        }
+      if (area != (int64_t)n[ta] * n[tb])
+        return fail(FVM_EINVAL, "block %d face %d: connection patches cover %lld of %lld vertices",
+                    block_index(ctx, b), f, (long long)area, (long long)n[ta] * n[tb]);
+    }
+  if (h.size() > ctx->patch_cap) {
+    if (ctx->d_patch) HIP_TRY(hipFree(ctx->d_patch));
+    ctx->d_patch = nullptr;
+    if (hipMalloc(&ctx->d_patch, h.size() * sizeof(PatchDesc)) != hipSuccess)
+      return fail(FVM_ENOMEM, "cannot allocate the patch table");
+    ctx->patch_cap = h.size();
+  }
+  HIP_TRY(hipMemcpyAsync(ctx->d_patch, h.data(), h.size() * sizeof(PatchDesc), hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->patch_points = start;
+  return FVM_OK;
+}
+
+// Fill the connection-patch ghosts of state array `buf` (after a stage, or
+// after the face fills of an upload).
+int launch_patches(fvm_ctx* ctx, int buf) {
+  if (ctx->patch_points <= 0) return FVM_OK;
+  const int64_t blocks = std::min<int64_t>(1024, (ctx->patch_points + 255) / 256);
+  hipLaunchKernelGGL(patch_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, ctx->d_desc,
+                     ctx->d_patch, (int)ctx->patches.size(), ctx->patch_points, buf);
+  HIP_TRY(hipGetLastError());
+  return FVM_OK;
+}
+
 int sync_desc(fvm_ctx* ctx) {
   if (!ctx->desc_dirty) return FVM_OK;
+  if (int rc = sync_patches(ctx)) return rc;
   std::vector<BlockDesc> h(ctx->blocks.size());
   fill_desc(ctx, h.data());
   if (h.size() > ctx->desc_cap) {
@@ -567,7 +706,7 @@ int fill_ghosts_all(fvm_ctx* ctx, int buf) {
         HIP_TRY(hipGetLastError());
       }
   }
-  return FVM_OK;
+  return launch_patches(ctx, buf);
 }
 
 template <class Fam>
@@ -711,6 +850,7 @@ int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_stage_all(ctx, a)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
+    if (int rc = launch_patches(ctx, a.out_buf)) return rc;
   }
   return FVM_OK;
 }
@@ -797,6 +937,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
   (void)hipFree(ctx->d_desc);
   (void)hipFree(ctx->d_slot);
   (void)hipFree(ctx->d_norm);
+  (void)hipFree(ctx->d_patch);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return FVM_OK;
@@ -863,6 +1004,14 @@ int fvm_block_destroy(fvm_block* b) {
       if (o->nbr[f] == idx) { o->nbr[f] = -1; o->g.bc[f] = BC_EXTRAP; }
       else if (o->nbr[f] > idx) --o->nbr[f];
     }
+  // a face with a connection patch from the block loses all its patches and
+  // falls back to extrapolation
+  for (const fvm_ctx::Patch& q : ctx->patches)
+    if (q.donor == b) q.recv->g.bc[q.face] = BC_EXTRAP;
+  std::vector<fvm_ctx::Patch> keep;
+  for (const fvm_ctx::Patch& q : ctx->patches)
+    if (q.recv != b && q.recv->g.bc[q.face] == BC_PATCH) keep.push_back(q);
+  ctx->patches.swap(keep);
   free_block_state(b, nullptr);
   ctx->blocks.erase(std::remove(ctx->blocks.begin(), ctx->blocks.end(), b), ctx->blocks.end());
   delete b;
@@ -879,7 +1028,10 @@ int fvm_block_set_bc(fvm_block* b, int face, int kind) {
   const int d = face / 2, other = 2 * d + (1 - face % 2);
   if (kind == FVM_BC_PERIODIC && (b->g.bc[face] == BC_CONN || b->g.bc[other] == BC_CONN))
     return fail(FVM_EINVAL, "face %d is connected to another block", face);
-  if (b->g.bc[face] == BC_CONN) return fail(FVM_EINVAL, "face %d is connected to another block", face);
+  if (b->g.bc[face] == BC_CONN || b->g.bc[face] == BC_PATCH)
+    return fail(FVM_EINVAL, "face %d is connected to another block", face);
+  if (kind == FVM_BC_PERIODIC && b->g.bc[other] == BC_PATCH)
+    return fail(FVM_EINVAL, "face %d is connected to another block", other);
   b->g.bc[face] = kind;
   if (kind == FVM_BC_PERIODIC) b->g.bc[other] = kind;              // periodic on both faces
   else if (b->g.bc[other] == BC_PERIODIC) b->g.bc[other] = BC_EXTRAP;
@@ -910,6 +1062,79 @@ int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
   const int ia = block_index(ctx, a), ib = block_index(ctx, b);
   a->g.bc[face] = BC_CONN; a->nbr[face] = ib; a->nbr_shared[face] = shared ? 1 : 0;
   b->g.bc[bface] = BC_CONN; b->nbr[bface] = ia; b->nbr_shared[bface] = shared ? 1 : 0;
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
+  return FVM_OK;
+}
+
+int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t range_max[3],
+                 fvm_block* donor, const int64_t donor_range_min[3], const int64_t donor_range_max[3],
+                 const int donor_type[3]) {
+  if (!recv || !donor) return fail(FVM_EINVAL, "null block");
+  if (!range_min || !range_max || !donor_range_min || !donor_range_max || !donor_type)
+    return fail(FVM_EINVAL, "null range or transform");
+  if (recv->ctx != donor->ctx) return fail(FVM_EINVAL, "blocks belong to different contexts");
+  fvm_ctx* ctx = recv->ctx;
+  if (ctx->visc_mu_ref > 0.0)
+    return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
+  const int dim = recv->dim;
+  const int nr[3] = {recv->g.nx, recv->g.ny, recv->g.nz};
+  const int nd[3] = {donor->g.nx, donor->g.ny, donor->g.nz};
+  // the transform: a signed permutation (2-D: k maps to k)
+  int used[3] = {0, 0, 0};
+  fvm_ctx::Patch q{};
+  for (int e = 0; e < 3; ++e) {
+    const int t = donor_type[e];
+    if (t == 0 || t < -3 || t > 3 || used[std::abs(t) - 1]++)
+      return fail(FVM_EINVAL, "donor_type must be a signed permutation of 1, 2, 3");
+    q.map_dir[e] = std::abs(t) - 1;
+    q.map_sign[e] = t > 0 ? 1 : -1;
+  }
+  if (dim == 2 && donor_type[2] != 3) return fail(FVM_EINVAL, "2-D blocks: donor_type[2] must be 3");
+  // the receiving range: a rectangle on a boundary face (1-based, inclusive);
+  // the donor range corresponds point for point through the transform
+  if (dim == 2 && (nr[2] != 1 || range_min[2] != 1 || range_max[2] != 1))
+    return fail(FVM_EINVAL, "2-D blocks: the k range must be 1..1");
+  for (int e = 0; e < 3; ++e) {
+    if (range_min[e] < 1 || range_max[e] > nr[e] || range_min[e] > range_max[e])
+      return fail(FVM_EINVAL, "range out of the receiving block in direction %d", e);
+    const int b = q.map_dir[e];
+    if (donor_range_max[e] - donor_range_min[e] != q.map_sign[e] * (range_max[e] - range_min[e]))
+      return fail(FVM_EINVAL, "donor range does not match the range in direction %d", e);
+    const int64_t lo = std::min(donor_range_min[e], donor_range_max[e]);
+    const int64_t hi = std::max(donor_range_min[e], donor_range_max[e]);
+    if (lo < 1 || hi > nd[b]) return fail(FVM_EINVAL, "donor range out of the donor block");
+    q.rmin[e] = range_min[e] - 1;
+    q.rmax[e] = range_max[e] - 1;
+    q.dmin[b] = donor_range_min[e] - 1;
+  }
+  // the face: a direction where the range is one vertex plane on a boundary
+  // of the receiver, mapped onto a boundary plane of the donor so that the
+  // receiver's ghosts land in the donor's interior (the interface vertex
+  // plane belongs to both blocks)
+  int face = -1;
+  for (int e = 0; e < dim && face < 0; ++e) {
+    if (range_min[e] != range_max[e] || nr[e] < 2) continue;
+    const int side = range_min[e] == nr[e] ? 1 : (range_min[e] == 1 ? 0 : -1);
+    if (side < 0) continue;
+    const int bn = q.map_dir[e];
+    const int64_t dface = q.dmin[bn];
+    if (dface != 0 && dface != nd[bn] - 1) continue;
+    const int inward = dface == 0 ? 1 : -1;
+    if ((side ? 1 : -1) * q.map_sign[e] != inward) continue;
+    if (nd[bn] < 2 * kGhost) return fail(FVM_EINVAL, "the donor needs >= %d vertices across the face", 2 * kGhost);
+    face = 2 * e + side;
+  }
+  if (face < 0)
+    return fail(FVM_EINVAL, "the ranges do not describe a receiver face onto a donor face with ghosts "
+                            "mapped into the donor");
+  if (recv->g.bc[face] == BC_CONN || recv->g.bc[face] == BC_PERIODIC)
+    return fail(FVM_EINVAL, "face %d of the receiving block already has a connection or periodic BC", face);
+  q.recv = recv;
+  q.donor = donor;
+  q.face = face;
+  ctx->patches.push_back(q);
+  recv->g.bc[face] = BC_PATCH;
   ctx->desc_dirty = true;
   ctx->ghosts_dirty = true;
   return FVM_OK;
@@ -1114,7 +1339,7 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
     return fail(FVM_EINVAL, "viscous parameters need t_ref, prandtl, r_gas > 0 and s >= 0");
   for (fvm_block* b : ctx->blocks)
     for (int f = 0; f < 6; ++f)
-      if (b->g.bc[f] == BC_CONN)
+      if (b->g.bc[f] >= BC_CONN)
         return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
   if (int rc = set_device(ctx)) return rc;
   for (fvm_block* b : ctx->blocks)

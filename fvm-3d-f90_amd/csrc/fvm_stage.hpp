@@ -301,14 +301,15 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? l
 // Index of a stencil point along one direction.  Real vertices map to
 // themselves; outside the block, to the vertex its ghost copy holds (the
 // periodic image, or the boundary vertex under extrapolation,
-// src/base/defs_block_data.f90:226-256), except across a connected face,
-// whose ghost layer the neighbour block writes.  Reading the source vertex
+// src/base/defs_block_data.f90:226-256), except across a connected face
+// (BC_CONN: the neighbour's stage writes the ghost layer; BC_PATCH: a copy
+// kernel fills it after every stage), whose ghosts are read.  Reading the source vertex
 // instead of a ghost copy gives the same value, so the stage writes no
 // ghost images of its own.
 __device__ __forceinline__ int bmap(int x, int n, int lo, int hi, int h) {
   x = clampi(x, -h, n + h - 1);
-  if (x < 0) return lo == BC_PERIODIC ? x + n : (lo == BC_CONN ? x : 0);
-  if (x >= n) return hi == BC_PERIODIC ? x - n : (hi == BC_CONN ? x : n - 1);
+  if (x < 0) return lo == BC_PERIODIC ? x + n : (lo >= BC_CONN ? x : 0);
+  if (x >= n) return hi == BC_PERIODIC ? x - n : (hi >= BC_CONN ? x : n - 1);
   return x;
 }
 

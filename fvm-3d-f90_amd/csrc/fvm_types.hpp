@@ -36,7 +36,24 @@ struct BlockGeom {
 // written by that block's stage kernel.  `shared` faces hold the interface
 // vertex in both blocks (a grid split through a vertex plane); abutting faces
 // join vertex n-1 of one block to vertex 0 of the next one spacing apart.
-enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1, BC_CONN = 2 };
+// BC_PATCH: the face's ghost layers are filled after every stage from 1-to-1
+// connection patches (donor ranges with a direction transform).
+enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1, BC_CONN = 2, BC_PATCH = 3 };
+
+// One 1-to-1 connection patch of a receiving block face: the receiver's
+// ghost points beyond the patch take the donor vertices the linear index map
+// d = dmin + S (r - rmin) gives (S: a signed permutation), extended through
+// the ghost layers.  Indices 0-based.
+struct PatchDesc {
+  int recv, donor;                 // block indices in the context's table
+  int dn, side;                    // normal direction of the receiving face, 0: low, 1: high
+  int ta, tb;                      // the two tangential directions
+  int na, nb;                      // patch extent along ta, tb (vertices)
+  int nc;
+  int map_dir[3], map_sign[3];     // receiver direction e -> donor direction, sign
+  int64_t rmin[3], dmin[3];        // corresponding corners
+  int64_t start;                   // first point of this patch in the launch's point list
+};
 
 // One block as the stage kernels see it.
 struct BlockDesc {

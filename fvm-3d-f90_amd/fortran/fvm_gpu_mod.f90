@@ -31,7 +31,7 @@ module fvm_gpu
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
-  public :: fvm_block_connect, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
+  public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
   public :: fvm_check, fvm_error_text, scheme_index
 
@@ -138,6 +138,18 @@ module fvm_gpu
       integer(c_int), value :: face, shared
       integer(c_int) :: rc
     end function fvm_block_connect
+
+    ! the reference's face metadata (range_min/max, donor_range_min/max,
+    ! donor_type), 1-based vertex indices
+    function fvm_conn_add(recv, range_min, range_max, donor, donor_range_min, donor_range_max, &
+                          donor_type) bind(C, name="fvm_conn_add") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: recv, donor
+      integer(c_int64_t), intent(in) :: range_min(3), range_max(3)
+      integer(c_int64_t), intent(in) :: donor_range_min(3), donor_range_max(3)
+      integer(c_int), intent(in) :: donor_type(3)
+      integer(c_int) :: rc
+    end function fvm_conn_add
 
     function fvm_viscous_set(ctx, mu_ref, t_ref, sutherland_s, prandtl, r_gas) &
         bind(C, name="fvm_viscous_set") result(rc)

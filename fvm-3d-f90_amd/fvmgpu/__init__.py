@@ -42,7 +42,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
-           "fvm_set_graph")
+           "fvm_set_graph", "fvm_conn_add")
 
 
 class FvmError(RuntimeError):
@@ -96,6 +96,8 @@ def load_library(path=LIB_PATH):
         "fvm_set_time": ([vp, cd, cd], ci),
         "fvm_viscous_set": ([vp, cd, cd, cd, cd, cd], ci),
         "fvm_set_graph": ([vp, ci], ci),
+        "fvm_conn_add": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64), vp, ctypes.POINTER(i64),
+                          ctypes.POINTER(i64), ctypes.POINTER(ci)], ci),
         "fvm_time": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
@@ -138,6 +140,17 @@ class Block:
     def connect(self, face, other, shared=True):
         """Face `face` of this block abuts the opposite face of `other`."""
         _check(_lib.fvm_block_connect(self._h, face, other._h, int(bool(shared))))
+
+    def conn_add(self, range_min, range_max, donor, donor_range_min, donor_range_max,
+                 donor_type=(1, 2, 3)):
+        """One 1-to-1 connection patch (1-based inclusive vertex ranges, CGNS
+        Conn-1to1 point range, donor range and transform): the ghosts of this
+        block's face range are filled from `donor`."""
+        def i3(v):
+            return (ctypes.c_int64 * 3)(*(int(x) for x in v))
+        _check(_lib.fvm_conn_add(self._h, i3(range_min), i3(range_max), donor._h,
+                                 i3(donor_range_min), i3(donor_range_max),
+                                 (ctypes.c_int * 3)(*(int(x) for x in donor_type))))
 
     def set_periodic(self, dims=(0, 1, 2)):
         for d in dims:

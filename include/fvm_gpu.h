@@ -123,6 +123,26 @@ int fvm_block_set_bc(fvm_block *blk, int face, int kind);
  * Both blocks must belong to the same context. */
 int fvm_block_connect(fvm_block *a, int face, fvm_block *b, int shared);
 
+/* One 1-to-1 connection patch in the reference's face metadata
+ * (ClassGeomFace range_min/range_max, donor_type, donor_range_min/max,
+ * src/base/defs_face_geom.f90:43-63; donor ranges from the transform,
+ * defs_face_geom.f90:384-403), as a CGNS 1-to-1 connection carries it
+ * (src/base/interface_cgns.f90:105-112: range, donor range, transform).
+ * Indices are 1-based block vertex indices, inclusive.  range_min..range_max
+ * is a rectangle on a boundary face of `recv` (one vertex plane in the face
+ * direction); donor_type[d] = +-(1 + e): this face's direction d runs along
+ * donor direction e, forward or backward; donor_range_min[d] is the donor
+ * index along that direction that corresponds to range_min[d] (so
+ * donor_range_max[d] - donor_range_min[d] = sign * (range_max[d] -
+ * range_min[d])).  The interface vertex plane belongs to both blocks: ghost
+ * layer l of `recv` receives the donor vertex l planes inside its face.  The
+ * patches on a face must tile it exactly; a face may mix donors.  The ghosts
+ * are refilled on the device after every stage.  2-D blocks: donor_type[2]
+ * = 3 and k ranges 1..1.  Not with viscous terms. */
+int fvm_conn_add(fvm_block *recv, const int64_t range_min[3], const int64_t range_max[3],
+                 fvm_block *donor, const int64_t donor_range_min[3],
+                 const int64_t donor_range_max[3], const int donor_type[3]);
+
 /* How a stage is launched over the blocks of a context (no reference
  * counterpart):
  *   FVM_LAUNCH_BATCHED (default): one launch per stage over the tile list of
