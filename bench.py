@@ -80,6 +80,12 @@ def parse():
                     help="exact: bit-identical to the reference arithmetic; fast: FMA + "
                          "reciprocals, <= 1e-12 rel. after 1 step, <= 1e-9 after 1000 (tested)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--output-every", type=int, default=0,
+                    help="vortex: write the output variables to a file every K steps inside "
+                         "the timed region (0: no output)")
+    ap.add_argument("--output-mode", choices=("snapshot", "download"), default="snapshot",
+                    help="snapshot: asynchronous device snapshot saved on a background thread; "
+                         "download: blocking download, q -> p and write on the host")
     ap.add_argument("--persistent", action="store_true",
                     help="blob27: persistent launch (grid of 2 workgroups per CU) instead of "
                          "one workgroup per tile")
@@ -182,10 +188,15 @@ def bench_vortex(args):
 
     ctx.timing(TIMING_EVERY)
     ctx.sync()
-    t0 = time.perf_counter()
-    dt_last = ctx.step(steps, cfl=cfl)
-    ctx.sync()
-    t1 = time.perf_counter()
+    if args.output_every:
+        t0, t1, dt_last, output = timed_with_output(fvmgpu, cases, ctx, blk, steps, cfl,
+                                                    args.output_every, args.output_mode)
+    else:
+        output = None
+        t0 = time.perf_counter()
+        dt_last = ctx.step(steps, cfl=cfl)
+        ctx.sync()
+        t1 = time.perf_counter()
     stage_ms, launches, _ = ctx.timing_read()
     ctx.timing(False)
     norms, nan = ctx.norms()
@@ -216,6 +227,8 @@ def bench_vortex(args):
               "grid": [n, n, n], "stages_per_step": 4, "dt_last": dt_last,
               "numerics": args.numerics, "tile_xy_chunk": list(tile),
               "pct_hbm_roofline": 100.0 * value * B_ALG["rk4_3d"] / (HBM_PEAK_GBS * 1e9)}
+    if output:
+        config["output"] = output
     cpu = None
     if not args.no_cpu:
         cpu = cpu_measure(
@@ -225,6 +238,45 @@ def bench_vortex(args):
                                      bc=(O.BC_PERIODIC,) * 6),
             ncell * 4 * args.cpu_steps, f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload")
     return line(args, value, steps, sec, config, roof, cpu)
+
+
+def timed_with_output(fvmgpu, cases, ctx, blk, steps, cfl, every, mode):
+    """The timed steps in calls of `every`, each followed by writing the
+    output variables (density, velocity, pressure) of the state to a file in
+    a temporary directory (two files used in turn, removed afterwards)."""
+    import tempfile
+    if steps % every:
+        raise SystemExit("bench.py: --steps must be a multiple of --output-every")
+    tmp = tempfile.mkdtemp(prefix="fvm_out_")
+    paths = [os.path.join(tmp, f"out{i}.fvmsnap") for i in range(2)]
+    pending = []
+    nout = 0
+    t0 = time.perf_counter()
+    for c in range(steps // every):
+        dt_last = ctx.step(every, cfl=cfl)
+        path = paths[c % 2]
+        if mode == "snapshot":
+            if len(pending) == 2:            # the file this one overwrites is written
+                ctx.snapshot_release(pending.pop(0))
+            sid = ctx.snapshot(fvmgpu.PRIMITIVE_VARS)
+            ctx.snapshot_save(sid, path)
+            pending.append(sid)
+        else:
+            pv = np.moveaxis(cases.to_primitive(blk.download()), -1, 0)
+            with open(path, "wb") as f:
+                f.write(np.ascontiguousarray(pv).tobytes())
+        nout += 1
+    for sid in pending:
+        ctx.snapshot_release(sid)
+    ctx.sync()
+    t1 = time.perf_counter()
+    nbytes = os.path.getsize(paths[0])
+    for q in paths:
+        if os.path.exists(q):
+            os.remove(q)
+    os.rmdir(tmp)
+    return t0, t1, dt_last, {"every": every, "mode": mode, "outputs": nout,
+                             "bytes_per_output": nbytes}
 
 
 # ---------------------------------------------------------------------------

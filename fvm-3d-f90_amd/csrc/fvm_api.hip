@@ -51,6 +51,8 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 constexpr int kGhost = 3;   // ghost layers per side (src/base/defs_ghost_cell.f90:42)
+constexpr int kSnapSlots = 4;   // snapshots in flight per context
+constexpr int kSnapMaxVars = 10;
 
 }  // namespace
 
@@ -96,6 +98,28 @@ struct fvm_ctx {
     int map_dir[3], map_sign[3];
   };
   std::vector<Patch> patches;
+  // asynchronous snapshots (fvm_snapshot_*): slots of device and pinned host
+  // memory; `id` 0 marks a free slot
+  struct Snap {
+    int64_t id = 0;
+    int nvars = 0;
+    int which[kSnapMaxVars] = {};     // snapshot_kernel selector per variable
+    int codes[kSnapMaxVars] = {};     // the caller's variable codes
+    double* d_buf = nullptr;
+    double* h_buf = nullptr;
+    size_t cap = 0;                   // doubles in d_buf and h_buf
+    hipEvent_t conv = nullptr;        // snapshot_kernel has written d_buf
+    hipEvent_t done = nullptr;        // the copy into h_buf has landed
+    double time = 0.0;
+    int64_t step = 0;
+    struct Part { fvm_block* blk; int64_t off, n[3]; double h[3]; };
+    std::vector<Part> parts;
+    std::thread writer;               // fvm_snapshot_save
+    int write_rc = 0;
+    std::string write_err;
+  } snaps[kSnapSlots];
+  hipStream_t copy_stream = nullptr;
+  int64_t next_snap = 1;
   PatchDesc* d_patch = nullptr;
   size_t patch_cap = 0;
   int64_t patch_points = 0;        // ghost points over all patches
@@ -258,6 +282,47 @@ __global__ __launch_bounds__(256) void compact_kernel(const double* __restrict__
     const int64_t bi = c * g.cs + gidx(g, i, j, k);
     if (to_block) dst[bi] = src[e];
     else dst[e] = src[bi];
+  }
+}
+
+// Snapshot variables of one block's u^n (real vertices) into `out`, one
+// contiguous array per variable, i fastest: the reference's output array
+// vdata(ni, nj, nk, nvars) (src/solve/class_output.f90:377-410).  Selector
+// 0..4: density, velocity x/y/z, static pressure (q -> p as
+// src/solve/class_flow_field.f90:383-396); 5..9: the conserved components
+// rho, rho u, rho v, rho w, rho E (2-D: w = rho w = 0).
+struct SnapSel {
+  int n;
+  int which[kSnapMaxVars];
+};
+
+template <int NC>
+__global__ __launch_bounds__(256) void snapshot_kernel(const double* __restrict__ q, BlockGeom g,
+                                                      double gamma, SnapSel sel,
+                                                      double* __restrict__ out) {
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ncell;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.nx);
+    const int j = (int)((e / g.nx) % g.ny);
+    const int k = (int)(e / ((int64_t)g.nx * g.ny));
+    const int64_t ci = gidx(g, i, j, k);
+    double v[NC], pr[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = q[ci + c * g.cs];
+    cons_to_prim<NC>(v, pr, gamma);
+    double all[10];
+    if constexpr (NC == 5) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        all[c] = pr[c];
+        all[5 + c] = v[c];
+      }
+    } else {
+      all[0] = pr[0]; all[1] = pr[1]; all[2] = pr[2]; all[3] = 0.0; all[4] = pr[3];
+      all[5] = v[0]; all[6] = v[1]; all[7] = v[2]; all[8] = 0.0; all[9] = v[3];
+    }
+    for (int n = 0; n < sel.n; ++n) out[n * ncell + e] = all[sel.which[n]];
   }
 }
 
@@ -931,6 +996,15 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
     free_block_state(b, nullptr);
     delete b;
   }
+  for (auto& sn : ctx->snaps) {
+    if (sn.writer.joinable()) sn.writer.join();
+    if (sn.done) (void)hipEventSynchronize(sn.done);
+    (void)hipFree(sn.d_buf);
+    (void)hipHostFree(sn.h_buf);
+    if (sn.done) (void)hipEventDestroy(sn.done);
+    if (sn.conv) (void)hipEventDestroy(sn.conv);
+  }
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   for (auto& sg : ctx->graphs)
     if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
@@ -1508,6 +1582,268 @@ int fvm_sync(fvm_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (int rc = set_device(ctx)) return rc;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+}  // extern "C"
+
+// ---- asynchronous snapshots and restart files -------------------------------
+
+namespace {
+
+// variable code -> snapshot_kernel selector
+int snap_selector(int code) {
+  switch (code) {
+    case FVM_VAR_DENSITY: return 0;
+    case FVM_VAR_VELOCITY_X: return 1;
+    case FVM_VAR_VELOCITY_Y: return 2;
+    case FVM_VAR_VELOCITY_Z: return 3;
+    case FVM_VAR_PRESSURE: return 4;
+    case FVM_VAR_RHO: return 5;
+    case FVM_VAR_RHOU: return 6;
+    case FVM_VAR_RHOV: return 7;
+    case FVM_VAR_RHOW: return 8;
+    case FVM_VAR_RHOE: return 9;
+  }
+  return -1;
+}
+
+fvm_ctx::Snap* find_snap(fvm_ctx* ctx, int64_t id) {
+  for (auto& sn : ctx->snaps)
+    if (id > 0 && sn.id == id) return &sn;
+  return nullptr;
+}
+
+constexpr char kSnapMagic[8] = {'F', 'V', 'M', 'S', 'N', 'A', 'P', '1'};
+
+// Snapshot file (little endian): magic "FVMSNAP1", int32 nblocks, int32
+// nvars, int32 codes[nvars], float64 time, int64 step; per block int64
+// ni, nj, nk and float64 dx, dy, dz; then per block nvars arrays of
+// ni*nj*nk float64, i fastest.
+void snap_write(fvm_ctx::Snap* sn, int device, std::string path) {
+  sn->write_rc = FVM_OK;
+  if (hipSetDevice(device) != hipSuccess || hipEventSynchronize(sn->done) != hipSuccess) {
+    sn->write_rc = FVM_EHIP;
+    sn->write_err = "snapshot copy failed";
+    return;
+  }
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    sn->write_rc = FVM_EINVAL;
+    sn->write_err = "cannot open " + path + " for writing";
+    return;
+  }
+  bool ok = std::fwrite(kSnapMagic, 1, 8, f) == 8;
+  const int32_t nb = (int32_t)sn->parts.size(), nv = sn->nvars;
+  ok = ok && std::fwrite(&nb, 4, 1, f) == 1 && std::fwrite(&nv, 4, 1, f) == 1;
+  for (int v = 0; v < nv && ok; ++v) {
+    const int32_t c = sn->codes[v];
+    ok = std::fwrite(&c, 4, 1, f) == 1;
+  }
+  ok = ok && std::fwrite(&sn->time, 8, 1, f) == 1 && std::fwrite(&sn->step, 8, 1, f) == 1;
+  for (const auto& pt : sn->parts)
+    ok = ok && std::fwrite(pt.n, 8, 3, f) == 3 && std::fwrite(pt.h, 8, 3, f) == 3;
+  size_t total = 0;
+  for (const auto& pt : sn->parts) total += (size_t)nv * pt.n[0] * pt.n[1] * pt.n[2];
+  ok = ok && std::fwrite(sn->h_buf, 8, total, f) == total;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    sn->write_rc = FVM_EINVAL;
+    sn->write_err = "write to " + path + " failed";
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_snapshot_begin(fvm_ctx* ctx, int nvars, const int* vars, int64_t* id) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!id || !vars) return fail(FVM_EINVAL, "null variable list or id");
+  if (nvars < 1 || nvars > kSnapMaxVars) return fail(FVM_EINVAL, "nvars %d out of range 1..%d", nvars, kSnapMaxVars);
+  if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
+  for (fvm_block* b : ctx->blocks)
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
+  SnapSel sel{};
+  sel.n = nvars;
+  for (int v = 0; v < nvars; ++v) {
+    sel.which[v] = snap_selector(vars[v]);
+    if (sel.which[v] < 0) return fail(FVM_EINVAL, "unknown variable code %d", vars[v]);
+  }
+  fvm_ctx::Snap* sn = nullptr;
+  for (auto& c : ctx->snaps)
+    if (c.id == 0) { sn = &c; break; }
+  if (!sn) return fail(FVM_ESTATE, "all %d snapshot slots are in use: release one", kSnapSlots);
+  if (int rc = set_device(ctx)) return rc;
+  if (!ctx->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  if (!sn->done) {   // recorded once at creation, so every later wait on it is defined
+    HIP_TRY(hipEventCreateWithFlags(&sn->done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(sn->done, ctx->copy_stream));
+  }
+  if (!sn->conv) HIP_TRY(hipEventCreateWithFlags(&sn->conv, hipEventDisableTiming));
+  sn->parts.clear();
+  size_t total = 0;
+  for (fvm_block* b : ctx->blocks) {
+    fvm_ctx::Snap::Part pt{b, (int64_t)total, {b->g.nx, b->g.ny, b->g.nz}, {b->g.dx, b->g.dy, b->g.dz}};
+    sn->parts.push_back(pt);
+    total += (size_t)nvars * b->g.nx * b->g.ny * b->g.nz;
+  }
+  if (total > sn->cap) {
+    // the slot's last copy (its snapshot was released) must land first
+    HIP_TRY(hipEventSynchronize(sn->done));
+    (void)hipFree(sn->d_buf);
+    (void)hipHostFree(sn->h_buf);
+    sn->d_buf = nullptr;
+    sn->h_buf = nullptr;
+    sn->cap = 0;
+    if (hipMalloc(&sn->d_buf, total * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&sn->h_buf, total * sizeof(double)) != hipSuccess) {
+      (void)hipFree(sn->d_buf);
+      (void)hipHostFree(sn->h_buf);
+      sn->d_buf = nullptr;
+      sn->h_buf = nullptr;
+      return fail(FVM_ENOMEM, "cannot allocate a %zu-byte snapshot slot", total * sizeof(double));
+    }
+    sn->cap = total;
+  }
+  // the kernel overwrites d_buf only after the slot's previous copy read it
+  HIP_TRY(hipStreamWaitEvent(ctx->stream, sn->done, 0));
+  for (size_t i = 0; i < ctx->blocks.size(); ++i) {
+    fvm_block* b = ctx->blocks[i];
+    const int64_t ncell = (int64_t)b->g.nx * b->g.ny * b->g.nz;
+    const unsigned grid = (unsigned)std::min<int64_t>(1024, (ncell + 255) / 256);
+    double* out = sn->d_buf + sn->parts[i].off;
+    if (b->nc == 5)
+      hipLaunchKernelGGL(snapshot_kernel<5>, dim3(grid), dim3(256), 0, ctx->stream, b->q[0], b->g,
+                         ctx->P.gamma, sel, out);
+    else
+      hipLaunchKernelGGL(snapshot_kernel<4>, dim3(grid), dim3(256), 0, ctx->stream, b->q[0], b->g,
+                         ctx->P.gamma, sel, out);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(sn->conv, ctx->stream));
+  HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, sn->conv, 0));
+  HIP_TRY(hipMemcpyAsync(sn->h_buf, sn->d_buf, total * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->copy_stream));
+  HIP_TRY(hipEventRecord(sn->done, ctx->copy_stream));
+  sn->nvars = nvars;
+  for (int v = 0; v < nvars; ++v) {
+    sn->which[v] = sel.which[v];
+    sn->codes[v] = vars[v];
+  }
+  sn->time = ctx->time;
+  sn->step = ctx->steps_done;
+  sn->id = ctx->next_snap++;
+  *id = sn->id;
+  return FVM_OK;
+}
+
+int fvm_snapshot_ready(fvm_ctx* ctx, int64_t id, int* ready) {
+  if (int rc = check_ctx(ctx)) return rc;
+  fvm_ctx::Snap* sn = find_snap(ctx, id);
+  if (!sn || !ready) return fail(FVM_EINVAL, "unknown snapshot %lld", (long long)id);
+  if (int rc = set_device(ctx)) return rc;
+  const hipError_t e = hipEventQuery(sn->done);
+  if (e == hipErrorNotReady) {
+    *ready = 0;
+    return FVM_OK;
+  }
+  HIP_TRY(e);
+  *ready = 1;
+  return FVM_OK;
+}
+
+int fvm_snapshot_read(fvm_ctx* ctx, int64_t id, fvm_block* blk, double* out, double* time,
+                      int64_t* step) {
+  if (int rc = check_ctx(ctx)) return rc;
+  fvm_ctx::Snap* sn = find_snap(ctx, id);
+  if (!sn) return fail(FVM_EINVAL, "unknown snapshot %lld", (long long)id);
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(hipEventSynchronize(sn->done));
+  if (time) *time = sn->time;
+  if (step) *step = sn->step;
+  if (!blk) return FVM_OK;
+  if (!out) return fail(FVM_EINVAL, "null output array");
+  for (const auto& pt : sn->parts)
+    if (pt.blk == blk) {
+      const size_t n = (size_t)sn->nvars * pt.n[0] * pt.n[1] * pt.n[2];
+      std::memcpy(out, sn->h_buf + pt.off, n * sizeof(double));
+      return FVM_OK;
+    }
+  return fail(FVM_EINVAL, "the block is not in snapshot %lld", (long long)id);
+}
+
+int fvm_snapshot_save(fvm_ctx* ctx, int64_t id, const char* path) {
+  if (int rc = check_ctx(ctx)) return rc;
+  fvm_ctx::Snap* sn = find_snap(ctx, id);
+  if (!sn) return fail(FVM_EINVAL, "unknown snapshot %lld", (long long)id);
+  if (!path) return fail(FVM_EINVAL, "null path");
+  if (sn->writer.joinable()) return fail(FVM_ESTATE, "snapshot %lld is already being saved", (long long)id);
+  sn->write_rc = FVM_OK;
+  sn->write_err.clear();
+  sn->writer = std::thread(snap_write, sn, ctx->device, std::string(path));
+  return FVM_OK;
+}
+
+int fvm_snapshot_release(fvm_ctx* ctx, int64_t id) {
+  if (int rc = check_ctx(ctx)) return rc;
+  fvm_ctx::Snap* sn = find_snap(ctx, id);
+  if (!sn) return fail(FVM_EINVAL, "unknown snapshot %lld", (long long)id);
+  int rc = FVM_OK;
+  if (sn->writer.joinable()) {
+    sn->writer.join();
+    rc = sn->write_rc;
+    if (rc) fail(rc, "%s", sn->write_err.c_str());
+  }
+  sn->id = 0;
+  sn->parts.clear();
+  return rc;
+}
+
+int fvm_restart_load(fvm_ctx* ctx, const char* path) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!path) return fail(FVM_EINVAL, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(FVM_EINVAL, "cannot open %s", path);
+  struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+  char magic[8];
+  int32_t nb = 0, nv = 0;
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kSnapMagic, 8) != 0 ||
+      std::fread(&nb, 4, 1, f) != 1 || std::fread(&nv, 4, 1, f) != 1 || nv < 1 || nv > kSnapMaxVars)
+    return fail(FVM_EINVAL, "%s is not a snapshot file", path);
+  int32_t codes[kSnapMaxVars];
+  if (std::fread(codes, 4, nv, f) != (size_t)nv) return fail(FVM_EINVAL, "%s: truncated header", path);
+  static const int kCons[5] = {FVM_VAR_RHO, FVM_VAR_RHOU, FVM_VAR_RHOV, FVM_VAR_RHOW, FVM_VAR_RHOE};
+  if (nv != 5 || !std::equal(codes, codes + 5, kCons))
+    return fail(FVM_EINVAL, "%s does not hold the conserved state (rho, rho u, rho v, rho w, rho E)", path);
+  double time = 0.0;
+  int64_t step = 0;
+  if (std::fread(&time, 8, 1, f) != 1 || std::fread(&step, 8, 1, f) != 1)
+    return fail(FVM_EINVAL, "%s: truncated header", path);
+  if (nb != (int32_t)ctx->blocks.size())
+    return fail(FVM_EINVAL, "%s holds %d blocks, the context %zu", path, nb, ctx->blocks.size());
+  for (int32_t i = 0; i < nb; ++i) {
+    int64_t n[3];
+    double h[3];
+    if (std::fread(n, 8, 3, f) != 3 || std::fread(h, 8, 3, f) != 3)
+      return fail(FVM_EINVAL, "%s: truncated header", path);
+    const BlockGeom& g = ctx->blocks[i]->g;
+    if (n[0] != g.nx || n[1] != g.ny || n[2] != g.nz)
+      return fail(FVM_EINVAL, "%s: block %d is %lldx%lldx%lld, the context's %dx%dx%d", path, i,
+                  (long long)n[0], (long long)n[1], (long long)n[2], g.nx, g.ny, g.nz);
+    if (h[0] != g.dx || h[1] != g.dy || h[2] != g.dz)
+      return fail(FVM_EINVAL, "%s: block %d spacing differs from the context's", path, i);
+  }
+  std::vector<double> buf;
+  for (int32_t i = 0; i < nb; ++i) {
+    fvm_block* b = ctx->blocks[i];
+    buf.resize((size_t)5 * b->g.nx * b->g.ny * b->g.nz);
+    if (std::fread(buf.data(), 8, buf.size(), f) != buf.size())
+      return fail(FVM_EINVAL, "%s: truncated data of block %d", path, i);
+    if (int rc = fvm_upload(b, buf.data(), 0)) return rc;
+  }
+  ctx->time = time;
+  ctx->steps_done = step;
   return FVM_OK;
 }
 

@@ -27,12 +27,21 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PER_BLOCK = 0, FVM_LAUNCH_BATCHED = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PERSISTENT = 2
+!...Snapshot variables: the reference's output variables
+!   (src/defs/type_defs_variables.f90:25-30) and the conserved components
+  integer(c_int), parameter, public :: FVM_VAR_DENSITY = 201, FVM_VAR_VELOCITY_X = 202
+  integer(c_int), parameter, public :: FVM_VAR_VELOCITY_Y = 203, FVM_VAR_VELOCITY_Z = 204
+  integer(c_int), parameter, public :: FVM_VAR_PRESSURE = 301
+  integer(c_int), parameter, public :: FVM_VAR_RHO = 1, FVM_VAR_RHOU = 2, FVM_VAR_RHOV = 3
+  integer(c_int), parameter, public :: FVM_VAR_RHOW = 4, FVM_VAR_RHOE = 5
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
+  public :: fvm_snapshot_begin, fvm_snapshot_ready, fvm_snapshot_read, fvm_snapshot_save
+  public :: fvm_snapshot_release, fvm_restart_load
   public :: fvm_check, fvm_error_text, scheme_index
 
   interface
@@ -150,6 +159,56 @@ module fvm_gpu
       integer(c_int), intent(in) :: donor_type(3)
       integer(c_int) :: rc
     end function fvm_conn_add
+
+    function fvm_snapshot_begin(ctx, nvars, vars, id) bind(C, name="fvm_snapshot_begin") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nvars
+      integer(c_int), intent(in) :: vars(*)
+      integer(c_int64_t), intent(out) :: id
+      integer(c_int) :: rc
+    end function fvm_snapshot_begin
+
+    function fvm_snapshot_ready(ctx, id, ready) bind(C, name="fvm_snapshot_ready") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: id
+      integer(c_int), intent(out) :: ready
+      integer(c_int) :: rc
+    end function fvm_snapshot_ready
+
+    ! out: c_loc of vdata(ni, nj, nk, nvars) (or c_null_ptr with blk = c_null_ptr)
+    function fvm_snapshot_read(ctx, id, blk, out, time, step) bind(C, name="fvm_snapshot_read") &
+        result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx, blk, out
+      integer(c_int64_t), value :: id
+      real(c_double), intent(out) :: time
+      integer(c_int64_t), intent(out) :: step
+      integer(c_int) :: rc
+    end function fvm_snapshot_read
+
+    function fvm_snapshot_save(ctx, id, path) bind(C, name="fvm_snapshot_save") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: id
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function fvm_snapshot_save
+
+    function fvm_snapshot_release(ctx, id) bind(C, name="fvm_snapshot_release") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: id
+      integer(c_int) :: rc
+    end function fvm_snapshot_release
+
+    function fvm_restart_load(ctx, path) bind(C, name="fvm_restart_load") result(rc)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function fvm_restart_load
 
     function fvm_viscous_set(ctx, mu_ref, t_ref, sutherland_s, prandtl, r_gas) &
         bind(C, name="fvm_viscous_set") result(rc)

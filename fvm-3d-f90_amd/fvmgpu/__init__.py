@@ -25,6 +25,12 @@ ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
 NUMERICS_EXACT, NUMERICS_FAST = 0, 1
 LAUNCH_PER_BLOCK, LAUNCH_BATCHED, LAUNCH_PERSISTENT = 0, 1, 2
+# snapshot variables: the reference's output variables
+# (src/defs/type_defs_variables.f90:25-30) and the conserved components
+VAR_DENSITY, VAR_VELOCITY_X, VAR_VELOCITY_Y, VAR_VELOCITY_Z, VAR_PRESSURE = 201, 202, 203, 204, 301
+VAR_RHO, VAR_RHOU, VAR_RHOV, VAR_RHOW, VAR_RHOE = 1, 2, 3, 4, 5
+PRIMITIVE_VARS = (VAR_DENSITY, VAR_VELOCITY_X, VAR_VELOCITY_Y, VAR_VELOCITY_Z, VAR_PRESSURE)
+CONSERVED_VARS = (VAR_RHO, VAR_RHOU, VAR_RHOV, VAR_RHOW, VAR_RHOE)
 
 # scheme names as the case file spells them (type_defs_scheme.f90:41-84)
 SCHEME_INDEX = {
@@ -42,7 +48,8 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_block_launch_shape", "fvm_block_connect", "fvm_set_launch_mode",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
-           "fvm_set_graph", "fvm_conn_add")
+           "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
+           "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load")
 
 
 class FvmError(RuntimeError):
@@ -99,6 +106,12 @@ def load_library(path=LIB_PATH):
         "fvm_conn_add": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64), vp, ctypes.POINTER(i64),
                           ctypes.POINTER(i64), ctypes.POINTER(ci)], ci),
         "fvm_time": ([vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
+        "fvm_snapshot_begin": ([vp, ci, ctypes.POINTER(ci), ctypes.POINTER(i64)], ci),
+        "fvm_snapshot_ready": ([vp, i64, ctypes.POINTER(ci)], ci),
+        "fvm_snapshot_read": ([vp, i64, vp, vp, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
+        "fvm_snapshot_save": ([vp, i64, ctypes.c_char_p], ci),
+        "fvm_snapshot_release": ([vp, i64], ci),
+        "fvm_restart_load": ([vp, ctypes.c_char_p], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
@@ -266,6 +279,41 @@ class Context:
     def sync(self):
         _check(_lib.fvm_sync(self._h))
 
+    def snapshot(self, variables=PRIMITIVE_VARS):
+        """Start an asynchronous snapshot of u^n of every block; returns its id."""
+        v = (ctypes.c_int * len(variables))(*(int(x) for x in variables))
+        sid = ctypes.c_int64()
+        _check(_lib.fvm_snapshot_begin(self._h, len(variables), v, ctypes.byref(sid)))
+        self._snap_nvars = getattr(self, "_snap_nvars", {})
+        self._snap_nvars[sid.value] = len(variables)
+        return sid.value
+
+    def snapshot_ready(self, sid):
+        r = ctypes.c_int()
+        _check(_lib.fvm_snapshot_ready(self._h, sid, ctypes.byref(r)))
+        return bool(r.value)
+
+    def snapshot_read(self, sid, blk):
+        """(array (nvars, nk, nj, ni), time, steps) of block `blk` in the snapshot."""
+        out = np.empty((self._snap_nvars[sid], blk.nk, blk.nj, blk.ni))
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        _check(_lib.fvm_snapshot_read(self._h, sid, blk._h, out.ctypes.data, ctypes.byref(t),
+                                      ctypes.byref(n)))
+        return out, t.value, n.value
+
+    def snapshot_save(self, sid, path):
+        """Write the snapshot to `path` on a background thread."""
+        _check(_lib.fvm_snapshot_save(self._h, sid, os.fsencode(path)))
+
+    def snapshot_release(self, sid):
+        """Free the slot (waits for a pending save and reports its status)."""
+        self._snap_nvars.pop(sid, None)
+        _check(_lib.fvm_snapshot_release(self._h, sid))
+
+    def restart(self, path):
+        """Load a conserved-state snapshot file into this context's blocks."""
+        _check(_lib.fvm_restart_load(self._h, os.fsencode(path)))
+
     def timing(self, every=1):
         """Time the stage launches of every `every`-th step (0 / False: off)."""
         _check(_lib.fvm_timing_enable(self._h, int(every)))
@@ -274,6 +322,31 @@ class Context:
         ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         _check(_lib.fvm_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cu)))
         return ms.value, n.value, cu.value
+
+
+def read_snapshot_file(path):
+    """Snapshot file -> dict(codes, time, step, blocks=[(shape (ni, nj, nk),
+    spacing, array (nvars, nk, nj, ni))])."""
+    with open(path, "rb") as f:
+        if f.read(8) != b"FVMSNAP1":
+            raise FvmError(f"{path} is not a snapshot file")
+        nb, nv = np.frombuffer(f.read(8), "<i4")
+        codes = np.frombuffer(f.read(4 * int(nv)), "<i4").tolist()
+        time = float(np.frombuffer(f.read(8), "<f8")[0])
+        step = int(np.frombuffer(f.read(8), "<i8")[0])
+        heads = []
+        for _ in range(int(nb)):
+            n = np.frombuffer(f.read(24), "<i8")
+            h = np.frombuffer(f.read(24), "<f8")
+            heads.append((tuple(int(x) for x in n), tuple(float(x) for x in h)))
+        blocks = []
+        for n, h in heads:
+            cnt = int(nv) * n[0] * n[1] * n[2]
+            a = np.frombuffer(f.read(8 * cnt), "<f8")
+            if a.size != cnt:
+                raise FvmError(f"{path}: truncated")
+            blocks.append((n, h, a.reshape(int(nv), n[2], n[1], n[0]).copy()))
+    return dict(codes=codes, time=time, step=step, blocks=blocks)
 
 
 class Server:

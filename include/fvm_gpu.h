@@ -61,6 +61,21 @@ extern "C" {
 #define FVM_ESTATE        -4     /* call out of order (e.g. no scheme set)     */
 #define FVM_ENAN          -5     /* NaN in the CFL wave speed                 */
 
+/* ---- snapshot variables ------------------------------------------------ */
+/* the reference's output variables (src/defs/type_defs_variables.f90:25-30,
+ * written by src/solve/class_output.f90:398-410) */
+#define FVM_VAR_DENSITY     201
+#define FVM_VAR_VELOCITY_X  202
+#define FVM_VAR_VELOCITY_Y  203
+#define FVM_VAR_VELOCITY_Z  204
+#define FVM_VAR_PRESSURE    301
+/* conserved components, copied unchanged (restart state; no reference code) */
+#define FVM_VAR_RHO         1
+#define FVM_VAR_RHOU        2
+#define FVM_VAR_RHOV        3
+#define FVM_VAR_RHOW        4
+#define FVM_VAR_RHOE        5
+
 typedef struct fvm_ctx fvm_ctx;
 typedef struct fvm_block fvm_block;
 
@@ -266,6 +281,41 @@ int fvm_server_result(fvm_server *srv, int64_t id, double *q_out, double info[8]
 
 /* Wait for all work of the context. */
 int fvm_sync(fvm_ctx *ctx);
+
+/* Asynchronous snapshot of the solution (the reference writes the output
+ * variables of every block at output times, ClassOutput%write_flowfield,
+ * src/solve/class_output.f90:204-237, from get_data :377-410, and names a
+ * restart file :194-195).  begin: at the current point of the context's
+ * stream a kernel writes the chosen variables of u^n of every block into a
+ * device slot, and a copy stream moves the slot into pinned host memory
+ * while later steps run; returns at once.  Up to 4 snapshots in flight; a
+ * slot is reused after release.  vars: FVM_VAR_* codes (up to 10).
+ * ready: 1 when the copy has landed (does not block).
+ * read: waits for the copy, then copies block blk's part into out as
+ * vdata(ni, nj, nk, nvars) in Fortran order (= C array [nvars][nk][nj][ni]),
+ * and the solution time and step count of the snapshot (blk may be NULL to
+ * read time and step only).
+ * save: writes the snapshot to a file on a background thread once the copy
+ * has landed (format below); release waits for the write and returns its
+ * status.  File: "FVMSNAP1", int32 nblocks, int32 nvars, int32 codes[nvars],
+ * float64 time, int64 step, per block int64 ni, nj, nk, float64 dx, dy, dz,
+ * then per block the nvars arrays, i fastest, little endian.
+ * Variables 201-301 convert q -> p as the reference does
+ * (src/solve/class_flow_field.f90:383-396); 1-5 copy the conserved state
+ * bit for bit. */
+int fvm_snapshot_begin(fvm_ctx *ctx, int nvars, const int *vars, int64_t *id);
+int fvm_snapshot_ready(fvm_ctx *ctx, int64_t id, int *ready);
+int fvm_snapshot_read(fvm_ctx *ctx, int64_t id, fvm_block *blk, double *out, double *time,
+                      int64_t *step);
+int fvm_snapshot_save(fvm_ctx *ctx, int64_t id, const char *path);
+int fvm_snapshot_release(fvm_ctx *ctx, int64_t id);
+
+/* Resume from a snapshot file holding the conserved state (variables 1..5
+ * in order): the context must hold the same blocks (count, order, extents
+ * and spacing) with their boundaries and connections set; uploads every
+ * block's state and sets the solution time and step count (the reference's
+ * restart file, src/solve/class_output.f90:194-195, has no reader). */
+int fvm_restart_load(fvm_ctx *ctx, const char *path);
 
 #ifdef __cplusplus
 }
