@@ -96,6 +96,11 @@ struct fvm_ctx {
     int face;
     int64_t rmin[3], rmax[3], dmin[3];
     int map_dir[3], map_sign[3];
+    int bc;              // 0: a connection; else the boundary kind that made it
+                         // (wall / symmetry: a mirror onto the block itself;
+                         // inflow / farfield: a fixed state)
+    bool fixed;
+    double prim[5];      // fixed state, primitive (rho, u, v, w, p)
   };
   std::vector<Patch> patches;
   // asynchronous snapshots (fvm_snapshot_*): slots of device and pinned host
@@ -246,8 +251,13 @@ __global__ void patch_copy_kernel(const BlockDesc* __restrict__ tab, const Patch
     for (int ex = 0; ex < 3; ++ex)
       d[P.map_dir[ex]] = P.dmin[P.map_dir[ex]] + P.map_sign[ex] * (r[ex] - P.rmin[ex]);
     const BlockGeom& gr = tab[P.recv].g;
-    const BlockGeom& gd = tab[P.donor].g;
     double* dst = tab[P.recv].q[buf];
+    if (P.fixed_state) {
+      const int64_t ri = gidx(gr, (int)r[0], (int)r[1], (int)r[2]);
+      for (int c = 0; c < P.nc; ++c) dst[ri + c * gr.cs] = P.fixed[c];
+      continue;
+    }
+    const BlockGeom& gd = tab[P.donor].g;
     const double* src = tab[P.donor].q[buf];
     const int64_t ri = gidx(gr, (int)r[0], (int)r[1], (int)r[2]);
     const int64_t di = gidx(gd, (int)d[0], (int)d[1], (int)d[2]);
@@ -675,6 +685,21 @@ int sync_patches(fvm_ctx* ctx) {
       d.dmin[e] = q.dmin[e];
     }
     d.start = start;
+    d.fixed_state = q.fixed ? 1 : 0;
+    if (q.fixed) {
+      // p -> q in the order of src/solve/class_flow_field.f90:353-377
+      const double rho = q.prim[0], u = q.prim[1], v = q.prim[2], w = q.prim[3], pr = q.prim[4];
+      const double e = pr / (ctx->P.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+      d.fixed[0] = rho;
+      d.fixed[1] = rho * u;
+      d.fixed[2] = rho * v;
+      if (q.recv->nc == 5) {
+        d.fixed[3] = rho * w;
+        d.fixed[4] = e;
+      } else {
+        d.fixed[3] = e;
+      }
+    }
     start += (int64_t)d.na * d.nb * q.recv->g.ng;
     h.push_back(d);
   }
@@ -1094,24 +1119,119 @@ int fvm_block_destroy(fvm_block* b) {
   return FVM_OK;
 }
 
-int fvm_block_set_bc(fvm_block* b, int face, int kind) {
+}  // extern "C"
+
+namespace {
+
+// Patches on one face of a block: drop them all.
+void drop_face_patches(fvm_ctx* ctx, fvm_block* b, int face) {
+  std::vector<fvm_ctx::Patch> keep;
+  for (const fvm_ctx::Patch& q : ctx->patches)
+    if (!(q.recv == b && q.face == face)) keep.push_back(q);
+  ctx->patches.swap(keep);
+}
+
+// The face's ghosts come from another block (a connection or connection
+// patches), not from a boundary kind.
+bool face_connected(fvm_ctx* ctx, fvm_block* b, int face) {
+  if (b->g.bc[face] == BC_CONN) return true;
+  if (b->g.bc[face] != BC_PATCH) return false;
+  for (const fvm_ctx::Patch& q : ctx->patches)
+    if (q.recv == b && q.face == face && q.bc == 0) return true;
+  return false;
+}
+
+enum BcClass { BCC_UNKNOWN, BCC_EXTRAP, BCC_PERIODIC, BCC_MIRROR, BCC_STATE, BCC_NOSLIP };
+
+BcClass bc_class(int kind) {
+  switch (kind) {
+    case FVM_BC_EXTRAP: case FVM_BC_OUTFLOW: case FVM_BC_OUTFLOW_SUBSONIC:
+    case FVM_BC_OUTFLOW_SUPERSONIC: case FVM_BC_EXTRAPOLATE: return BCC_EXTRAP;
+    case FVM_BC_PERIODIC: return BCC_PERIODIC;
+    case FVM_BC_WALL: case FVM_BC_WALL_INVISCID: case FVM_BC_SYMMETRY: return BCC_MIRROR;
+    case FVM_BC_INFLOW: case FVM_BC_INFLOW_SUBSONIC: case FVM_BC_INFLOW_SUPERSONIC:
+    case FVM_BC_FARFIELD: return BCC_STATE;
+    case 32: case 33: return BCC_NOSLIP;   // WallViscousIsothermal, WallViscous
+  }
+  return BCC_UNKNOWN;
+}
+
+int set_face_kind(fvm_block* b, int face, int kind, const double* prim) {
   if (!b) return fail(FVM_EINVAL, "null block");
   if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
-  if (kind != FVM_BC_EXTRAP && kind != FVM_BC_PERIODIC) return fail(FVM_EINVAL, "unknown BC kind %d", kind);
   if (b->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
+  const BcClass cls = bc_class(kind);
+  if (cls == BCC_UNKNOWN) return fail(FVM_EINVAL, "unknown BC kind %d", kind);
+  if (cls == BCC_NOSLIP) return fail(FVM_EINVAL, "no-slip walls (BC kind %d) are not supported", kind);
+  if (cls == BCC_STATE && !prim)
+    return fail(FVM_EINVAL, "BC kind %d needs a state: fvm_block_set_bc_state", kind);
+  if (cls != BCC_STATE && prim) return fail(FVM_EINVAL, "BC kind %d takes no state", kind);
+  fvm_ctx* ctx = b->ctx;
   const int d = face / 2, other = 2 * d + (1 - face % 2);
-  if (kind == FVM_BC_PERIODIC && (b->g.bc[face] == BC_CONN || b->g.bc[other] == BC_CONN))
-    return fail(FVM_EINVAL, "face %d is connected to another block", face);
-  if (b->g.bc[face] == BC_CONN || b->g.bc[face] == BC_PATCH)
-    return fail(FVM_EINVAL, "face %d is connected to another block", face);
-  if (kind == FVM_BC_PERIODIC && b->g.bc[other] == BC_PATCH)
+  if (face_connected(ctx, b, face)) return fail(FVM_EINVAL, "face %d is connected to another block", face);
+  if (cls == BCC_PERIODIC && face_connected(ctx, b, other))
     return fail(FVM_EINVAL, "face %d is connected to another block", other);
-  b->g.bc[face] = kind;
-  if (kind == FVM_BC_PERIODIC) b->g.bc[other] = kind;              // periodic on both faces
-  else if (b->g.bc[other] == BC_PERIODIC) b->g.bc[other] = BC_EXTRAP;
-  b->ctx->desc_dirty = true;
-  b->ctx->ghosts_dirty = true;
+  if ((cls == BCC_MIRROR || cls == BCC_STATE) && ctx->visc_mu_ref > 0.0)
+    return fail(FVM_EINVAL, "wall, symmetry, inflow and farfield faces are not supported with viscous terms");
+  const int n[3] = {b->g.nx, b->g.ny, b->g.nz};
+  if (cls == BCC_MIRROR && n[d] < kGhost + 1)
+    return fail(FVM_EINVAL, "a mirrored face needs >= %d vertices across the block", kGhost + 1);
+  if (cls == BCC_STATE) {
+    for (int c = 0; c < 5; ++c)
+      if (!std::isfinite(prim[c])) return fail(FVM_EINVAL, "non-finite boundary state");
+    if (!(prim[0] > 0.0) || !(prim[4] > 0.0))
+      return fail(FVM_EINVAL, "boundary state needs rho > 0 and p > 0");
+    if (b->dim == 2 && prim[3] != 0.0) return fail(FVM_EINVAL, "2-D block: w of the boundary state must be 0");
+  }
+  drop_face_patches(ctx, b, face);
+  if (cls == BCC_PERIODIC) drop_face_patches(ctx, b, other);
+  if (b->g.bc[other] == BC_PERIODIC && cls != BCC_PERIODIC) b->g.bc[other] = BC_EXTRAP;
+  switch (cls) {
+    case BCC_EXTRAP:
+      b->g.bc[face] = BC_EXTRAP;
+      break;
+    case BCC_PERIODIC:
+      b->g.bc[face] = b->g.bc[other] = BC_PERIODIC;    // periodic on both faces
+      break;
+    default: {
+      // the whole face as one patch onto the block itself: a mirror about the
+      // boundary plane (direction d reversed) or a fixed state
+      fvm_ctx::Patch q{};
+      q.recv = q.donor = b;
+      q.face = face;
+      q.bc = kind;
+      for (int e = 0; e < 3; ++e) {
+        q.rmin[e] = 0;
+        q.rmax[e] = n[e] - 1;
+        q.map_dir[e] = e;
+        q.map_sign[e] = 1;
+      }
+      q.rmin[d] = q.rmax[d] = face % 2 ? n[d] - 1 : 0;
+      for (int e = 0; e < 3; ++e) q.dmin[e] = q.rmin[e];
+      if (cls == BCC_STATE) {
+        q.fixed = true;
+        for (int c = 0; c < 5; ++c) q.prim[c] = prim[c];
+      } else {
+        q.map_sign[d] = -1;
+      }
+      ctx->patches.push_back(q);
+      b->g.bc[face] = BC_PATCH;
+    }
+  }
+  ctx->desc_dirty = true;
+  ctx->ghosts_dirty = true;
   return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_block_set_bc(fvm_block* b, int face, int kind) { return set_face_kind(b, face, kind, nullptr); }
+
+int fvm_block_set_bc_state(fvm_block* b, int face, int kind, const double prim[5]) {
+  if (!prim) return fail(FVM_EINVAL, "null boundary state");
+  return set_face_kind(b, face, kind, prim);
 }
 
 int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
@@ -1133,6 +1253,11 @@ int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
     return fail(FVM_EINVAL, "a block cannot share a vertex plane with itself");
   }
   fvm_ctx* ctx = a->ctx;
+  if ((a->g.bc[face] == BC_PATCH && face_connected(ctx, a, face)) ||
+      (b->g.bc[bface] == BC_PATCH && face_connected(ctx, b, bface)))
+    return fail(FVM_EINVAL, "a face already has connection patches");
+  drop_face_patches(ctx, a, face);
+  drop_face_patches(ctx, b, bface);
   const int ia = block_index(ctx, a), ib = block_index(ctx, b);
   a->g.bc[face] = BC_CONN; a->nbr[face] = ib; a->nbr_shared[face] = shared ? 1 : 0;
   b->g.bc[bface] = BC_CONN; b->nbr[bface] = ia; b->nbr_shared[bface] = shared ? 1 : 0;
@@ -1204,6 +1329,8 @@ int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t rang
                             "mapped into the donor");
   if (recv->g.bc[face] == BC_CONN || recv->g.bc[face] == BC_PERIODIC)
     return fail(FVM_EINVAL, "face %d of the receiving block already has a connection or periodic BC", face);
+  if (recv->g.bc[face] == BC_PATCH && !face_connected(ctx, recv, face))
+    drop_face_patches(ctx, recv, face);   // a wall / symmetry / state face becomes connected
   q.recv = recv;
   q.donor = donor;
   q.face = face;

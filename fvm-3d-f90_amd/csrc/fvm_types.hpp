@@ -53,6 +53,8 @@ struct PatchDesc {
   int map_dir[3], map_sign[3];     // receiver direction e -> donor direction, sign
   int64_t rmin[3], dmin[3];        // corresponding corners
   int64_t start;                   // first point of this patch in the launch's point list
+  int fixed_state;                 // 1: the ghosts take `fixed` (farfield / inflow faces)
+  double fixed[5];                 // conserved state, device component order
 };
 
 // One block as the stage kernels see it.

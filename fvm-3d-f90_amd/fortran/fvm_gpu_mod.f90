@@ -24,6 +24,13 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_ROE = 300, FVM_VANLEER = 301, FVM_AUSM = 303
   integer(c_int), parameter, public :: FVM_AUSMPLUS = 1303
   integer(c_int), parameter, public :: FVM_BC_EXTRAP = 0, FVM_BC_PERIODIC = 1
+!...the reference's BC codes (src/defs/type_defs_bc.f90:11-30)
+  integer(c_int), parameter, public :: FVM_BC_INFLOW = 10, FVM_BC_INFLOW_SUBSONIC = 11
+  integer(c_int), parameter, public :: FVM_BC_INFLOW_SUPERSONIC = 12, FVM_BC_OUTFLOW = 20
+  integer(c_int), parameter, public :: FVM_BC_OUTFLOW_SUBSONIC = 21
+  integer(c_int), parameter, public :: FVM_BC_OUTFLOW_SUPERSONIC = 22, FVM_BC_WALL = 30
+  integer(c_int), parameter, public :: FVM_BC_WALL_INVISCID = 31, FVM_BC_FARFIELD = 40
+  integer(c_int), parameter, public :: FVM_BC_EXTRAPOLATE = 41, FVM_BC_SYMMETRY = 50
   integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PER_BLOCK = 0, FVM_LAUNCH_BATCHED = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PERSISTENT = 2
@@ -36,7 +43,7 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_VAR_RHOW = 4, FVM_VAR_RHOE = 5
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
-  public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc
+  public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc, fvm_block_set_bc_state
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
@@ -140,6 +147,16 @@ module fvm_gpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function fvm_sync
+
+    ! prim = (rho, u, v, w, p) of an inflow / farfield face
+    function fvm_block_set_bc_state(blk, face, kind, prim) bind(C, name="fvm_block_set_bc_state") &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: blk
+      integer(c_int), value :: face, kind
+      real(c_double), intent(in) :: prim(5)
+      integer(c_int) :: rc
+    end function fvm_block_set_bc_state
 
     function fvm_block_connect(a, face, b, shared) bind(C, name="fvm_block_connect") result(rc)
       import :: c_int, c_ptr

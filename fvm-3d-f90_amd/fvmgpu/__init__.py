@@ -23,6 +23,10 @@ SIMPLE, MUSCL, WENO = 200, 201, 202
 LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
+# the reference's BC codes (src/defs/type_defs_bc.f90:11-30)
+BC_INFLOW, BC_INFLOW_SUBSONIC, BC_INFLOW_SUPERSONIC = 10, 11, 12
+BC_OUTFLOW, BC_OUTFLOW_SUBSONIC, BC_OUTFLOW_SUPERSONIC = 20, 21, 22
+BC_WALL, BC_WALL_INVISCID, BC_FARFIELD, BC_EXTRAPOLATE, BC_SYMMETRY = 30, 31, 40, 41, 50
 NUMERICS_EXACT, NUMERICS_FAST = 0, 1
 LAUNCH_PER_BLOCK, LAUNCH_BATCHED, LAUNCH_PERSISTENT = 0, 1, 2
 # snapshot variables: the reference's output variables
@@ -49,7 +53,8 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_server_create", "fvm_server_destroy", "fvm_server_submit", "fvm_server_poll",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
-           "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load")
+           "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
+           "fvm_block_set_bc_state")
 
 
 class FvmError(RuntimeError):
@@ -112,6 +117,7 @@ def load_library(path=LIB_PATH):
         "fvm_snapshot_save": ([vp, i64, ctypes.c_char_p], ci),
         "fvm_snapshot_release": ([vp, i64], ci),
         "fvm_restart_load": ([vp, ctypes.c_char_p], ci),
+        "fvm_block_set_bc_state": ([vp, ci, ci, ctypes.POINTER(cd)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
@@ -149,6 +155,11 @@ class Block:
 
     def set_bc(self, face, kind):
         _check(_lib.fvm_block_set_bc(self._h, face, kind))
+
+    def set_bc_state(self, face, kind, prim):
+        """Inflow / farfield face: ghosts hold the primitive state (rho, u, v, w, p)."""
+        _check(_lib.fvm_block_set_bc_state(self._h, face, kind,
+                                           (ctypes.c_double * 5)(*(float(x) for x in prim))))
 
     def connect(self, face, other, shared=True):
         """Face `face` of this block abuts the opposite face of `other`."""

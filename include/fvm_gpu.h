@@ -52,6 +52,19 @@ extern "C" {
 /* ---- boundary kinds (fvm_block_set_bc) --------------------------------- */
 #define FVM_BC_EXTRAP      0     /* zeroth-order ghost copy, the reference's only BC */
 #define FVM_BC_PERIODIC    1     /* nodal periodic: vertex n is vertex 0       */
+/* the reference's BC codes (src/defs/type_defs_bc.f90:11-30), which it
+ * declares but never applies: */
+#define FVM_BC_INFLOW             10  /* fixed ghost state (fvm_block_set_bc_state) */
+#define FVM_BC_INFLOW_SUBSONIC    11  /* fixed ghost state                          */
+#define FVM_BC_INFLOW_SUPERSONIC  12  /* fixed ghost state                          */
+#define FVM_BC_OUTFLOW            20  /* extrapolation                               */
+#define FVM_BC_OUTFLOW_SUBSONIC   21  /* extrapolation                               */
+#define FVM_BC_OUTFLOW_SUPERSONIC 22  /* extrapolation                               */
+#define FVM_BC_WALL               30  /* inviscid (slip) wall: mirror, normal momentum negated */
+#define FVM_BC_WALL_INVISCID      31  /* as FVM_BC_WALL                              */
+#define FVM_BC_FARFIELD           40  /* fixed free-stream ghost state (fvm_block_set_bc_state) */
+#define FVM_BC_EXTRAPOLATE        41  /* extrapolation                               */
+#define FVM_BC_SYMMETRY           50  /* mirror, normal momentum negated             */
 
 /* ---- error codes ------------------------------------------------------- */
 #define FVM_OK             0
@@ -123,8 +136,20 @@ int fvm_block_destroy(fvm_block *blk);
 
 /* Boundary kind of one block face: face 0..5 = i-, i+, j-, j+, k-, k+.
  * Default FVM_BC_EXTRAP (src/base/defs_block_data.f90:226-256).  Periodic
- * applies to both faces of the direction. */
+ * applies to both faces of the direction.  Wall and symmetry faces mirror
+ * the block about the boundary vertex plane (ghost l = vertex l inside,
+ * momentum normal to the face negated); outflow and extrapolate kinds copy
+ * the boundary vertex.  Inflow and farfield faces take a state:
+ * fvm_block_set_bc_state.  A face's kind replaces its previous one
+ * (connected faces excepted).  Wall, symmetry and state faces are refilled
+ * on the device after every stage; not with viscous terms. */
 int fvm_block_set_bc(fvm_block *blk, int face, int kind);
+
+/* Inflow / farfield face (FVM_BC_INFLOW*, FVM_BC_FARFIELD): every ghost
+ * layer holds the primitive state prim = (rho, u, v, w, p) along the block's
+ * axes, converted to conserved variables as the reference's initial state
+ * is (src/solve/class_flow_field.f90:353-377; 2-D blocks: w must be 0). */
+int fvm_block_set_bc_state(fvm_block *blk, int face, int kind, const double prim[5]);
 
 /* Block-to-block connectivity (the reference builds the metadata,
  * ClassGeomBlock%connect src/base/defs_block_geom.f90:953-990 and donor ranges
