@@ -336,6 +336,95 @@ __global__ __launch_bounds__(256) void snapshot_kernel(const double* __restrict_
   }
 }
 
+// ---- initial states computed on the device ---------------------------------
+constexpr int kInitMaxRegions = 16;
+struct InitSpec {
+  int kind, nregions;
+  double origin[3];
+  double gamma;
+  // regions: per region xmin, xmax, ymin, ymax, zmin, zmax, rho, u, v, w, p;
+  // vortex: length, beta, xc, yc, u_inf, v_inf, w_inf; TGV: Mach number
+  double v[11 * kInitMaxRegions];
+};
+
+// Python's float modulo for a positive period: the result takes the sign of
+// the period
+__device__ __forceinline__ double floor_mod(double a, double period) {
+  double r = fmod(a, period);
+  if (r != 0.0) {
+    if (r < 0.0) r += period;
+  } else {
+    r = 0.0;
+  }
+  return r;
+}
+
+// Primitive state of one vertex at (x, y, z) for the given initial state.
+__device__ void init_state(const InitSpec& s, double x, double y, double z, double p[5]) {
+  const double pi = 3.141592653589793;
+  if (s.kind == FVM_IC_REGIONS) {
+    // src/base/defs_block_data.f90:166-201: regions in order, inclusive bounds
+    p[0] = p[1] = p[2] = p[3] = p[4] = 0.0;
+    for (int r = 0; r < s.nregions; ++r) {
+      const double* b = s.v + 11 * r;
+      if (x >= b[0] && x <= b[1] && y >= b[2] && y <= b[3] && z >= b[4] && z <= b[5])
+        for (int c = 0; c < 5; ++c) p[c] = b[6 + c];
+    }
+  } else if (s.kind == FVM_IC_VORTEX) {
+    // Yee's isentropic vortex about (xc, yc), periodic in x and y with period
+    // `length`; free stream rho = p = 1
+    const double L = s.v[0], beta = s.v[1], g = s.gamma;
+    const double dxv = floor_mod(x - s.v[2] + 0.5 * L, L) - 0.5 * L;
+    const double dyv = floor_mod(y - s.v[3] + 0.5 * L, L) - 0.5 * L;
+    const double r2 = dxv * dxv + dyv * dyv;
+    const double e = exp(0.5 * (1.0 - r2));
+    const double t = 1.0 - (g - 1.0) * beta * beta / (8.0 * g * pi * pi) * e * e;
+    const double rho = pow(t, 1.0 / (g - 1.0));
+    p[0] = rho;
+    p[1] = s.v[4] - beta / (2.0 * pi) * e * dyv;
+    p[2] = s.v[5] + beta / (2.0 * pi) * e * dxv;
+    p[3] = s.v[6];
+    p[4] = pow(rho, g);
+  } else {
+    // Taylor-Green vortex, rho = p / (R T0) with R = 1 / (gamma Ma^2), T0 = 1
+    const double r = 1.0 / (s.gamma * s.v[0] * s.v[0]);
+    const double pr = r + (cos(2.0 * x) + cos(2.0 * y)) * (cos(2.0 * z) + 2.0) / 16.0;
+    p[0] = pr / r;
+    p[1] = sin(x) * cos(y) * cos(z);
+    p[2] = -cos(x) * sin(y) * cos(z);
+    p[3] = 0.0;
+    p[4] = pr;
+  }
+}
+
+// u^n of the real vertices of one block from an initial-state spec; p -> q
+// in the order of src/solve/class_flow_field.f90:353-377.
+__global__ __launch_bounds__(256) void init_kernel(double* __restrict__ q, BlockGeom g, int nc,
+                                                  InitSpec s) {
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ncell;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.nx);
+    const int j = (int)((e / g.nx) % g.ny);
+    const int k = (int)(e / ((int64_t)g.nx * g.ny));
+    double p[5];
+    init_state(s, s.origin[0] + g.dx * (double)i, s.origin[1] + g.dy * (double)j,
+               s.origin[2] + g.dz * (double)k, p);
+    const double rho = p[0], u = p[1], v = p[2], w = p[3];
+    const double en = p[4] / (s.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+    const int64_t ci = gidx(g, i, j, k);
+    q[ci] = rho;
+    q[ci + g.cs] = rho * u;
+    q[ci + 2 * g.cs] = rho * v;
+    if (nc == 5) {
+      q[ci + 3 * g.cs] = rho * w;
+      q[ci + 4 * g.cs] = en;
+    } else {
+      q[ci + 3 * g.cs] = en;
+    }
+  }
+}
+
 // Max over real vertices of sum_d (|u_d| + c) / dr_d.
 template <int NC>
 __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__ q, BlockGeom g,
@@ -1709,6 +1798,58 @@ int fvm_sync(fvm_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (int rc = set_device(ctx)) return rc;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int fvm_block_init(fvm_block* b, int kind, int nparams, const double* params, const double origin[3]) {
+  if (!b) return fail(FVM_EINVAL, "null block");
+  fvm_ctx* ctx = b->ctx;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called (gamma)");
+  if (nparams < 0 || (nparams > 0 && !params)) return fail(FVM_EINVAL, "null parameters");
+  InitSpec s{};
+  s.kind = kind;
+  s.gamma = ctx->P.gamma;
+  for (int e = 0; e < 3; ++e) s.origin[e] = origin ? origin[e] : 0.0;
+  for (int e = 0; e < 3; ++e)
+    if (!std::isfinite(s.origin[e])) return fail(FVM_EINVAL, "non-finite origin");
+  for (int c = 0; c < nparams; ++c)
+    if (std::isnan(params[c])) return fail(FVM_EINVAL, "NaN parameter %d", c);
+  switch (kind) {
+    case FVM_IC_REGIONS:
+      if (nparams < 11 || nparams % 11 || nparams / 11 > kInitMaxRegions)
+        return fail(FVM_EINVAL, "regions: 11 parameters per region, 1..%d regions", kInitMaxRegions);
+      s.nregions = nparams / 11;
+      for (int r = 0; r < s.nregions; ++r) {
+        const double* st = params + 11 * r + 6;
+        if (!(st[0] > 0.0) || !(st[4] > 0.0)) return fail(FVM_EINVAL, "region %d: rho and p must be > 0", r);
+        if (b->dim == 2 && st[3] != 0.0) return fail(FVM_EINVAL, "2-D block: region %d has w != 0", r);
+      }
+      break;
+    case FVM_IC_VORTEX:
+      if (nparams != 7) return fail(FVM_EINVAL, "vortex: 7 parameters (length, beta, xc, yc, u, v, w)");
+      if (!(params[0] > 0.0) || !std::isfinite(params[0])) return fail(FVM_EINVAL, "vortex: length must be > 0");
+      if (b->dim == 2 && params[6] != 0.0) return fail(FVM_EINVAL, "2-D block: w must be 0");
+      break;
+    case FVM_IC_TGV:
+      if (nparams != 1 || !(params[0] > 0.0)) return fail(FVM_EINVAL, "Taylor-Green: 1 parameter, Mach > 0");
+      if (b->dim == 2) return fail(FVM_EINVAL, "Taylor-Green needs a 3-D block");
+      break;
+    default:
+      return fail(FVM_EINVAL, "unknown initial state kind %d", kind);
+  }
+  for (int c = 0; c < nparams; ++c) s.v[c] = params[c];
+  if (int rc = set_device(ctx)) return rc;
+  const int64_t ncell = (int64_t)b->g.nx * b->g.ny * b->g.nz;
+  const unsigned grid = (unsigned)std::min<int64_t>(2048, (ncell + 255) / 256);
+  hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(256), 0, ctx->stream, b->q[0], b->g, b->nc, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  b->uploaded = true;
+  ctx->ghosts_dirty = true;
   return FVM_OK;
 }
 

@@ -15,7 +15,9 @@
 !  Extra group (optional, not in the reference):
 !    &GPU  device = 0, numerics = "exact" | "fast", cfl = 0.5,
 !          max_step = 0 (0: no limit), print_every = 1,
-!          output_file = "" (raw dump of the final q with 3 ghost layers) /
+!          output_file = "" (raw dump of the final q with 3 ghost layers),
+!          snapshot_prefix = "" (write the &Output variables at the output
+!          times of &Time_Stepping to <prefix>_<step>.fvmsnap) /
 !
 !=============================================================================80
 program fvm_driver
@@ -39,11 +41,12 @@ program fvm_driver
   real(dp) :: fix_timestep = 0.0_dp, total_time = 0.0_dp, output_time = 0.0_dp
   real(dp) :: absolute_residual = 0.0_dp, relative_residual = 0.0_dp
   character(len=64) :: format = ""
-  logical :: Density = .false., VelocityX = .false., VelocityY = .false., Pressure = .false.
+  logical :: Density = .false., VelocityX = .false., VelocityY = .false., VelocityZ = .false.
+  logical :: Pressure = .false.
   integer :: device = 0, max_step = 0, print_every = 1
   character(len=16) :: numerics = "exact"
   real(dp) :: cfl = 0.5_dp
-  character(len=256) :: output_file = ""
+  character(len=256) :: output_file = "", snapshot_prefix = ""
 
   namelist /General/ name, dimension, dimensionless, equation, gasModel
   namelist /Geometry/ type, geoType, geoSize, gridType, gridSize, writeGrid
@@ -52,15 +55,19 @@ program fvm_driver
   ! Initial_Condition) carry nothing and are skipped by the namelist reads
   namelist /Time_Stepping/ time_scheme, timestep_set, fix_timestep, total_time, &
                            output_time, absolute_residual, relative_residual
-  namelist /Output/ format, Density, VelocityX, VelocityY, Pressure
-  namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file
+  namelist /Output/ format, Density, VelocityX, VelocityY, VelocityZ, Pressure
+  namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file, snapshot_prefix
 
   character(len=512) :: arg, casefile
   integer :: unit, ios, i, j, k, ni, nj, nk, step, nan_flag, eq
   real(dp) :: dx, dy, dz, gamma, time, dt, dt_used, res(6), res0(6)
   real(dp), allocatable, target :: q(:,:,:,:)
-  real(dp), allocatable :: x(:), y(:), p(:,:,:,:)
   type(c_ptr) :: ctx, blk
+  real(dp) :: regions(11, 4), time_out
+  integer(c_int) :: vars(5), nvars
+  integer(c_int64_t) :: snaps(4)
+  integer :: nsnap
+  logical :: writing
   integer(c_int) :: rc, recon_c, lim_c, flux_c, rk_c, mode
   logical :: converged
 
@@ -84,6 +91,7 @@ program fvm_driver
   rewind(unit); read(unit, nml=Geometry, iostat=ios);      call need(ios, "Geometry")
   rewind(unit); read(unit, nml=Inviscid_Flux, iostat=ios); call need(ios, "Inviscid_Flux")
   rewind(unit); read(unit, nml=Time_Stepping, iostat=ios); call need(ios, "Time_Stepping")
+  rewind(unit); read(unit, nml=Output, iostat=ios)         ! optional group
   rewind(unit); read(unit, nml=GPU, iostat=ios)            ! optional group
   close(unit)
 
@@ -103,35 +111,23 @@ program fvm_driver
   dy = geoSize / real(nj - 1, dp)
   dz = 1.0_dp
   if (nk > 1) dz = geoSize / real(nk - 1, dp)
-  allocate(x(ni), y(nj))
-  do i = 1, ni
-    x(i) = dx * real(i - 1, dp)
-  end do
-  do j = 1, nj
-    y(j) = dy * real(j - 1, dp)
-  end do
 
-!...Initial state: four quadrants, regions applied in order with inclusive
-!   bounds so later regions win on the x = 0.8 / y = 0.8 lines
-  allocate(p(5, ni, nj, nk))
-  p = 0.0_dp
-  call set_region(0.8_dp, 1.0_dp, 0.8_dp, 1.0_dp, [1.5_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.5_dp])
-  call set_region(0.0_dp, 0.8_dp, 0.8_dp, 1.0_dp, [0.5323_dp, 1.206_dp, 0.0_dp, 0.0_dp, 0.3_dp])
-  call set_region(0.0_dp, 0.8_dp, 0.0_dp, 0.8_dp, [0.138_dp, 1.206_dp, 1.206_dp, 0.0_dp, 0.029_dp])
-  call set_region(0.8_dp, 1.0_dp, 0.0_dp, 0.8_dp, [0.5323_dp, 0.0_dp, 1.206_dp, 0.0_dp, 0.3_dp])
+!...Initial state: the four quadrants of src/solve/class_flow_field.f90:
+!   207-243, regions applied in order with inclusive bounds (later regions
+!   win on the x = 0.8 / y = 0.8 lines), computed on the device
+  regions(:, 1) = [0.8_dp, 1.0_dp, 0.8_dp, 1.0_dp, -huge(1.0_dp), huge(1.0_dp), &
+                   1.5_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.5_dp]
+  regions(:, 2) = [0.0_dp, 0.8_dp, 0.8_dp, 1.0_dp, -huge(1.0_dp), huge(1.0_dp), &
+                   0.5323_dp, 1.206_dp, 0.0_dp, 0.0_dp, 0.3_dp]
+  regions(:, 3) = [0.0_dp, 0.8_dp, 0.0_dp, 0.8_dp, -huge(1.0_dp), huge(1.0_dp), &
+                   0.138_dp, 1.206_dp, 1.206_dp, 0.0_dp, 0.029_dp]
+  regions(:, 4) = [0.8_dp, 1.0_dp, 0.0_dp, 0.8_dp, -huge(1.0_dp), huge(1.0_dp), &
+                   0.5323_dp, 0.0_dp, 1.206_dp, 0.0_dp, 0.3_dp]
   if (nk == 1) then
     allocate(q(ni + 2*NG, nj + 2*NG, 1, 5))
   else
     allocate(q(ni + 2*NG, nj + 2*NG, nk + 2*NG, 5))
   end if
-  q = 0.0_dp
-  do k = 1, nk
-    do j = 1, nj
-      do i = 1, ni
-        call p_to_q(p(:, i, j, k), q(i + NG, j + NG, k + merge(0, NG, nk == 1), :))
-      end do
-    end do
-  end do
 
 !...Scheme selection (codes of src/defs/type_defs_scheme.f90)
   recon_c = scheme_index(reconstruct)
@@ -151,7 +147,18 @@ program fvm_driver
   call fvm_check(fvm_set_numerics(ctx, mode), "set_numerics")
   call fvm_check(fvm_block_create(ctx, int(ni, c_int64_t), int(nj, c_int64_t), &
                                   int(nk, c_int64_t), dx, dy, dz, blk), "block_create")
-  call fvm_check(fvm_upload(blk, c_loc(q), int(NG, c_int)), "upload")
+  call fvm_check(fvm_block_init(blk, FVM_IC_REGIONS, 44_c_int, regions, c_null_ptr), "init")
+
+!...Output variables of &Output (src/solve/class_output.f90:337-374)
+  nvars = 0
+  if (Density) call add_var(FVM_VAR_DENSITY)
+  if (VelocityX) call add_var(FVM_VAR_VELOCITY_X)
+  if (VelocityY) call add_var(FVM_VAR_VELOCITY_Y)
+  if (VelocityZ) call add_var(FVM_VAR_VELOCITY_Z)
+  if (Pressure) call add_var(FVM_VAR_PRESSURE)
+  writing = len_trim(snapshot_prefix) > 0 .and. nvars > 0 .and. output_time > 0.0_dp
+  nsnap = 0
+  time_out = 0.0_dp
 
 !...Time loop (src/process/solver_sub.f90:196-206)
   write(*, '(a, a, a, i0, a, i0, a, i0)') "case ", trim(name), ": ", ni, " x ", nj, " x ", nk
@@ -159,6 +166,7 @@ program fvm_driver
   time = 0.0_dp
   step = 0
   converged = .false.
+  if (writing) call write_output()          ! the initial flowfield
   do while (.not. converged)
     step = step + 1
     if (trim(timestep_set) == "CFL") then
@@ -169,6 +177,10 @@ program fvm_driver
     rc = fvm_step(ctx, 1_c_int, dt, cfl, dt_used)
     call fvm_check(rc, "step")
     time = time + dt_used
+    ! ClassOutput%write_flowfield: at the output times, within 1e-9
+    if (writing) then
+      if (abs(time - time_out) <= 1.0e-9_dp) call write_output()
+    end if
     call fvm_check(fvm_norms(ctx, res, nan_flag), "norms")
     if (step == 1) res0 = res
     if (nan_flag /= 0) then
@@ -184,6 +196,9 @@ program fvm_driver
   end do
   write(*, '(a, i0, a, es14.6)') "finished after ", step, " steps at t = ", time
 
+  do i = 1, nsnap
+    call fvm_check(fvm_snapshot_release(ctx, snaps(i)), "snapshot write")
+  end do
   call fvm_check(fvm_download(blk, c_loc(q), int(NG, c_int)), "download")
   if (len_trim(output_file) > 0) then
     open(newunit=unit, file=trim(output_file), access="stream", form="unformatted", &
@@ -206,29 +221,32 @@ contains
     end if
   end subroutine need
 
-  !> init_value_sub (src/base/defs_block_data.f90:166-201) for one region
-  subroutine set_region(x0, x1, y0, y1, state)
-    real(dp), intent(in) :: x0, x1, y0, y1, state(5)
-    integer :: ii, jj, kk
-    do kk = 1, nk
-      do jj = 1, nj
-        do ii = 1, ni
-          if (x(ii) >= x0 .and. x(ii) <= x1 .and. y(jj) >= y0 .and. y(jj) <= y1) &
-            p(:, ii, jj, kk) = state
-        end do
-      end do
-    end do
-  end subroutine set_region
+  subroutine add_var(code)
+    integer(c_int), intent(in) :: code
+    nvars = nvars + 1
+    vars(nvars) = code
+  end subroutine add_var
 
-  !> convert_p_to_q_blk_sub (src/solve/class_flow_field.f90:353-377)
-  subroutine p_to_q(pv, qv)
-    real(dp), intent(in) :: pv(5)
-    real(dp), intent(out) :: qv(5)
-    qv(1) = pv(1)
-    qv(2) = pv(1) * pv(2)
-    qv(3) = pv(1) * pv(3)
-    qv(4) = pv(1) * pv(4)
-    qv(5) = pv(5) / (gamma - 1.0_dp) + 0.5_dp * pv(1) * (pv(2)**2 + pv(3)**2 + pv(4)**2)
-  end subroutine p_to_q
+  !> ClassOutput%write_flowfield (src/solve/class_output.f90:204-237): an
+  !! asynchronous snapshot of the output variables, saved on a background
+  !! thread; at most four in flight
+  subroutine write_output()
+    integer(c_int64_t) :: sid
+    character(len=300) :: fname
+    integer :: m
+    if (nsnap == size(snaps)) then
+      call fvm_check(fvm_snapshot_release(ctx, snaps(1)), "snapshot write")
+      do m = 1, nsnap - 1
+        snaps(m) = snaps(m + 1)
+      end do
+      nsnap = nsnap - 1
+    end if
+    call fvm_check(fvm_snapshot_begin(ctx, nvars, vars, sid), "snapshot")
+    write(fname, '(a, "_", i6.6, ".fvmsnap")') trim(snapshot_prefix), step
+    call fvm_check(fvm_snapshot_save(ctx, sid, trim(fname) // c_null_char), "snapshot save")
+    nsnap = nsnap + 1
+    snaps(nsnap) = sid
+    time_out = time_out + output_time
+  end subroutine write_output
 
 end program fvm_driver

@@ -34,6 +34,8 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PER_BLOCK = 0, FVM_LAUNCH_BATCHED = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PERSISTENT = 2
+!...Initial states computed on the device (fvm_block_init)
+  integer(c_int), parameter, public :: FVM_IC_REGIONS = 1, FVM_IC_VORTEX = 2, FVM_IC_TGV = 3
 !...Snapshot variables: the reference's output variables
 !   (src/defs/type_defs_variables.f90:25-30) and the conserved components
   integer(c_int), parameter, public :: FVM_VAR_DENSITY = 201, FVM_VAR_VELOCITY_X = 202
@@ -44,6 +46,7 @@ module fvm_gpu
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc, fvm_block_set_bc_state
+  public :: fvm_block_init
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
@@ -147,6 +150,17 @@ module fvm_gpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function fvm_sync
+
+    ! params: FVM_IC_REGIONS 11 values per region (xmin, xmax, ymin, ymax, zmin,
+    ! zmax, rho, u, v, w, p); origin: c_loc of 3 reals or c_null_ptr
+    function fvm_block_init(blk, kind, nparams, params, origin) bind(C, name="fvm_block_init") &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: blk, origin
+      integer(c_int), value :: kind, nparams
+      real(c_double), intent(in) :: params(*)
+      integer(c_int) :: rc
+    end function fvm_block_init
 
     ! prim = (rho, u, v, w, p) of an inflow / farfield face
     function fvm_block_set_bc_state(blk, face, kind, prim) bind(C, name="fvm_block_set_bc_state") &

@@ -23,6 +23,7 @@ SIMPLE, MUSCL, WENO = 200, 201, 202
 LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
+IC_REGIONS, IC_VORTEX, IC_TGV = 1, 2, 3
 # the reference's BC codes (src/defs/type_defs_bc.f90:11-30)
 BC_INFLOW, BC_INFLOW_SUBSONIC, BC_INFLOW_SUPERSONIC = 10, 11, 12
 BC_OUTFLOW, BC_OUTFLOW_SUBSONIC, BC_OUTFLOW_SUPERSONIC = 20, 21, 22
@@ -54,7 +55,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
-           "fvm_block_set_bc_state")
+           "fvm_block_set_bc_state", "fvm_block_init")
 
 
 class FvmError(RuntimeError):
@@ -118,6 +119,7 @@ def load_library(path=LIB_PATH):
         "fvm_snapshot_release": ([vp, i64], ci),
         "fvm_restart_load": ([vp, ctypes.c_char_p], ci),
         "fvm_block_set_bc_state": ([vp, ci, ci, ctypes.POINTER(cd)], ci),
+        "fvm_block_init": ([vp, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(cd)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
@@ -192,6 +194,12 @@ class Block:
         if q.shape != self.host_shape(ng):
             raise FvmError(f"upload shape {q.shape} != {self.host_shape(ng)}")
         _check(_lib.fvm_upload(self._h, q.ctypes.data, ng))
+
+    def init(self, kind, params, origin=None):
+        """Initial state computed on the device (IC_REGIONS / IC_VORTEX / IC_TGV)."""
+        pv = (ctypes.c_double * max(1, len(params)))(*(float(x) for x in params))
+        ov = None if origin is None else (ctypes.c_double * 3)(*(float(x) for x in origin))
+        _check(_lib.fvm_block_init(self._h, kind, len(params), pv, ov))
 
     def launch_shape(self):
         tx, ty, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -333,6 +341,15 @@ class Context:
         ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         _check(_lib.fvm_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cu)))
         return ms.value, n.value, cu.value
+
+
+def region_params(regions, zrange=(-1e300, 1e300)):
+    """[((xmin, xmax), (ymin, ymax), (rho, u, v, w, p)), ...] (2-D regions as
+    cases.REF_QUADRANTS spells them) -> the flat IC_REGIONS parameter list."""
+    out = []
+    for (xa, xb), (ya, yb), st in regions:
+        out += [xa, xb, ya, yb, zrange[0], zrange[1], *st]
+    return out
 
 
 def read_snapshot_file(path):

@@ -198,6 +198,30 @@ int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 /* Host <-> device copies of the conserved state (host layout above).  Upload
  * also fills the device ghost layers (ClassFlowField%initial's ghost update,
  * src/solve/class_flow_field.f90:249-253). */
+/* Initial state computed on the device, with no host array: the block's
+ * u^n at vertices x = origin[0] + i dx, y = origin[1] + j dy, z = origin[2]
+ * + k dz (i, j, k from 0; the reference's grid is x = dx (i - 1),
+ * src/base/method_gridgen.f90:120-131, i.e. origin 0), converted p -> q as
+ * the reference does (src/solve/class_flow_field.f90:353-377) with the
+ * context's gamma (fvm_scheme_set first).  origin may be NULL (0, 0, 0).
+ *  FVM_IC_REGIONS: the reference's init_value (src/base/defs_block_data.f90:
+ *    166-201): per region xmin, xmax, ymin, ymax, zmin, zmax, rho, u, v, w, p
+ *    (11 values, up to 16 regions), applied in order with inclusive bounds,
+ *    so a later region wins on shared lines; vertices in no region get zero
+ *    (cover the block).  The reference's four-quadrant case is its regions
+ *    1-4 (src/solve/class_flow_field.f90:207-243).
+ *  FVM_IC_VORTEX: Yee's isentropic vortex, free stream rho = p = 1;
+ *    params length (period in x and y), beta, xc, yc, u_inf, v_inf, w_inf.
+ *  FVM_IC_TGV: the Taylor-Green vortex on [0, 2 pi)^3, params Mach number:
+ *    u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+ *    p = R + (cos 2x + cos 2y)(cos 2z + 2)/16, rho = p / R, R = 1/(gamma Ma^2).
+ * (No reference counterpart for the vortex and Taylor-Green states.) */
+#define FVM_IC_REGIONS  1
+#define FVM_IC_VORTEX   2
+#define FVM_IC_TGV      3
+int fvm_block_init(fvm_block *blk, int kind, int nparams, const double *params,
+                   const double origin[3]);
+
 int fvm_upload(fvm_block *blk, const double *q, int host_ng);
 int fvm_download(fvm_block *blk, double *q, int host_ng);
 

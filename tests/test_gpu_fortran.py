@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+import fvmgpu
 from fvmgpu import cases
 
 pytestmark = pytest.mark.gpu
@@ -38,3 +39,33 @@ def test_fortran_driver_riemann2d(tmp_path):
     assert np.array_equal(q[:, 0, 3:-3, 3:-3], o["q"][:, 0, 3:-3, 3:-3])
     # ghost faces hold the zeroth-order extrapolation
     np.testing.assert_array_equal(q[:, 0, 3:-3, 0], q[:, 0, 3:-3, 3])
+
+
+def test_fortran_driver_output_snapshots(tmp_path):
+    """&Output variables at the output times of &Time_Stepping (the reference's
+    write_flowfield, src/solve/class_output.f90:204-237): the initial field and
+    every output_time, each equal to the oracle's state at that step."""
+    prefix = tmp_path / "flow"
+    text = open(CASE).read()
+    text = text.replace('"gpurun_out/fortran_riemann2d_101.bin"', '""')
+    text = text.replace("output_time  = 0.01", "output_time  = 1.0E-3")
+    text = text.replace("print_every = 5", f'print_every = 5\n  snapshot_prefix = "{prefix}"')
+    nml = tmp_path / "case.nml"
+    nml.write_text(text)
+    print("case fortran driver snapshots every 5 steps", flush=True)
+    r = subprocess.run([DRIVER, f"--input={nml}"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    p, dx = cases.riemann2d(101, seed=0)
+    for step in (0, 5, 10, 15, 20):
+        f = fvmgpu.read_snapshot_file(f"{prefix}_{step:06d}.fvmsnap")
+        assert f["step"] == step and f["codes"] == [201, 202, 203, 301]
+        (n, h, arr), = f["blocks"]
+        assert n == (101, 101, 1)
+        if step:
+            o = O.oracle_solve(p, 2, (dx, dx, 1.0), dt=2e-4, nsteps=step, p_from_q=1)
+            q = o["q"][:, :1, 3:-3, 3:-3]
+        else:
+            q = cases.to_conserved(p)
+        pv = np.moveaxis(cases.to_primitive(q), -1, 0)
+        assert np.array_equal(arr, pv[[0, 1, 2, 4]])
