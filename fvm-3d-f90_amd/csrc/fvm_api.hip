@@ -1355,6 +1355,17 @@ int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
   return FVM_OK;
 }
 
+int fvm_cut_uniform(int64_t range_min, int64_t range_max, int ncut, int64_t* pos) {
+  if (!pos) return fail(FVM_EINVAL, "null position array");
+  if (ncut < 1 || ncut > range_max - range_min)
+    return fail(FVM_EINVAL, "ncut %d out of range 1..%lld cells", ncut, (long long)(range_max - range_min));
+  // each cut takes the remaining cells divided by the remaining pieces
+  pos[0] = range_min;
+  for (int c = 1; c < ncut; ++c) pos[c] = pos[c - 1] + (range_max - pos[c - 1]) / (ncut - c + 1);
+  pos[ncut] = range_max;
+  return FVM_OK;
+}
+
 int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t range_max[3],
                  fvm_block* donor, const int64_t donor_range_min[3], const int64_t donor_range_max[3],
                  const int donor_type[3]) {

@@ -55,7 +55,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
-           "fvm_block_set_bc_state", "fvm_block_init")
+           "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform")
 
 
 class FvmError(RuntimeError):
@@ -120,6 +120,7 @@ def load_library(path=LIB_PATH):
         "fvm_restart_load": ([vp, ctypes.c_char_p], ci),
         "fvm_block_set_bc_state": ([vp, ci, ci, ctypes.POINTER(cd)], ci),
         "fvm_block_init": ([vp, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(cd)], ci),
+        "fvm_cut_uniform": ([i64, i64, ci, ctypes.POINTER(i64)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
@@ -255,6 +256,35 @@ class Context:
         self.blocks.append(b)
         return b
 
+    def cut_blocks(self, q, spacing, ncut, periodic=(False, False, False)):
+        """The conserved state q (5, nk, nj, ni) as ncut = (ci, cj, ck) blocks
+        of a uniform cut (fvm_cut_uniform), uploaded and connected across their
+        shared planes; periodic directions wrap across the outer faces.
+        Returns {(a, b, c): (block, (i0, j0, k0))}."""
+        nk, nj, ni = q.shape[1:]
+        ncut = tuple(ncut) + (1,) * (3 - len(ncut))
+        ranges = [cut_uniform(n, c) for n, c in zip((ni, nj, nk), ncut)]
+        blocks = {}
+        for a, (x0, x1) in enumerate(ranges[0]):
+            for b, (y0, y1) in enumerate(ranges[1]):
+                for c, (z0, z1) in enumerate(ranges[2]):
+                    blk = self.block((x1 - x0 + 1, y1 - y0 + 1, z1 - z0 + 1), spacing)
+                    blk.upload(q[:, z0:z1 + 1, y0:y1 + 1, x0:x1 + 1])
+                    blocks[(a, b, c)] = (blk, (x0, y0, z0))
+        for idx, (blk, _) in blocks.items():
+            for d in range(3):
+                nb = list(idx)
+                if idx[d] + 1 < ncut[d]:
+                    nb[d] += 1
+                    blk.connect(2 * d + 1, blocks[tuple(nb)][0], shared=True)
+                elif periodic[d] and (d < 2 or nk > 1):
+                    nb[d] = 0
+                    if ncut[d] == 1:
+                        blk.set_bc(2 * d, BC_PERIODIC)
+                    else:
+                        blk.connect(2 * d + 1, blocks[tuple(nb)][0], shared=False)
+        return blocks
+
     def launch_mode(self, mode):
         """LAUNCH_BATCHED (default), LAUNCH_PERSISTENT or LAUNCH_PER_BLOCK."""
         _check(_lib.fvm_set_launch_mode(self._h, mode))
@@ -341,6 +371,15 @@ class Context:
         ms, n, cu = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         _check(_lib.fvm_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cu)))
         return ms.value, n.value, cu.value
+
+
+def cut_uniform(n, ncut):
+    """Vertex ranges (0-based, inclusive) of a uniform cut of n vertices into
+    ncut pieces sharing their interface planes (fvm_cut_uniform)."""
+    load_library()
+    pos = (ctypes.c_int64 * (ncut + 1))()
+    _check(_lib.fvm_cut_uniform(1, int(n), int(ncut), pos))
+    return [(pos[c] - 1, pos[c + 1] - 1) for c in range(ncut)]
 
 
 def region_params(regions, zrange=(-1e300, 1e300)):

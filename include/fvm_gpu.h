@@ -183,6 +183,15 @@ int fvm_conn_add(fvm_block *recv, const int64_t range_min[3], const int64_t rang
                  fvm_block *donor, const int64_t donor_range_min[3],
                  const int64_t donor_range_max[3], const int donor_type[3]);
 
+/* Cut positions of a uniform block cut along one direction
+ * (ClassGeomBlock%cut_uniform, src/base/defs_block_geom.f90:757-769): for
+ * vertices range_min..range_max and ncut pieces, pos[0] = range_min,
+ * pos[c] = pos[c-1] + (range_max - pos[c-1]) / (ncut - c + 1) (integer
+ * division), pos[ncut] = range_max; piece c spans vertices pos[c]..pos[c+1],
+ * so neighbours share their interface plane (connect them with shared = 1).
+ * Requires 1 <= ncut <= range_max - range_min.  Host only (no device). */
+int fvm_cut_uniform(int64_t range_min, int64_t range_max, int ncut, int64_t *pos);
+
 /* How a stage is launched over the blocks of a context (no reference
  * counterpart):
  *   FVM_LAUNCH_BATCHED (default): one launch per stage over the tile list of

@@ -175,3 +175,31 @@ def test_independent_cases_2d_batched_equal_alone(recon, limiter, flux):
             b.upload(cases.to_conserved(p))
             ctx.step(20, dt=1e-4)
             assert np.array_equal(b.download(), q), (ni, nj)
+
+
+@pytest.mark.parametrize("ncut,periodic", [((3, 2, 2), (True, True, True)),
+                                           ((2, 3, 1), (False, False, False))])
+def test_cut_uniform_equals_single_block(ncut, periodic):
+    """The reference's uniform block cut (shared planes, connected faces)
+    reproduces the single block bit for bit."""
+    if periodic[0]:
+        p, h, _ = cases.isentropic_vortex(30, seed=4)
+        sp = (h, h, h)
+        bc = (O.BC_PERIODIC,) * 6
+    else:
+        p = cases.smooth_3d((17, 25, 33), seed=12)
+        sp = (1 / 32, 1 / 24, 1 / 16)
+        bc = None
+    scheme = dict(recon=O.MUSCL, limiter=O.LIM_VANLEER, flux=O.ROE, rk=O.RK_SSP3)
+    print(f"case cut_uniform {ncut} periodic={periodic}", flush=True)
+    with fvmgpu.Context() as ctx:
+        ctx.scheme(**scheme)
+        ctx.numerics(0)
+        blocks = ctx.cut_blocks(cases.to_conserved(p), sp, ncut, periodic=periodic)
+        ctx.step(4, dt=2e-3)
+        out = np.zeros((5,) + p.shape[:3])
+        for blk, (x0, y0, z0) in blocks.values():
+            q = blk.download()
+            out[:, z0:z0 + q.shape[1], y0:y0 + q.shape[2], x0:x0 + q.shape[3]] = q
+    qo, _ = run_oracle(p, sp, 4, dt=2e-3, bc=bc, **scheme)
+    assert np.array_equal(out, qo)
