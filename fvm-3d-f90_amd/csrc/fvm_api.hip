@@ -338,6 +338,7 @@ __global__ __launch_bounds__(256) void snapshot_kernel(const double* __restrict_
 
 // ---- initial states computed on the device ---------------------------------
 constexpr int kInitMaxRegions = 16;
+static_assert(11 * kInitMaxRegions == FVM_IC_MAX_PARAMS, "region parameter capacity");
 struct InitSpec {
   int kind, nregions;
   double origin[3];
@@ -1816,47 +1817,67 @@ int fvm_sync(fvm_ctx* ctx) {
 
 extern "C" {
 
-int fvm_block_init(fvm_block* b, int kind, int nparams, const double* params, const double origin[3]) {
-  if (!b) return fail(FVM_EINVAL, "null block");
-  fvm_ctx* ctx = b->ctx;
-  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called (gamma)");
+}  // extern "C"
+
+namespace {
+
+// Validate an initial-state spec for a block of dimension `dim` and fill `s`.
+int make_init_spec(int dim, int kind, int nparams, const double* params, const double* origin,
+                   double gamma, InitSpec* s) {
   if (nparams < 0 || (nparams > 0 && !params)) return fail(FVM_EINVAL, "null parameters");
-  InitSpec s{};
-  s.kind = kind;
-  s.gamma = ctx->P.gamma;
-  for (int e = 0; e < 3; ++e) s.origin[e] = origin ? origin[e] : 0.0;
+  *s = InitSpec{};
+  s->kind = kind;
+  s->gamma = gamma;
+  for (int e = 0; e < 3; ++e) s->origin[e] = origin ? origin[e] : 0.0;
   for (int e = 0; e < 3; ++e)
-    if (!std::isfinite(s.origin[e])) return fail(FVM_EINVAL, "non-finite origin");
+    if (!std::isfinite(s->origin[e])) return fail(FVM_EINVAL, "non-finite origin");
   for (int c = 0; c < nparams; ++c)
     if (std::isnan(params[c])) return fail(FVM_EINVAL, "NaN parameter %d", c);
   switch (kind) {
     case FVM_IC_REGIONS:
       if (nparams < 11 || nparams % 11 || nparams / 11 > kInitMaxRegions)
         return fail(FVM_EINVAL, "regions: 11 parameters per region, 1..%d regions", kInitMaxRegions);
-      s.nregions = nparams / 11;
-      for (int r = 0; r < s.nregions; ++r) {
+      s->nregions = nparams / 11;
+      for (int r = 0; r < s->nregions; ++r) {
         const double* st = params + 11 * r + 6;
         if (!(st[0] > 0.0) || !(st[4] > 0.0)) return fail(FVM_EINVAL, "region %d: rho and p must be > 0", r);
-        if (b->dim == 2 && st[3] != 0.0) return fail(FVM_EINVAL, "2-D block: region %d has w != 0", r);
+        if (dim == 2 && st[3] != 0.0) return fail(FVM_EINVAL, "2-D block: region %d has w != 0", r);
       }
       break;
     case FVM_IC_VORTEX:
       if (nparams != 7) return fail(FVM_EINVAL, "vortex: 7 parameters (length, beta, xc, yc, u, v, w)");
       if (!(params[0] > 0.0) || !std::isfinite(params[0])) return fail(FVM_EINVAL, "vortex: length must be > 0");
-      if (b->dim == 2 && params[6] != 0.0) return fail(FVM_EINVAL, "2-D block: w must be 0");
+      if (dim == 2 && params[6] != 0.0) return fail(FVM_EINVAL, "2-D block: w must be 0");
       break;
     case FVM_IC_TGV:
       if (nparams != 1 || !(params[0] > 0.0)) return fail(FVM_EINVAL, "Taylor-Green: 1 parameter, Mach > 0");
-      if (b->dim == 2) return fail(FVM_EINVAL, "Taylor-Green needs a 3-D block");
+      if (dim == 2) return fail(FVM_EINVAL, "Taylor-Green needs a 3-D block");
       break;
     default:
       return fail(FVM_EINVAL, "unknown initial state kind %d", kind);
   }
-  for (int c = 0; c < nparams; ++c) s.v[c] = params[c];
-  if (int rc = set_device(ctx)) return rc;
+  for (int c = 0; c < nparams; ++c) s->v[c] = params[c];
+  return FVM_OK;
+}
+
+void launch_init(const fvm_block* b, const InitSpec& s, hipStream_t stream) {
   const int64_t ncell = (int64_t)b->g.nx * b->g.ny * b->g.nz;
   const unsigned grid = (unsigned)std::min<int64_t>(2048, (ncell + 255) / 256);
-  hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(256), 0, ctx->stream, b->q[0], b->g, b->nc, s);
+  hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(256), 0, stream, b->q[0], b->g, b->nc, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_block_init(fvm_block* b, int kind, int nparams, const double* params, const double origin[3]) {
+  if (!b) return fail(FVM_EINVAL, "null block");
+  fvm_ctx* ctx = b->ctx;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called (gamma)");
+  InitSpec s;
+  if (int rc = make_init_spec(b->dim, kind, nparams, params, origin, ctx->P.gamma, &s)) return rc;
+  if (int rc = set_device(ctx)) return rc;
+  launch_init(b, s, ctx->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   b->uploaded = true;
@@ -2369,9 +2390,15 @@ int server_admit(fvm_server* s) {
       if (r.bc[f] == FVM_BC_PERIODIC)
         if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
     // the compact state goes to the stage array q[1] (unused until the first
-    // step) and a kernel scatters it into q[0]; the host copies nothing
+    // step) and a kernel scatters it into q[0]; the host copies nothing.  An
+    // initial-state spec is computed into q[0] instead.
     const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
-    if (b->dim == 3) {
+    if (r.ic_kind != 0) {
+      InitSpec spec;
+      if (int rc = make_init_spec(b->dim, r.ic_kind, r.ic_nparams, r.ic_params, r.ic_origin, r.gamma, &spec))
+        return rc;
+      launch_init(b, spec, s->cstream);
+    } else if (b->dim == 3) {
       HIP_TRY(hipMemcpyAsync(b->q[1], j->q0, 5 * ncell * sizeof(double), hipMemcpyHostToDevice,
                              s->cstream));
     } else {   // 2-D: rho, rho u, rho v, E (rho w is zero)
@@ -2380,8 +2407,9 @@ int server_admit(fvm_server* s) {
       HIP_TRY(hipMemcpyAsync(b->q[1] + 3 * ncell, j->q0 + 4 * ncell, ncell * sizeof(double),
                              hipMemcpyHostToDevice, s->cstream));
     }
-    hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[1], b->q[0], b->g,
-                       b->nc, 1);
+    if (r.ic_kind == 0)
+      hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[1], b->q[0], b->g,
+                         b->nc, 1);
     HIP_TRY(hipGetLastError());
     b->uploaded = true;
     const int slot = s->free_slots.back();
@@ -2641,21 +2669,30 @@ int fvm_server_destroy(fvm_server* s) {
 }
 
 int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
-  if (!s || !r || !r->q0) return fail(FVM_EINVAL, "null server, request or initial state");
+  if (!s || !r) return fail(FVM_EINVAL, "null server or request");
+  if (!r->q0 && r->ic_kind == 0) return fail(FVM_EINVAL, "null initial state");
   if (r->n[0] < 4 || r->n[1] < 4 || r->n[2] < 1) return fail(FVM_EINVAL, "bad grid size");
   if (!(r->t_end > 0.0) || !(r->cfl > 0.0)) return fail(FVM_EINVAL, "t_end and cfl must be positive");
   const int dim = r->n[2] == 1 ? 2 : 3;
   const int nc = dim == 3 ? 5 : 4;
   const double cs = (double)(r->n[0] + 6) * (r->n[1] + 6) * (dim == 3 ? r->n[2] + 6 : 1);
   const size_t ncell = (size_t)r->n[0] * r->n[1] * r->n[2];
-  if (dim == 2)
+  if (r->ic_kind != 0) {
+    InitSpec spec;
+    if (r->ic_nparams > FVM_IC_MAX_PARAMS) return fail(FVM_EINVAL, "ic_nparams above %d", FVM_IC_MAX_PARAMS);
+    if (int rc = make_init_spec(dim, r->ic_kind, r->ic_nparams, r->ic_params, r->ic_origin, r->gamma, &spec))
+      return rc;
+  } else if (dim == 2) {
     for (size_t e = 0; e < ncell; ++e)
       if (r->q0[3 * ncell + e] != 0.0) return fail(FVM_EINVAL, "2-D request: rho*w must be zero");
+  }
   Job* j = new Job();
   j->id = s->next_id++;
   j->req = *r;
   j->req.q0 = nullptr;
-  if (r->borrow) {
+  if (r->ic_kind != 0) {   // computed on the device at admission: nothing to stage
+    j->staged.store(1, std::memory_order_release);
+  } else if (r->borrow) {
     j->src = r->q0;
   } else {
     j->q0 = pin_get(s, 5 * ncell, &j->q0_cap);
@@ -2674,7 +2711,7 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   j->t_submit = now_s(s);
   s->jobs[j->id] = j;
   s->pending.push_back(j);
-  if (r->borrow) {
+  if (r->borrow && r->ic_kind == 0) {
     {
       std::lock_guard<std::mutex> lk(s->st_mu);
       s->to_stage.push_back(j);

@@ -24,6 +24,7 @@ LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
 IC_REGIONS, IC_VORTEX, IC_TGV = 1, 2, 3
+IC_MAX_PARAMS = 176
 # the reference's BC codes (src/defs/type_defs_bc.f90:11-30)
 BC_INFLOW, BC_INFLOW_SUBSONIC, BC_INFLOW_SUPERSONIC = 10, 11, 12
 BC_OUTFLOW, BC_OUTFLOW_SUBSONIC, BC_OUTFLOW_SUPERSONIC = 20, 21, 22
@@ -70,7 +71,9 @@ class Request(ctypes.Structure):
                 ("entropy_fix", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("ref_compat", ctypes.c_int), ("numerics", ctypes.c_int),
                 ("bc", ctypes.c_int * 6), ("t_end", ctypes.c_double), ("cfl", ctypes.c_double),
-                ("q0", ctypes.c_void_p), ("borrow", ctypes.c_int)]
+                ("q0", ctypes.c_void_p), ("borrow", ctypes.c_int),
+                ("ic_kind", ctypes.c_int), ("ic_nparams", ctypes.c_int),
+                ("ic_params", ctypes.c_double * IC_MAX_PARAMS), ("ic_origin", ctypes.c_double * 3)]
 
 
 _lib = None
@@ -441,23 +444,40 @@ class Server:
 
     def submit(self, q0, spacing, t_end, cfl, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE,
                rk=RK_SSP3, kappa=0.33333, entropy_fix=0.2, gamma=1.4, ref_compat=1,
-               numerics=NUMERICS_EXACT, periodic=(False, False, False), borrow=True):
+               numerics=NUMERICS_EXACT, periodic=(False, False, False), borrow=True, ic=None):
         """q0: (5, nk, nj, ni) conserved state.  Returns the request id.
         borrow: the server's staging thread copies q0 after submit returns
         (the array is held here until the result is taken); otherwise submit
-        copies it."""
-        q0 = np.ascontiguousarray(q0, dtype=np.float64)
-        _, nk, nj, ni = q0.shape
+        copies it.  ic = (shape (ni, nj, nk), kind, params[, origin]): the
+        state is computed on the device at admission instead (q0 is None)."""
+        ic_kind, ic_np = 0, 0
+        ic_params = (ctypes.c_double * IC_MAX_PARAMS)()
+        ic_origin = (ctypes.c_double * 3)()
+        if ic is not None:
+            shape, ic_kind, params = ic[0], ic[1], list(ic[2])
+            if len(params) > IC_MAX_PARAMS:
+                raise FvmError("too many initial-state parameters")
+            ic_np = len(params)
+            for k, v in enumerate(params):
+                ic_params[k] = float(v)
+            for k, v in enumerate(ic[3] if len(ic) > 3 else (0.0, 0.0, 0.0)):
+                ic_origin[k] = float(v)
+            ni, nj, nk = (int(x) for x in shape)
+            ptr, borrow = None, False
+        else:
+            q0 = np.ascontiguousarray(q0, dtype=np.float64)
+            _, nk, nj, ni = q0.shape
+            ptr = q0.ctypes.data
         bc = [0] * 6
         for d in range(3):
             if periodic[d]:
                 bc[2 * d] = bc[2 * d + 1] = BC_PERIODIC
         r = Request((ni, nj, nk), tuple(float(x) for x in spacing), recon, limiter, flux, rk,
                     kappa, entropy_fix, gamma, ref_compat, numerics, tuple(bc), float(t_end),
-                    float(cfl), q0.ctypes.data, 1 if borrow else 0)
+                    float(cfl), ptr, 1 if borrow else 0, ic_kind, ic_np, ic_params, ic_origin)
         rid = ctypes.c_int64()
         _check(_lib.fvm_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
-        self.shapes[rid.value] = q0.shape
+        self.shapes[rid.value] = (5, nk, nj, ni)
         if borrow:
             self._hold[rid.value] = q0
         return rid.value

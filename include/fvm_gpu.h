@@ -228,6 +228,7 @@ int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 #define FVM_IC_REGIONS  1
 #define FVM_IC_VORTEX   2
 #define FVM_IC_TGV      3
+#define FVM_IC_MAX_PARAMS 176   /* 16 regions of 11 values */
 int fvm_block_init(fvm_block *blk, int kind, int nparams, const double *params,
                    const double origin[3]);
 
@@ -319,6 +320,13 @@ typedef struct fvm_request {
                                copied later by the server's staging thread; it
                                must stay valid and unchanged until the request
                                is done (fvm_server_poll counts it done)         */
+  int ic_kind;              /* 0: q0 holds the initial state.  FVM_IC_*: the
+                               state is computed on the device at admission
+                               (fvm_block_init's kinds and parameters, gamma of
+                               the request); q0 is not read                     */
+  int ic_nparams;
+  double ic_params[FVM_IC_MAX_PARAMS];
+  double ic_origin[3];
 } fvm_request;
 
 /* hbm_budget_bytes <= 0: 80 % of the free device memory. */

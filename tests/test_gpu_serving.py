@@ -158,3 +158,35 @@ def test_plain_end_time():
             assert steps == info["steps"] and t == info["t_final"]
             assert np.array_equal(b.download(), q)
             assert not nan and norms[5] > 0.0
+
+
+def test_device_initial_state_requests():
+    """Requests whose initial state is computed on the device at admission
+    (no host field crosses PCIe) equal the same solves submitted with that
+    state uploaded: the reference's quadrants (the host generator is bit for
+    bit the device regions) and a vortex (its state taken from a plain
+    context's device init, the same kernel)."""
+    p, dx = cases.riemann2d(41, seed=0)
+    quads = fvmgpu.region_params(cases.REF_QUADRANTS)
+    n = 20
+    _, h, centre = cases.isentropic_vortex(n, seed=4)
+    vparams = [10.0, 5.0, centre[0], centre[1], 1.0, 1.0, 0.0]
+    with fvmgpu.Context() as ctx:
+        ctx.scheme()
+        b = ctx.block((n, n, n), (h, h, h))
+        b.init(fvmgpu.IC_VORTEX, vparams)
+        qv = b.download()
+    two = dict(spacing=(dx, dx, 1.0), t_end=0.01, cfl=0.5, recon=O.MUSCL, limiter=O.LIM_MINMOD,
+               flux=O.ROE, rk=O.RK_SSP3)
+    vort = dict(spacing=(h, h, h), t_end=0.05, cfl=0.6, recon=O.WENO, limiter=O.LIM_NONE,
+                flux=O.AUSMPLUS, rk=O.RK_SSP3, ref_compat=0, periodic=(True, True, True))
+    print("case serving device initial states", flush=True)
+    dev = run([dict(q0=None, ic=((41, 41, 1), fvmgpu.IC_REGIONS, quads), **two),
+               dict(q0=None, ic=((n, n, n), fvmgpu.IC_VORTEX, vparams), **vort)])
+    host = run([dict(q0=cases.to_conserved(p), **two), dict(q0=qv, **vort)])
+    for (qd, idv), (qh, ih) in zip(dev, host):
+        assert idv["status"] == 2 and ih["status"] == 2
+        assert np.array_equal(qd, qh)
+    with fvmgpu.Server() as srv:
+        with pytest.raises(fvmgpu.FvmError):
+            srv.submit(None, (0.1, 0.1, 0.1), 0.1, 0.5, ic=((8, 8, 8), fvmgpu.IC_TGV, [-1.0]))
