@@ -209,8 +209,9 @@ def bench_vortex(args):
     traffic, flops = None, None
     tpath = os.path.join(ROOT, "profiles", "stage3d_muscl_roe_vortex128_traffic.json")
     if os.path.exists(tpath) and n == 128:
+        # counters of the tile shape this run launched (the autotuner's pick)
         with open(tpath) as f:
-            prof = json.load(f)
+            prof = json.load(f).get("shapes", {}).get(f"{tile[0]}x{tile[1]}", {})
         traffic = prof.get("hbm_bytes_per_launch")
         if args.numerics == "fast":
             flops = prof.get("fp64_flops_per_cell_update")
