@@ -182,6 +182,7 @@ def bench_vortex(args):
     if not args.no_tune:
         ctx.autotune(reps=3)
     tile = blk.launch_shape()
+    reg = blk.launch_tile() >= 4          # the register-reconstruction kernel
     if args.warmup:
         ctx.step(args.warmup, cfl=cfl)
     ctx.sync()
@@ -211,12 +212,13 @@ def bench_vortex(args):
     if os.path.exists(tpath) and n == 128:
         # counters of the tile shape this run launched (the autotuner's pick)
         with open(tpath) as f:
-            prof = json.load(f).get("shapes", {}).get(f"{tile[0]}x{tile[1]}", {})
+            prof = json.load(f).get("shapes", {}).get(f"{tile[0]}x{tile[1]}" + ("-reg" if reg else ""), {})
         traffic = prof.get("hbm_bytes_per_launch")
         if args.numerics == "fast":
             flops = prof.get("fp64_flops_per_cell_update")
     roof = roofline(stage_ms, launches, ncell, B_ALG["rk4_3d"], traffic,
-                    f"fvm::{args.numerics}::stage3d_kernel<ReconMuscl, 300, {tile[0]}, {tile[1]}>")
+                    f"fvm::{args.numerics}::{'stage3d_reg_kernel' if reg else 'stage3d_kernel'}"
+                    f"<ReconMuscl, 300, {tile[0]}, {tile[1]}>")
     if flops:
         # the co-limiting roof: counted FP64 VALU work against the vector FP64 peak
         tflops = flops * ncell / (roof["avg_launch_us"] * 1e-6) / 1e12

@@ -543,7 +543,7 @@ void plan_launch(fvm_block* b, int force_tile = -1, int force_chunk = -1) {
   } else {
     const int w64 = (g.nx + 63) / 64 * 64 - g.nx, w32 = (g.nx + 31) / 32 * 32 - g.nx;
     int tile = w32 <= w64 ? 1 : 0;
-    if (tile_env >= 0) tile = tile_env;
+    if (tile_env >= 0 && (!b->ctx || tile3d_ok(b->ctx->recon, tile_env))) tile = tile_env;
     b->tile = tile;
     const int tx = kTile3d[tile][0], ty = kTile3d[tile][1];
     b->tiles_x = (g.nx + tx - 1) / tx;
@@ -1154,6 +1154,9 @@ int fvm_scheme_set(fvm_ctx* ctx, int recon, int limiter, int flux, int rk, doubl
   ctx->P.limiter = limiter;
   ctx->P.weno_textbook = ref_compat ? 0 : 1;
   ctx->scheme_set = true;
+  // a launch shape the new reconstruction cannot use falls back to the default
+  for (fvm_block* b : ctx->blocks)
+    if (b->dim == 3 && !tile3d_ok(recon, b->tile)) plan_launch(b);
   ++ctx->gen;
   return FVM_OK;
 }
@@ -1739,6 +1742,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     for (int tile = 0; tile < ntiles; ++tile)
       for (int ci = 0; ci < 4; ++ci) {
         if (dim == 2 && tile > 0 && ci > 0) break;   // a 2-D tile does not march
+        if (dim == 3 && !tile3d_ok(ctx->recon, tile)) break;
         const int chunk = dim == 3 ? chunks3[ci] : chunks2[ci];
         for (fvm_block* b : sel) plan_launch(b, tile, chunk);
         if (int rc = sync_desc(ctx)) return rc;
@@ -1776,6 +1780,12 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     for (fvm_block* b : sel) plan_launch(b, best_tile, best_chunk);
   }
   if (int rc = sync_desc(ctx)) return rc;
+  return FVM_OK;
+}
+
+int fvm_block_launch_tile(fvm_block* b, int* tile_id) {
+  if (!b || !tile_id) return fail(FVM_EINVAL, "null block or output");
+  *tile_id = b->tile;
   return FVM_OK;
 }
 

@@ -42,6 +42,16 @@ struct Launcher {
     return hipGetLastError();
   }
 
+  template <int FLUX, int TX, int TY, bool CONN>
+  static hipError_t go3d_reg(const StageArgs& a, int grid, hipStream_t s) {
+    if constexpr (RC::RV <= 1) {
+      hipLaunchKernelGGL((stage3d_reg_kernel<RC, FLUX, TX, TY, CONN>), dim3(grid), dim3(TX * TY), 0, s, a);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
+
   template <int FLUX, bool CONN>
   static hipError_t go3d_tile(int tile, const StageArgs& a, int grid, hipStream_t s) {
     switch (tile) {
@@ -49,6 +59,8 @@ struct Launcher {
       case 1: return go3d<FLUX, kTile3d[1][0], kTile3d[1][1], CONN>(a, grid, s);
       case 2: return go3d<FLUX, kTile3d[2][0], kTile3d[2][1], CONN>(a, grid, s);
       case 3: return go3d<FLUX, kTile3d[3][0], kTile3d[3][1], CONN>(a, grid, s);
+      case 4: return go3d_reg<FLUX, kTile3d[4][0], kTile3d[4][1], CONN>(a, grid, s);
+      case 5: return go3d_reg<FLUX, kTile3d[5][0], kTile3d[5][1], CONN>(a, grid, s);
       default: return hipErrorInvalidValue;
     }
   }

@@ -9,9 +9,14 @@
 
 namespace fvm {
 
-// 3-D tile shapes the library is built with: id -> (TX, TY)
-constexpr int kNumTiles3d = 4;
-constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}, {16, 16}};
+// 3-D tile shapes the library is built with: id -> (TX, TY); ids 4 and 5
+// are the register-reconstruction kernel (stage3d_reg_kernel, simple and
+// MUSCL reconstructions only)
+constexpr int kNumTiles3d = 6;
+constexpr int kTile3d[kNumTiles3d][2] = {{64, 4}, {32, 8}, {64, 2}, {16, 16}, {32, 8}, {16, 16}};
+constexpr bool tile3d_reg(int tile) { return tile >= 4; }
+// the register-reconstruction kernel takes stencils of at most 3 vertices
+constexpr bool tile3d_ok(int recon, int tile) { return !tile3d_reg(tile) || recon != RECON_WENO; }
 // 2-D stage shapes: id 0 the row kernel (62 columns marching over rows),
 // then the tiled kernel's (TX, TY)
 constexpr int kNumTiles2d = 3;

@@ -1159,5 +1159,279 @@ void stage3d_kernel(StageArgs a) {
   }
 }
 
+
+// ============================================================================
+// 3-D stage, register-reconstruction layout (simple and MUSCL stencils): the
+// i/j vertex reconstructions never pass through LDS, so a workgroup holds the
+// plane and the two flux planes only (about half of stage3d_kernel's LDS).
+// Per plane k:
+//   A  as stage3d_kernel: plane k (owned vertex + i/j halo) into LDS as
+//      reconstruction variables, k-queue shift, next plane's loads in flight
+//   BC face tasks: the thread of an i-face (j-face) reads the 2 RV + 2 plane
+//      values across it, reconstructs the face's two vertices in registers
+//      and writes the flux; each owner also takes its k-face from its queue
+//   D  owner update as stage3d_kernel; norm partials summed per wave
+// Same per-vertex and per-face arithmetic as stage3d_kernel (results bit for
+// bit), about twice its i/j reconstruction work.
+// ============================================================================
+#ifndef FVM_WPE_REG
+#define FVM_WPE_REG 3
+#endif
+template <class RC, int FLUX, int TX, int TY, bool CONN>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FVM_WPE_REG)))
+void stage3d_reg_kernel(StageArgs a) {
+  constexpr int NC = 5, NT = TX * TY, NWAVE = NT / 64;
+  constexpr int RV = RC::RV, H = RC::H;
+  static_assert(RV <= 1, "register reconstruction takes the simple and MUSCL stencils");
+  constexpr int NW = RV + 2;                       // k-queue V(c - 1 .. c + RV)
+  constexpr int QOFF = 1;                          // queue index of the centre c
+  constexpr int VW = TX + 2 * H, VR = TY + 2 * H, VSN = VW * VR;
+  constexpr int FXW = TX + 1, FXN = TY * FXW, FYN = (TY + 1) * TX;
+  constexpr int NXH = 2 * H * TY, NYH = 2 * H * TX, NHALO = NXH + NYH;
+  constexpr int NHL = (NHALO + NT - 1) / NT;       // halo loads per thread
+  constexpr int FT = FXN + FYN;                    // i- then j-face tasks
+  constexpr int NWIN = 2 * RV + 2;                 // plane values across a face
+  __shared__ double Vs[NC * VSN];
+  __shared__ double Fx[NC * FXN];
+  __shared__ double Fy[NC * FYN];
+  __shared__ double red[NWAVE];
+  __shared__ double NrmW[NWAVE * NC];              // per-wave rhs^2 sums (norm stage)
+  __shared__ double Fk[NC * NT];                   // the owner's k-face flux k (own slot)
+
+  const Params& P = a.P;
+  const int t = threadIdx.x;
+  const int tx = t % TX, ty = t / TX;
+  const int wave = t / 64, lane = t % 64;
+  const bool want_smax = a.smax != nullptr || a.block_smax;
+
+  double dt_step = 0.0;
+  bool want_norm = false;
+  if (!stage_clock(a, dt_step, want_norm)) return;
+  const DescTable tab = as_const_table(a.blocks);
+  for (int it = blockIdx.x; it < a.ntiles; it += gridDim.x) {
+    const int tile = a.tile0 + xcd_remap(it, a.ntiles);
+    const int bi = block_of_tile(tab, a.nblocks, tile);
+    const auto& B = tab[bi];
+    if (B.halt && *B.halt) continue;   // uniform over the workgroup
+    const BlockGeom g = load_geom(B.g);
+    const double dt = a.cfl_dt ? dt_step : *B.dt_dev;
+    const double* __restrict__ rvp = B.rv;
+    const RKCoef kc = rk_coef(a.stage, dt);
+    double smax = 0.0;
+    const double* __restrict__ qin = B.q[a.in_buf];
+    const double* __restrict__ q0p = B.q[a.q0_buf];
+    double* __restrict__ qout = B.q[a.out_buf];
+    double* __restrict__ acc = a.use_acc ? B.q[3] : nullptr;
+    const int lt = tile - B.tile_base;
+    const int ti = lt % B.tiles_x;
+    const int tj = (lt / B.tiles_x) % B.tiles_y;
+    const int tk = lt / (B.tiles_x * B.tiles_y);
+    const int i0 = ti * TX, j0 = tj * TY;
+    const int k0 = tk * B.chunk, k1 = min(k0 + B.chunk, g.nz);
+    const int c = i0 + tx, r = j0 + ty;
+    const bool owner = c < g.nx && r < g.ny;
+    const int ci_ = bmap(c, g.nx, g.bc[0], g.bc[1], H), rj_ = bmap(r, g.ny, g.bc[2], g.bc[3], H);
+
+    auto gaddr = [&](int i, int j, int k) {
+      return gidx32(g, bmap(i, g.nx, g.bc[0], g.bc[1], H), bmap(j, g.ny, g.bc[2], g.bc[3], H),
+                    bmap(k, g.nz, g.bc[4], g.bc[5], H));
+    };
+    // halo slot e (0..NHALO-1) of a plane: global (i, j) and LDS offset
+    auto halo_pos = [&](int e, int& hi, int& hj, int& slot) {
+      if (e < NXH) {
+        const int row = e / (2 * H), s = e % (2 * H);
+        hi = s < H ? i0 - H + s : i0 + TX + (s - H);
+        hj = j0 + row;
+        slot = (row + H) * VW + (s < H ? s : TX + H + (s - H));
+      } else {
+        const int e2 = e - NXH;
+        const int col = e2 % TX, s = e2 / TX;
+        hi = i0 + col;
+        hj = s < H ? j0 - H + s : j0 + TY + (s - H);
+        slot = (s < H ? s : TY + H + (s - H)) * VW + col + H;
+      }
+    };
+
+    // ---- k-queue warm-up: vertices k0-1 and k0, face k0
+    double Vq[NW][NC];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      double raw[NC];
+      load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 - 1 - QOFF + w), raw);
+      to_recon_vars<RC, NC>(raw, Vq[w], P.gamma);
+    }
+    double Lc[NC], Lt[NC], Rt[NC];
+    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+    {
+      double raw[NC];
+#pragma unroll
+      for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+      load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 + RV), raw);
+      to_recon_vars<RC, NC>(raw, Vq[NW - 1], P.gamma);
+    }
+    RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+    {
+      double Fc[NC];
+      face_flux<FLUX, NC, 2>(Lc, Rt, P, Fc);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Fk[m * NT + t] = Fc[m];
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+
+    // ---- prefetch for plane k0
+    double oraw[NC];
+    load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k0 + 1 + RV), oraw);
+    double hraw[NHL][NC];
+    int hslot[NHL];
+#pragma unroll
+    for (int h = 0; h < NHL; ++h) {
+      const int e = t + h * NT;
+      hslot[h] = -1;
+      if (e < NHALO) {
+        int hi, hj;
+        halo_pos(e, hi, hj, hslot[h]);
+        load_raw<RC, NC>(qin, g.cs, gaddr(hi, hj, k0), hraw[h]);
+      }
+    }
+    if (want_norm && t < NWAVE * NC) NrmW[t] = 0.0;
+
+    for (int k = k0; k < k1; ++k) {
+      // ---- A: queue shift (centre k+1), plane k into LDS, next loads
+#pragma unroll
+      for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Vq[w][m] = Vq[w + 1][m];
+      to_recon_vars<RC, NC>(oraw, Vq[NW - 1], P.gamma);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Vs[m * VSN + (ty + H) * VW + tx + H] = Vq[QOFF - 1][m];
+#pragma unroll
+      for (int h = 0; h < NHL; ++h) {
+        if (hslot[h] >= 0) {
+          double v[NC];
+          to_recon_vars<RC, NC>(hraw[h], v, P.gamma);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Vs[m * VSN + hslot[h]] = v[m];
+        }
+      }
+      __syncthreads();
+
+      // ---- BC: face tasks.  i-face s of row y lies between vertices s - 1
+      // and s of the row (s = 0..TX); j-face s of column x likewise.
+      auto face_task = [&](int e) {
+        int base, stride, out;
+        const bool xdir = e < FXN;
+        if (xdir) {
+          const int y = e / FXW, s = e % FXW;
+          base = (y + H) * VW + s - 1 - RV + H; stride = 1; out = e;
+        } else {
+          const int e1 = e - FXN, x = e1 % TX, s = e1 / TX;
+          base = (s - 1 - RV + H) * VW + x + H; stride = VW; out = e1;
+        }
+        double W[NWIN][NC], La[NC], Ra[NC], Lb[NC], Rb[NC], F[NC];
+#pragma unroll
+        for (int w = 0; w < NWIN; ++w)
+#pragma unroll
+          for (int m = 0; m < NC; ++m) W[w][m] = Vs[m * VSN + base + w * stride];
+        RC::template vertex<NC>(&W[0], P, La, Ra);    // vertex s - 1
+        RC::template vertex<NC>(&W[1], P, Lb, Rb);    // vertex s
+        if (xdir) {
+          face_flux<FLUX, NC, 0>(La, Rb, P, F);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Fx[m * FXN + out] = F[m];
+        } else {
+          face_flux<FLUX, NC, 1>(La, Rb, P, F);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) Fy[m * FYN + out] = F[m];
+        }
+      };
+      for (int e = t; e < FT; e += NT) face_task(e);
+      // the next plane's loads, in flight behind the k-face and the update
+      // (not live across the face tasks, where registers are the limit)
+      if (k + 1 < k1) {
+        load_raw<RC, NC>(qin, g.cs, gaddr(ci_, rj_, k + 2 + RV), oraw);
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) {
+          const int e = t + h * NT;
+          if (e < NHALO) {
+            int hi, hj, sl;
+            halo_pos(e, hi, hj, sl);
+            load_raw<RC, NC>(qin, g.cs, gaddr(hi, hj, k + 1), hraw[h]);
+          }
+        }
+      }
+      // k-direction vertex k+1 and face k+1 of the owned column (registers)
+      double Fn[NC];
+      RC::template vertex<NC>(&Vq[QOFF - RV], P, Lt, Rt);
+      face_flux<FLUX, NC, 2>(Lc, Rt, P, Fn);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Lc[m] = Lt[m];
+      __syncthreads();
+
+      // ---- D: update
+      double rr[NC];
+#pragma unroll
+      for (int m = 0; m < NC; ++m) rr[m] = 0.0;
+      if (owner) {
+        const uint32_t cidx = gidx32(g, c, r, k);
+        // operands of the RK combine (loaded here: registers are the limit
+        // during the face tasks, and the other waves cover the latency)
+        double pq0[NC], px[NC], out[NC];
+        rk_prefetch<NC, 3>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
+          const double fyd = Fy[m * FYN + (ty + 1) * TX + tx] - Fy[m * FYN + ty * TX + tx];
+          const double fzd = Fn[m] - Fk[m * NT + t];
+          if constexpr (kFast) {
+            rr[m] = -(fxd * g.rdx) - fyd * g.rdy - fzd * g.rdz;
+          } else {
+            rr[m] = 0.0 - fxd / g.dx;
+            rr[m] = rr[m] - fyd / g.dy;
+            rr[m] = rr[m] - fzd / g.dz;
+          }
+        }
+        if (rvp) {
+#pragma unroll
+          for (int m = 1; m < NC; ++m) rr[m] = rr[m] + ld_comp(rvp, g.cs, m - 1, cidx);
+        }
+        rk_apply_all<NC>(a.stage, kc, pq0, px, rr, out, acc, g.cs, cidx);
+        store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
+        if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
+      }
+      if (want_norm) {   // uniform branch
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          double v = rr[m] * rr[m];
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+          if (lane == 0) NrmW[wave * NC + m] += v;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) Fk[m * NT + t] = Fn[m];
+      __syncthreads();
+    }
+
+    if (want_norm && t == 0) {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        double s = 0.0;
+        for (int w = 0; w < NWAVE; ++w) s += NrmW[w * NC + m];
+        a.norm_part[(int64_t)tile * NC + m] = s;
+      }
+    }
+    if (want_smax) {
+      const double s = wg_max<NT>(smax, red);
+      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
+      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+    }
+    __syncthreads();   // the next tile's plane and norm sums
+  }
+}
+
 }  // namespace exact / fast
 }  // namespace fvm

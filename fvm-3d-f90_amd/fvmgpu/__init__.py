@@ -56,7 +56,8 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_server_result", "fvm_set_time", "fvm_time", "fvm_viscous_set",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
-           "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform")
+           "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform",
+           "fvm_block_launch_tile")
 
 
 class FvmError(RuntimeError):
@@ -124,6 +125,7 @@ def load_library(path=LIB_PATH):
         "fvm_block_set_bc_state": ([vp, ci, ci, ctypes.POINTER(cd)], ci),
         "fvm_block_init": ([vp, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(cd)], ci),
         "fvm_cut_uniform": ([i64, i64, ci, ctypes.POINTER(i64)], ci),
+        "fvm_block_launch_tile": ([vp, ctypes.POINTER(ci)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
         "fvm_server_destroy": ([vp], ci),
         "fvm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(i64)], ci),
@@ -204,6 +206,12 @@ class Block:
         pv = (ctypes.c_double * max(1, len(params)))(*(float(x) for x in params))
         ov = None if origin is None else (ctypes.c_double * 3)(*(float(x) for x in origin))
         _check(_lib.fvm_block_init(self._h, kind, len(params), pv, ov))
+
+    def launch_tile(self):
+        """Launch-shape id (the FVM_TILE / FVM_TILE2 value) of this block."""
+        v = ctypes.c_int()
+        _check(_lib.fvm_block_launch_tile(self._h, ctypes.byref(v)))
+        return v.value
 
     def launch_shape(self):
         tx, ty, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

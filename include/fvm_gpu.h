@@ -284,6 +284,11 @@ int fvm_norms(fvm_ctx *ctx, double out[6], int *nan_flag);
  * Call after fvm_upload.  fvm_block_launch_shape reports the choice. */
 int fvm_autotune(fvm_ctx *ctx, int reps);
 int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk);
+/* The launch-shape id behind that shape (the FVM_TILE / FVM_TILE2 values):
+ * 3-D ids 0-3 the plane-staged stage kernel at 64x4, 32x8, 64x2, 16x16; ids
+ * 4-5 the register-reconstruction kernel at 32x8, 16x16 (simple and MUSCL
+ * reconstructions); 2-D id 0 the row kernel, 1-2 the tiled kernel. */
+int fvm_block_launch_tile(fvm_block *blk, int *tile_id);
 
 /* Stage-kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only wall-clocks the whole solve,

@@ -17,8 +17,11 @@ COMBOS = [(O.SIMPLE, O.LIM_NONE, O.ROE), (O.MUSCL, O.LIM_MINMOD, O.ROE),
           (O.WENO, O.LIM_NONE, O.AUSM), (O.MUSCL, O.LIM_VANALBADA, O.AUSMPLUS)]
 
 
+@pytest.mark.parametrize("tile", ["", "4"])
 @pytest.mark.parametrize("recon,lim,flux", COMBOS)
-def test_fast_one_step_3d(recon, lim, flux):
+def test_fast_one_step_3d(recon, lim, flux, tile, monkeypatch):
+    if tile:   # the register-reconstruction kernel (WENO falls back to the default)
+        monkeypatch.setenv("FVM_TILE", tile)
     p = cases.smooth_3d((12, 14, 40), seed=5)
     sp = (1 / 39, 1 / 13, 1 / 11)
     print(f"case fast 1-step 3d recon={recon} lim={lim} flux={flux}", flush=True)

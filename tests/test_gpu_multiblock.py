@@ -58,13 +58,16 @@ def run_blocks(p, spacing, ranges, nsteps, dt, mode, periodic=(False, False, Fal
         return out, norms
 
 
-@pytest.mark.parametrize("mode", [fvmgpu.LAUNCH_BATCHED, fvmgpu.LAUNCH_PERSISTENT,
-                                  fvmgpu.LAUNCH_PER_BLOCK])
-def test_split_equals_single_block(mode):
+@pytest.mark.parametrize("mode,tile", [(fvmgpu.LAUNCH_BATCHED, ""), (fvmgpu.LAUNCH_PERSISTENT, ""),
+                                       (fvmgpu.LAUNCH_PER_BLOCK, ""), (fvmgpu.LAUNCH_BATCHED, "4"),
+                                       (fvmgpu.LAUNCH_PERSISTENT, "5")])
+def test_split_equals_single_block(mode, tile, monkeypatch):
     p = cases.smooth_3d((26, 30, 40), seed=11)
     sp = (1 / 39, 1 / 29, 1 / 25)
     ranges = (split_bounds(40, 2), split_bounds(30, 2), split_bounds(26, 2))
-    print(f"case split 2x2x2 of 40x30x26 mode={mode}", flush=True)
+    if tile:   # the register-reconstruction kernel's connected-face variant
+        monkeypatch.setenv("FVM_TILE", tile)
+    print(f"case split 2x2x2 of 40x30x26 mode={mode} tile={tile}", flush=True)
     scheme = dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
     q, _ = run_blocks(p, sp, ranges, 3, 1e-3, mode, **scheme)
     qo, _ = run_oracle(p, sp, 3, dt=1e-3, **scheme)

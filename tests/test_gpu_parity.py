@@ -48,11 +48,12 @@ def test_smooth3d_bitwise(recon, lim, flux):
     np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("tile", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("tile", ["0", "1", "2", "3", "4", "5"])
 @pytest.mark.parametrize("chunk", ["3", "8", "32"])
 def test_tile_shapes_bitwise(tile, chunk, monkeypatch):
     """Every compiled tile shape and k-chunk gives the same bits (partial
-    tiles in i, j and k included)."""
+    tiles in i, j and k included); shapes 4 and 5 are the
+    register-reconstruction kernel."""
     monkeypatch.setenv("FVM_TILE", tile)
     monkeypatch.setenv("FVM_CHUNK", chunk)
     p = cases.smooth_3d((19, 11, 67), seed=9)
@@ -115,3 +116,35 @@ def test_rk4_periodic_cfl_bitwise(recon, lim, flux):
     assert dt_last == r["dt"][-1]
     assert np.array_equal(qg, qo)
     np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12)
+
+
+@pytest.mark.parametrize("tile", ["4", "5"])
+@pytest.mark.parametrize("recon,lim,flux", [(O.SIMPLE, O.LIM_NONE, O.AUSM),
+                                            (O.MUSCL, O.LIM_MINMOD, O.VANLEER),
+                                            (O.MUSCL, O.LIM_NONE, O.AUSMPLUS),
+                                            (O.MUSCL, O.LIM_VANLEER, O.ROE)])
+def test_register_kernel_schemes_bitwise(tile, recon, lim, flux, monkeypatch):
+    """The register-reconstruction kernel with every flux and limiter kind,
+    CFL steps (its in-kernel wave-speed maximum and norms included)."""
+    monkeypatch.setenv("FVM_TILE", tile)
+    p = cases.smooth_3d((13, 21, 37), seed=17)
+    sp = (1 / 36, 1 / 20, 1 / 12)
+    print(f"case register kernel tile={tile} recon={recon} lim={lim} flux={flux}", flush=True)
+    with fvmgpu.Context() as ctx:
+        qg, ng, _, dt_last = run_gpu(ctx, p, sp, 3, cfl=0.6, recon=recon, limiter=lim, flux=flux)
+    qo, r = run_oracle(p, sp, 3, cfl=0.6, recon=recon, limiter=lim, flux=flux)
+    assert dt_last == r["dt"][-1]
+    assert np.array_equal(qg, qo)
+    np.testing.assert_allclose(ng, r["res"][-1], rtol=1e-12, atol=1e-300)
+
+
+def test_register_tile_refused_for_weno(monkeypatch):
+    """A WENO context asked for a register-reconstruction shape launches the
+    default shape (same bits as the oracle)."""
+    monkeypatch.setenv("FVM_TILE", "4")
+    p = cases.smooth_3d((10, 12, 20), seed=2)
+    sp = (1 / 19, 1 / 11, 1 / 9)
+    with fvmgpu.Context() as ctx:
+        qg, _, _, _ = run_gpu(ctx, p, sp, 2, dt=1e-3, recon=O.WENO, limiter=O.LIM_NONE, flux=O.ROE)
+    qo, _ = run_oracle(p, sp, 2, dt=1e-3, recon=O.WENO, limiter=O.LIM_NONE, flux=O.ROE)
+    assert np.array_equal(qg, qo)
