@@ -46,7 +46,7 @@ module fvm_gpu
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc, fvm_block_set_bc_state
-  public :: fvm_block_init
+  public :: fvm_block_init, fvm_cut_uniform, fvm_block_launch_tile
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
@@ -161,6 +161,23 @@ module fvm_gpu
       real(c_double), intent(in) :: params(*)
       integer(c_int) :: rc
     end function fvm_block_init
+
+    ! cut positions of ClassGeomBlock%cut_uniform along one direction: pos(0:ncut)
+    function fvm_cut_uniform(range_min, range_max, ncut, pos) bind(C, name="fvm_cut_uniform") &
+        result(rc)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: range_min, range_max
+      integer(c_int), value :: ncut
+      integer(c_int64_t), intent(out) :: pos(*)
+      integer(c_int) :: rc
+    end function fvm_cut_uniform
+
+    function fvm_block_launch_tile(blk, tile_id) bind(C, name="fvm_block_launch_tile") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: blk
+      integer(c_int), intent(out) :: tile_id
+      integer(c_int) :: rc
+    end function fvm_block_launch_tile
 
     ! prim = (rho, u, v, w, p) of an inflow / farfield face
     function fvm_block_set_bc_state(blk, face, kind, prim) bind(C, name="fvm_block_set_bc_state") &
