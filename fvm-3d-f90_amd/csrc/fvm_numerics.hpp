@@ -196,8 +196,9 @@ __device__ __forceinline__ void muscl_vertex_all(const double (*V)[NC], double k
       const double D_m = V[1][c] - V[0][c];
       const double D_p = V[2][c] - V[1][c];
       if constexpr (LIM == LIM_VANALBADA) {
-        num[c] = D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps);
-        den[c] = D_m * D_m + D_p * D_p + eps;
+        // D_m (D_p^2 + eps) + D_p (D_m^2 + eps) = (D_m + D_p)(D_m D_p + eps)
+        num[c] = (D_m + D_p) * (D_m * D_p + eps);
+        den[c] = D_m * D_m + (D_p * D_p + eps);
       } else {
         num[c] = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p;
         den[c] = fabs(D_m) + fabs(D_p) + eps;
@@ -326,11 +327,13 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   const double den = rho_l + rho_b;
   double u_b, v_b, w_b, H_b;
   if constexpr (kFast) {
-    const double rden = frcp(den);
-    u_b = (rho_l * u_l + rho_b * u_r) * rden;
-    v_b = (rho_l * v_l + rho_b * v_r) * rden;
-    w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) * rden : 0.0;
-    H_b = (rho_l * H_l + rho_b * H_r) * rden;
+    // (rho_l a_l + rho_b a_r) / (rho_l + rho_b) = a_l + wr (a_r - a_l),
+    // wr = rho_b / (rho_l + rho_b): one FMA per averaged quantity
+    const double wr = rho_b * frcp(den);
+    u_b = u_l + wr * du;
+    v_b = v_l + wr * dv;
+    w_b = (NC == 5) ? w_l + wr * dw : 0.0;
+    H_b = H_l + wr * (H_r - H_l);
   } else {
     u_b = (rho_l * u_l + rho_b * u_r) / den;
     v_b = (rho_l * v_l + rho_b * v_r) / den;
@@ -372,12 +375,39 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   }
   const double lr = lam2 * rho_b;
   // mass
-  F[0] = 0.5 * ((rho_l * Vc_l + rho_r * Vc_r) - (d1 + d2 + d5));
+  const double dsum = d1 + d2 + d5;
+  F[0] = 0.5 * ((rho_l * Vc_l + rho_r * Vc_r) - dsum);
   // momentum components; the normal one carries -/+ c and the pressure term
   const double ub[3] = {u_b, v_b, w_b};
   const double dd[3] = {du, dv, dw};
   const double vl[3] = {u_l, v_l, w_l};
   const double vr[3] = {u_r, v_r, w_r};
+  if constexpr (kFast) {
+    // the three waves' sums regrouped: d1 (u - c) + d2 u + d5 (u + c)
+    // = (d1 + d2 + d5) u + (d5 - d1) c, likewise for the energy
+    const double d51 = d5 - d1;
+#pragma unroll
+    for (int m = 0; m < NC - 2; ++m) {
+      double es, fl, fr;
+      if (m == D) {
+        es = dsum * ub[m] + d51 * c_b + lr * (dd[m] - dVl);
+        fl = rho_l * vl[m] * Vc_l + p_l;
+        fr = rho_r * vr[m] * Vc_r + p_r;
+      } else {
+        es = dsum * ub[m] + lr * dd[m];
+        fl = rho_l * vl[m] * Vc_l;
+        fr = rho_r * vr[m] * Vc_r;
+      }
+      F[1 + m] = 0.5 * ((fl + fr) - es);
+    }
+    double udu;
+    if constexpr (NC == 5) udu = u_b * du + v_b * dv + w_b * dw;
+    else udu = u_b * du + v_b * dv;
+    const double es = (d1 + d5) * H_b + d51 * (c_b * Vl_b) + d2 * (0.5 * q2_b) +
+                      lr * (udu - Vl_b * dVl);
+    F[NC - 1] = 0.5 * ((rho_l * H_l * Vc_l + rho_r * H_r * Vc_r) - es);
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < NC - 2; ++m) {
     double e1, e2, e5, fl, fr;
