@@ -384,19 +384,21 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   const double vr[3] = {u_r, v_r, w_r};
   if constexpr (kFast) {
     // the three waves' sums regrouped: d1 (u - c) + d2 u + d5 (u + c)
-    // = (d1 + d2 + d5) u + (d5 - d1) c, likewise for the energy
+    // = (d1 + d2 + d5) u + (d5 - d1) c, likewise for the energy; the
+    // convective fluxes reuse the mass fluxes rho V of each side
     const double d51 = d5 - d1;
+    const double ml = rho_l * Vc_l, mr = rho_r * Vc_r;
 #pragma unroll
     for (int m = 0; m < NC - 2; ++m) {
       double es, fl, fr;
       if (m == D) {
         es = dsum * ub[m] + d51 * c_b + lr * (dd[m] - dVl);
-        fl = rho_l * vl[m] * Vc_l + p_l;
-        fr = rho_r * vr[m] * Vc_r + p_r;
+        fl = ml * vl[m] + p_l;
+        fr = mr * vr[m] + p_r;
       } else {
         es = dsum * ub[m] + lr * dd[m];
-        fl = rho_l * vl[m] * Vc_l;
-        fr = rho_r * vr[m] * Vc_r;
+        fl = ml * vl[m];
+        fr = mr * vr[m];
       }
       F[1 + m] = 0.5 * ((fl + fr) - es);
     }
@@ -405,7 +407,7 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     else udu = u_b * du + v_b * dv;
     const double es = (d1 + d5) * H_b + d51 * (c_b * Vl_b) + d2 * (0.5 * q2_b) +
                       lr * (udu - Vl_b * dVl);
-    F[NC - 1] = 0.5 * ((rho_l * H_l * Vc_l + rho_r * H_r * Vc_r) - es);
+    F[NC - 1] = 0.5 * ((ml * H_l + mr * H_r) - es);
     return;
   }
 #pragma unroll
