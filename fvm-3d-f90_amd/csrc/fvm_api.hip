@@ -1377,6 +1377,8 @@ int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t rang
   if (!range_min || !range_max || !donor_range_min || !donor_range_max || !donor_type)
     return fail(FVM_EINVAL, "null range or transform");
   if (recv->ctx != donor->ctx) return fail(FVM_EINVAL, "blocks belong to different contexts");
+  // (a context holds blocks of one dimension; kept for the patch kernel's sake)
+  if (recv->dim != donor->dim) return fail(FVM_EINVAL, "a 2-D and a 3-D block cannot be connected");
   fvm_ctx* ctx = recv->ctx;
   if (ctx->visc_mu_ref > 0.0)
     return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
