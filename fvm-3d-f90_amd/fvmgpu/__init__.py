@@ -234,6 +234,7 @@ class Context:
         _check(_lib.fvm_ctx_create(device, ctypes.byref(h)))
         self._h = h
         self.blocks = []
+        self._snap_nvars = {}       # snapshot id -> number of variables
 
     def close(self):
         if self._h:
@@ -344,7 +345,6 @@ class Context:
         v = (ctypes.c_int * len(variables))(*(int(x) for x in variables))
         sid = ctypes.c_int64()
         _check(_lib.fvm_snapshot_begin(self._h, len(variables), v, ctypes.byref(sid)))
-        self._snap_nvars = getattr(self, "_snap_nvars", {})
         self._snap_nvars[sid.value] = len(variables)
         return sid.value
 
