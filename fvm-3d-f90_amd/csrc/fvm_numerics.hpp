@@ -604,6 +604,15 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
   const double m12 = m4p + m4m;
   const double mpos = 0.5 * (m12 + fabs(m12)), mneg = 0.5 * (m12 - fabs(m12));
   const double p12 = p5p * p_l + p5m * p_r;
+  if constexpr (kFast) {
+    // the interface speed times each side's density, once for all components
+    const double al = c12 * (mpos * rho_l), ar = c12 * (mneg * rho_r);
+    F[0] = al + ar;
+#pragma unroll
+    for (int m = 0; m < NC - 2; ++m) F[1 + m] = (m == D) ? al * vl[m] + ar * vr[m] + p12 : al * vl[m] + ar * vr[m];
+    F[NC - 1] = al * H_l + ar * H_r;
+    return;
+  }
   F[0] = c12 * (mpos * rho_l * 1.0 + mneg * rho_r * 1.0);
 #pragma unroll
   for (int m = 0; m < NC - 2; ++m) {
