@@ -957,6 +957,9 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
   v.V.r_gas = ctx->visc_r;
   v.V.gamma = ctx->P.gamma;
   v.V.cpr = ctx->P.gamma * ctx->visc_r / ((ctx->P.gamma - 1.0) * ctx->visc_pr);
+  v.V.rr_gas = 1.0 / ctx->visc_r;
+  v.V.rt_ref = 1.0 / ctx->visc_t_ref;
+  v.V.sfac = ctx->visc_t_ref + ctx->visc_s;
   for (fvm_block* b : ctx->blocks) {
     v.g = b->g;
     v.q = b->q[in_buf];

@@ -79,6 +79,7 @@ struct ViscParams {
   double r_gas;
   double cpr;                      // cp / Pr = gamma R / ((gamma - 1) Pr)
   double gamma;
+  double rr_gas, rt_ref, sfac;     // 1 / r_gas, 1 / t_ref, t_ref + s (fast family)
 };
 
 struct ViscArgs {

@@ -18,6 +18,7 @@
 // the primitive quantities of its tile (+1 halo) in LDS: a ring of three
 // k-planes in 3-D, one plane in 2-D.
 #pragma once
+#include "fvm_numerics.hpp"
 #include "fvm_types.hpp"
 
 namespace fvm {
@@ -48,6 +49,22 @@ __device__ __forceinline__ void prims(const double* __restrict__ q, const BlockG
                                       int k, const ViscParams& V, double out[5]) {
   const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
   const double rho = q[c];
+  if constexpr (kFastV) {
+    // one reciprocal of rho; reciprocals of R and T_ref from the host
+    const double rr = frcp(rho);
+    const double u = q[c + g.cs] * rr, v = q[c + 2 * g.cs] * rr;
+    const double w = NC == 5 ? q[c + 3 * g.cs] * rr : 0.0;
+    const double e = q[c + (NC - 1) * g.cs];
+    const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
+    const double t = p * (rr * V.rr_gas);
+    const double tr = t * V.rt_ref;
+    out[0] = u;
+    out[1] = v;
+    out[2] = w;
+    out[3] = t;
+    out[4] = V.mu_ref * tr * fsqrt(tr) * V.sfac * frcp(t + V.s);
+    return;
+  }
   const double u = q[c + g.cs] / rho, v = q[c + 2 * g.cs] / rho;
   const double w = NC == 5 ? q[c + 3 * g.cs] / rho : 0.0;
   const double e = q[c + (NC - 1) * g.cs];
@@ -163,6 +180,7 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
   load_plane(k0 - 1, 0);
   load_plane(k0, 1);
   int sm = 0, sc = 1, sp = 2;   // ring slots of k-1, k, k+1
+  double Fzm[4] = {0.0, 0.0, 0.0, 0.0};   // k- face flux of the owned column
   for (int k = k0; k < k1; ++k) {
     load_plane(k + 1, sp);
     __syncthreads();
@@ -172,8 +190,25 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
         const int slot = dz < 0 ? sm : (dz > 0 ? sp : sc);
         return S[slot][m][ctr + dy * W + dx];
       };
-      double acc[4];
-      visc::residual<3>(at, g, V, acc);
+      // x and y faces as visc::residual; the k- face is the previous
+      // plane's k+ face (same points, same arithmetic), carried in registers
+      double acc[4], Fp[4], Fm[4];
+      visc::face<3, 0, 0>(at, g, V, Fp);
+      visc::face<3, 0, -1>(at, g, V, Fm);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc[m] = visc::kFastV ? (Fp[m] - Fm[m]) * g.rdx : (Fp[m] - Fm[m]) / g.dx;
+      visc::face<3, 1, 0>(at, g, V, Fp);
+      visc::face<3, 1, -1>(at, g, V, Fm);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + (Fp[m] - Fm[m]) / g.dy;
+      visc::face<3, 2, 0>(at, g, V, Fp);
+      if (k == k0) visc::face<3, 2, -1>(at, g, V, Fzm);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fzm[m]) * g.rdz : acc[m] + (Fp[m] - Fzm[m]) / g.dz;
+        Fzm[m] = Fp[m];
+      }
       const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
 #pragma unroll
       for (int m = 0; m < 4; ++m) a.rv[(int64_t)m * g.cs + c] = acc[m];
