@@ -278,10 +278,13 @@ int fvm_time(fvm_ctx *ctx, double *t, int64_t *steps);
  * (converged_fun's abort tests, :183-191). */
 int fvm_norms(fvm_ctx *ctx, double out[6], int *nan_flag);
 
-/* Autotuner (no reference counterpart): times `reps` launches of the first
- * RK stage for every compiled tile shape and k-chunk of each block and keeps
- * the fastest.  The probe writes only stage buffers; the state is unchanged.
- * Call after fvm_upload.  fvm_block_launch_shape reports the choice. */
+/* Autotuner (no reference counterpart): times `reps` sweeps of one time
+ * step's stage kinds for every compiled launch shape and k-chunk (all blocks
+ * together in the batched launch modes, else per block), in two passes of
+ * which each candidate keeps its faster time (the first pass also warms the
+ * GPU), and keeps the fastest.  The probe writes only stage buffers; the
+ * state is unchanged.  Call after fvm_upload.  fvm_block_launch_shape and
+ * fvm_block_launch_tile report the choice. */
 int fvm_autotune(fvm_ctx *ctx, int reps);
 int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk);
 /* The launch-shape id behind that shape (the FVM_TILE / FVM_TILE2 values):
