@@ -75,7 +75,9 @@ def parse():
     ap.add_argument("--cfl", type=float, default=None)
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the CPU baseline sample (default per workload, about "
+                         "10-30 s of CPU work: vortex128 5, riemann400 30, blob27 40)")
     ap.add_argument("--numerics", choices=("exact", "fast"), default="fast",
                     help="exact: bit-identical to the reference arithmetic; fast: FMA + "
                          "reciprocals, <= 1e-12 rel. after 1 step, <= 1e-9 after 1000 (tested)")
@@ -234,12 +236,13 @@ def bench_vortex(args):
         config["output"] = output
     cpu = None
     if not args.no_cpu:
+        cs = args.cpu_steps or 5
         cpu = cpu_measure(
             lambda O: O.oracle_solve(p, 3, (h, h, h), recon=O.MUSCL, limiter=O.LIM_VANALBADA,
                                      flux=O.ROE, rk=O.RK4, kappa=0.33333, entfix=0.2, dt=0.0,
-                                     cfl=cfl, nsteps=args.cpu_steps, p_from_q=1,
+                                     cfl=cfl, nsteps=cs, p_from_q=1,
                                      bc=(O.BC_PERIODIC,) * 6),
-            ncell * 4 * args.cpu_steps, f"{args.cpu_steps} RK4 step(s) of the {n}^3 vortex workload")
+            ncell * 4 * cs, f"{cs} RK4 step(s) of the {n}^3 vortex workload")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
@@ -339,7 +342,7 @@ def bench_riemann(args):
               "pct_hbm_roofline": 100.0 * value * B_ALG["ssp3_2d"] / (HBM_PEAK_GBS * 1e9)}
     cpu = None
     if not args.no_cpu:
-        cs = max(args.cpu_steps, 10)
+        cs = args.cpu_steps or 30
         cpu = cpu_measure(
             lambda O: O.oracle_solve(p, 2, (dx, dx, 1.0), recon=O.MUSCL, limiter=O.LIM_MINMOD,
                                      flux=O.ROE, rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0,
@@ -455,12 +458,13 @@ def bench_blob27(args):
     if not args.no_cpu:
         key = max(states, key=lambda k: states[k].size)
         pb = states[key]
-        cells = float(pb.shape[0] * pb.shape[1] * pb.shape[2]) * 3 * args.cpu_steps
+        cs = args.cpu_steps or 40
+        cells = float(pb.shape[0] * pb.shape[1] * pb.shape[2]) * 3 * cs
         cpu = cpu_measure(
             lambda O: O.oracle_solve(pb, 3, sp, recon=O.WENO, limiter=O.LIM_NONE, flux=O.AUSMPLUS,
                                      rk=O.RK_SSP3, kappa=0.33333, entfix=0.2, dt=0.0, cfl=cfl,
-                                     nsteps=args.cpu_steps, p_from_q=1, weno_textbook=1),
-            cells, f"{args.cpu_steps} SSP-RK3 steps of the largest block {pb.shape[:3]} alone")
+                                     nsteps=cs, p_from_q=1, weno_textbook=1),
+            cells, f"{cs} SSP-RK3 steps of the largest block {pb.shape[:3]} alone")
     return line(args, value, steps, sec, config, roof, cpu)
 
 
