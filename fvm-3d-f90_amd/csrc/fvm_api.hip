@@ -231,9 +231,14 @@ __global__ void patch_copy_kernel(const BlockDesc* __restrict__ tab, const Patch
                                   int npatch, int64_t total, int buf) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    int p = 0;
-    while (p + 1 < npatch && pd[p + 1].start <= e) ++p;
-    const PatchDesc& P = pd[p];
+    // the last patch starting at or before e (starts ascend from 0)
+    int lo = 0, hi = npatch - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pd[mid].start <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const PatchDesc& P = pd[lo];
     const int64_t loc = e - P.start;
     const int ng = tab[P.recv].g.ng;
     const int a = (int)(loc % P.na);
