@@ -95,7 +95,7 @@ typedef struct fvm_block fvm_block;
 /* Text of the last error on the calling thread ("" if none). */
 const char *fvm_last_error(void);
 
-/* Library version string. */
+/* Library version string (no reference counterpart). */
 const char *fvm_version(void);
 
 /* Context on HIP device `device`: owns one stream and every block created on
@@ -204,9 +204,6 @@ int fvm_cut_uniform(int64_t range_min, int64_t range_max, int ncut, int64_t *pos
 #define FVM_LAUNCH_PERSISTENT 2
 int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 
-/* Host <-> device copies of the conserved state (host layout above).  Upload
- * also fills the device ghost layers (ClassFlowField%initial's ghost update,
- * src/solve/class_flow_field.f90:249-253). */
 /* Initial state computed on the device, with no host array: the block's
  * u^n at vertices x = origin[0] + i dx, y = origin[1] + j dy, z = origin[2]
  * + k dz (i, j, k from 0; the reference's grid is x = dx (i - 1),
@@ -232,6 +229,9 @@ int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 int fvm_block_init(fvm_block *blk, int kind, int nparams, const double *params,
                    const double origin[3]);
 
+/* Host <-> device copies of the conserved state (host layout above).  Upload
+ * also fills the device ghost layers (ClassFlowField%initial's ghost update,
+ * src/solve/class_flow_field.f90:249-253). */
 int fvm_upload(fvm_block *blk, const double *q, int host_ng);
 int fvm_download(fvm_block *blk, double *q, int host_ng);
 
@@ -287,7 +287,8 @@ int fvm_norms(fvm_ctx *ctx, double out[6], int *nan_flag);
  * fvm_block_launch_tile report the choice. */
 int fvm_autotune(fvm_ctx *ctx, int reps);
 int fvm_block_launch_shape(fvm_block *blk, int *tile_x, int *tile_y, int *chunk);
-/* The launch-shape id behind that shape (the FVM_TILE / FVM_TILE2 values):
+/* The launch-shape id behind that shape (no reference counterpart; the
+ * FVM_TILE / FVM_TILE2 values):
  * 3-D ids 0-3 the plane-staged stage kernel at 64x4, 32x8, 64x2, 16x16; ids
  * 4-5 the register-reconstruction kernel at 32x8, 16x16 (simple and MUSCL
  * reconstructions); 2-D id 0 the row kernel, 1-2 the tiled kernel. */
@@ -353,7 +354,7 @@ int fvm_server_poll(fvm_server *srv, int iterations, int steps_per_sync, int *n_
  * call for a request waits for its asynchronous download. */
 int fvm_server_result(fvm_server *srv, int64_t id, double *q_out, double info[8]);
 
-/* Wait for all work of the context. */
+/* Wait for all work of the context (no reference counterpart). */
 int fvm_sync(fvm_ctx *ctx);
 
 /* Asynchronous snapshot of the solution (the reference writes the output
