@@ -269,20 +269,28 @@ __device__ __forceinline__ void weno_js_pair(double u0, double u1, double u2, do
   const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
   const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
   const double P12 = E1 * E2, P02 = E0 * E2, P01 = E0 * E1;
+  double nr, dr, nl, dl;
   {
     const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
     const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
     const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
     const double b0 = c_0 * P12, b1 = c_1 * P02, b2 = c_2 * P01;
-    right = (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
+    nr = b0 * q0 + b1 * q1 + b2 * q2;
+    dr = b0 + b1 + b2;
   }
   {
     const double q0 = c13 * u4 - c76 * u3 + c116 * u2;
     const double q1 = -(c16 * u3) + c56 * u2 + c13 * u1;
     const double q2 = c13 * u2 + c56 * u1 - c16 * u0;
     const double b0 = c_0 * P01, b1 = c_1 * P02, b2 = c_2 * P12;
-    left = (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
+    nl = b0 * q0 + b1 * q1 + b2 * q2;
+    dl = b0 + b1 + b2;
   }
+  // one reciprocal for both weight sums (their product stays a normal double
+  // while each smoothness indicator is below ~1e38)
+  const double inv = frcp(dr * dl);
+  right = nr * (dl * inv);
+  left = nl * (dr * inv);
 }
 
 // ---- Harten fix, src/scheme/scheme_FDS_Roe.f90:167-177 ----------------------
