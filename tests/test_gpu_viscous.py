@@ -41,14 +41,19 @@ def oracle_run(p, spacing, nsteps, visc, dt=0.0, cfl=0.0, periodic=True, recon=O
 TGV_SCHEME = dict(recon=O.WENO, limiter=O.LIM_NONE, flux=O.VANLEER, rk=O.RK4, ref_compat=0)
 
 
-@pytest.mark.parametrize("scheme", [TGV_SCHEME,
-                                    dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE,
-                                         rk=O.RK_SSP3)])
-def test_tgv_exact_bitwise(scheme):
-    # Re = 100 so the viscous term is not negligible at this resolution
+@pytest.mark.parametrize("scheme,tile", [(TGV_SCHEME, ""),
+                                         (dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA,
+                                               flux=O.ROE, rk=O.RK_SSP3), ""),
+                                         (dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA,
+                                               flux=O.ROE, rk=O.RK_SSP3), "4")])
+def test_tgv_exact_bitwise(scheme, tile, monkeypatch):
+    # Re = 100 so the viscous term is not negligible at this resolution;
+    # shape 4: the register-reconstruction kernel adding the viscous residual
+    if tile:
+        monkeypatch.setenv("FVM_TILE", tile)
     p, h = cases.tgv(20)
     visc = cases.tgv_params(re=100.0)
-    print(f"case tgv20 exact {scheme}", flush=True)
+    print(f"case tgv20 exact {scheme} tile={tile}", flush=True)
     q, (norms, nan) = gpu_run(p, (h, h, h), 3, visc, dt=2e-3, **scheme)
     qo, r = oracle_run(p, (h, h, h), 3, visc, dt=2e-3, **scheme)
     assert np.array_equal(q, qo)
