@@ -55,6 +55,8 @@ def test_invalid_requests_are_errors():
             ctx.block((3, 8, 8), (0.1, 0.1, 0.1))         # below the minimum extent
         with pytest.raises(fvmgpu.FvmError):
             ctx.block((8, 8, 8), (0.1, -0.1, 0.1))        # negative spacing
+        with pytest.raises(fvmgpu.FvmError, match="2\\^29"):
+            ctx.block((1024, 1024, 512), (0.1, 0.1, 0.1))  # 32-bit component offsets
         with pytest.raises(fvmgpu.FvmError):
             ctx.scheme(recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=999, rk=O.RK_SSP3)
         a = ctx.block((8, 8, 8), (0.1, 0.1, 0.1))
