@@ -1751,7 +1751,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     const int ntiles = dim == 3 ? kNumTiles3d : kNumTiles2d;
     // two passes over the candidates, each keeping its faster time: the
     // first pass also brings the GPU's clocks up, so the shapes timed first
-    // are not judged on a cold device
+    // are not timed on a cold device
     for (int pass = 0; pass < 2; ++pass)
     for (int tile = 0; tile < ntiles; ++tile)
       for (int ci = 0; ci < 4; ++ci) {

@@ -20,7 +20,7 @@ CASE = os.path.join(ROOT, "tests", "cases", "riemann2d_101.nml")
 def test_fortran_driver_riemann2d(tmp_path):
     assert os.path.exists(DRIVER), "build the Fortran driver: make -C fvm-3d-f90_amd"
     out = tmp_path / "q.bin"
-    text = open(CASE).read().replace('"gpurun_out/fortran_riemann2d_101.bin"', f'"{out}"')
+    text = open(CASE).read().replace('"fortran_riemann2d_101.bin"', f'"{out}"')
     nml = tmp_path / "case.nml"
     nml.write_text(text)
     print("case fortran driver riemann2d 101x101, 20 steps", flush=True)
@@ -47,7 +47,7 @@ def test_fortran_driver_output_snapshots(tmp_path):
     every output_time, each equal to the oracle's state at that step."""
     prefix = tmp_path / "flow"
     text = open(CASE).read()
-    text = text.replace('"gpurun_out/fortran_riemann2d_101.bin"', '""')
+    text = text.replace('"fortran_riemann2d_101.bin"', '""')
     text = text.replace("output_time  = 0.01", "output_time  = 1.0E-3")
     text = text.replace("print_every = 5", f'print_every = 5\n  snapshot_prefix = "{prefix}"')
     nml = tmp_path / "case.nml"
