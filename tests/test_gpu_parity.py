@@ -3,6 +3,8 @@
 The kernels keep the reference's IEEE operation order and are built without
 FMA contraction, so the expected result is bit-identical: the assertion is
 exact equality of every conserved value, with the relative L-inf printed."""
+import os
+
 import numpy as np
 import pytest
 
@@ -148,3 +150,24 @@ def test_register_tile_refused_for_weno(monkeypatch):
         qg, _, _, _ = run_gpu(ctx, p, sp, 2, dt=1e-3, recon=O.WENO, limiter=O.LIM_NONE, flux=O.ROE)
     qo, _ = run_oracle(p, sp, 2, dt=1e-3, recon=O.WENO, limiter=O.LIM_NONE, flux=O.ROE)
     assert np.array_equal(qg, qo)
+
+
+def test_vortex128_full_size_bitwise():
+    """The benchmark's workload at its full size (128^3 isentropic vortex,
+    MUSCL-vanAlbada + Roe, RK4, periodic, CFL 0.8), exact family: two CFL
+    steps equal the oracle's bit for bit (the oracle runs on the host's
+    threads: a few seconds), and mass, momentum and energy are conserved on
+    the periodic box to round-off."""
+    p, h, _ = cases.isentropic_vortex(128, seed=0)
+    print("case vortex128 full size, 2 CFL steps", flush=True)
+    with fvmgpu.Context() as ctx:
+        qg, ng, _, dt_last = run_gpu(ctx, p, (h, h, h), 2, cfl=0.8, bc=(1,) * 6,
+                                     recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+    O.set_threads(int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1)
+    qo, r = run_oracle(p, (h, h, h), 2, cfl=0.8, bc=(1,) * 6, recon=O.MUSCL,
+                       limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+    assert dt_last == r["dt"][-1]
+    assert np.array_equal(qg, qo)
+    q0 = cases.to_conserved(p)
+    for c in range(5):
+        assert abs(qg[c].sum() - q0[c].sum()) <= 1e-12 * np.abs(q0[c]).sum() + 1e-12
