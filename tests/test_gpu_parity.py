@@ -171,3 +171,17 @@ def test_vortex128_full_size_bitwise():
     q0 = cases.to_conserved(p)
     for c in range(5):
         assert abs(qg[c].sum() - q0[c].sum()) <= 1e-12 * np.abs(q0[c]).sum() + 1e-12
+
+
+def test_riemann400_full_size_bitwise():
+    """The reference's own case at its full size (400^2 four-quadrant
+    Riemann problem, MUSCL-minmod + Roe, SSP-RK3, CFL 0.5), exact family:
+    ten CFL steps equal the oracle's bit for bit."""
+    p, dx = cases.riemann2d(400, seed=0)
+    print("case riemann400 full size, 10 CFL steps", flush=True)
+    with fvmgpu.Context() as ctx:
+        qg, ng, _, dt_last = run_gpu(ctx, p, (dx, dx, 1.0), 10, cfl=0.5)
+    O.set_threads(int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1)
+    qo, r = run_oracle(p, (dx, dx, 1.0), 10, cfl=0.5)
+    assert dt_last == r["dt"][-1]
+    assert np.array_equal(qg, qo)

@@ -115,3 +115,17 @@ def test_tgv_initial_dissipation_rate():
     rate = (e0 - e1) / t
     print("dissipation rate", rate, "closed form", 0.75 / re)
     assert abs(rate / (0.75 / re) - 1.0) < 0.05
+
+
+def test_tgv192_full_size_bitwise():
+    """The Taylor-Green workload at its full size (192^3, Re 1600, WENO5 +
+    Van Leer + viscous, RK4), exact family: one CFL step equals the oracle's
+    bit for bit (the oracle on the host's threads: a few seconds)."""
+    import os
+    p, h = cases.tgv(192)
+    visc = cases.tgv_params()
+    print("case tgv192 full size, 1 CFL step", flush=True)
+    q, _ = gpu_run(p, (h, h, h), 1, visc, cfl=0.5, **TGV_SCHEME)
+    O.set_threads(int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1)
+    qo, r = oracle_run(p, (h, h, h), 1, visc, cfl=0.5, **TGV_SCHEME)
+    assert np.array_equal(q, qo)
