@@ -2697,6 +2697,12 @@ int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   if (!r->q0 && r->ic_kind == 0) return fail(FVM_EINVAL, "null initial state");
   if (r->n[0] < 4 || r->n[1] < 4 || r->n[2] < 1) return fail(FVM_EINVAL, "bad grid size");
   if (!(r->t_end > 0.0) || !(r->cfl > 0.0)) return fail(FVM_EINVAL, "t_end and cfl must be positive");
+  for (int f = 0; f < 6; ++f) {
+    const BcClass cls = bc_class(r->bc[f]);
+    if (cls != BCC_EXTRAP && cls != BCC_PERIODIC)
+      return fail(FVM_EINVAL, "served solves take extrapolated or periodic faces (face %d: kind %d)", f,
+                  r->bc[f]);
+  }
   const int dim = r->n[2] == 1 ? 2 : 3;
   const int nc = dim == 3 ? 5 : 4;
   const double cs = (double)(r->n[0] + 6) * (r->n[1] + 6) * (dim == 3 ? r->n[2] + 6 : 1);

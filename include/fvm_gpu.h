@@ -322,7 +322,9 @@ typedef struct fvm_request {
   int recon, limiter, flux, rk;
   double kappa, entropy_fix, gamma;
   int ref_compat, numerics;
-  int bc[6];                /* FVM_BC_EXTRAP / FVM_BC_PERIODIC per face         */
+  int bc[6];                /* FVM_BC_EXTRAP (or the outflow / extrapolate
+                               codes) or FVM_BC_PERIODIC per face; other kinds
+                               are refused                                      */
   double t_end, cfl;
   const double *q0;         /* (5, nk, nj, ni) conserved state                  */
   int borrow;               /* 0: q0 is copied by fvm_server_submit.  1: q0 is

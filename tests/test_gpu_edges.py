@@ -1,6 +1,8 @@
 """Edge cases of the GPU path: the smallest blocks, extents one past and one
 short of a tile, thin blocks, and the errors the C ABI reports for invalid
 requests (forced from the host; no kernel sees bad input)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -89,6 +91,16 @@ def test_invalid_requests_are_errors():
             srv.submit(np.ones((5, 1, 8, 8)), (0.1, 0.1, 1.0), t_end=0.0, cfl=0.5)   # t_end
         with pytest.raises(fvmgpu.FvmError):
             srv.result(12345)                                                        # unknown id
+        r = fvmgpu.Request()
+        r.n[0], r.n[1], r.n[2] = 8, 8, 8
+        r.spacing[0] = r.spacing[1] = r.spacing[2] = 0.1
+        r.recon, r.limiter, r.flux, r.rk = O.MUSCL, O.LIM_MINMOD, O.ROE, O.RK_SSP3
+        r.gamma, r.t_end, r.cfl = 1.4, 0.1, 0.5
+        r.bc[2] = fvmgpu.BC_WALL                                                     # not served
+        r.ic_kind, r.ic_nparams = fvmgpu.IC_TGV, 1
+        r.ic_params[0] = 0.1
+        rid = ctypes.c_int64()
+        assert fvmgpu.load_library().fvm_server_submit(srv._h, ctypes.byref(r), ctypes.byref(rid)) < 0
 
 
 def test_nan_state_is_reported():
