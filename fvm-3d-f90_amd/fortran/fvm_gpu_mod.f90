@@ -52,7 +52,26 @@ module fvm_gpu
   public :: fvm_set_launch_mode, fvm_set_graph
   public :: fvm_snapshot_begin, fvm_snapshot_ready, fvm_snapshot_read, fvm_snapshot_save
   public :: fvm_snapshot_release, fvm_restart_load
+  public :: fvm_server_create, fvm_server_destroy, fvm_server_submit, fvm_server_poll
+  public :: fvm_server_result
   public :: fvm_check, fvm_error_text, scheme_index
+
+!...A served solve (fvm_request of include/fvm_gpu.h, same layout)
+  integer(c_int), parameter, public :: FVM_IC_MAX_PARAMS = 176
+  type, bind(C), public :: fvm_request
+    integer(c_int64_t) :: n(3) = 0
+    real(c_double)     :: spacing(3) = 0.0_c_double
+    integer(c_int)     :: recon = 0, limiter = 0, flux = 0, rk = 0
+    real(c_double)     :: kappa = 0.0_c_double, entropy_fix = 0.0_c_double, gamma = 1.4_c_double
+    integer(c_int)     :: ref_compat = 1, numerics = 0
+    integer(c_int)     :: bc(6) = 0
+    real(c_double)     :: t_end = 0.0_c_double, cfl = 0.0_c_double
+    type(c_ptr)        :: q0 = c_null_ptr
+    integer(c_int)     :: borrow = 0
+    integer(c_int)     :: ic_kind = 0, ic_nparams = 0
+    real(c_double)     :: ic_params(176) = 0.0_c_double
+    real(c_double)     :: ic_origin(3) = 0.0_c_double
+  end type fvm_request
 
   interface
     function fvm_last_error() bind(C, name="fvm_last_error") result(msg)
@@ -144,6 +163,47 @@ module fvm_gpu
       integer(c_int), intent(out) :: nan_flag
       integer(c_int) :: rc
     end function fvm_norms
+
+    function fvm_server_create(device, hbm_budget, max_active, srv) &
+        bind(C, name="fvm_server_create") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: device, max_active
+      real(c_double), value :: hbm_budget
+      type(c_ptr), intent(out) :: srv
+      integer(c_int) :: rc
+    end function fvm_server_create
+
+    function fvm_server_destroy(srv) bind(C, name="fvm_server_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: srv
+      integer(c_int) :: rc
+    end function fvm_server_destroy
+
+    function fvm_server_submit(srv, req, id) bind(C, name="fvm_server_submit") result(rc)
+      import :: c_int, c_int64_t, c_ptr, fvm_request
+      type(c_ptr), value :: srv
+      type(fvm_request), intent(in) :: req
+      integer(c_int64_t), intent(out) :: id
+      integer(c_int) :: rc
+    end function fvm_server_submit
+
+    function fvm_server_poll(srv, iterations, steps_per_sync, n_pending, n_active, n_done) &
+        bind(C, name="fvm_server_poll") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: srv
+      integer(c_int), value :: iterations, steps_per_sync
+      integer(c_int), intent(out) :: n_pending, n_active, n_done
+      integer(c_int) :: rc
+    end function fvm_server_poll
+
+    ! q_out: c_loc of q(ni, nj, nk, 5); info(8) as include/fvm_gpu.h lists it
+    function fvm_server_result(srv, id, q_out, info) bind(C, name="fvm_server_result") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: srv, q_out
+      integer(c_int64_t), value :: id
+      real(c_double), intent(out) :: info(8)
+      integer(c_int) :: rc
+    end function fvm_server_result
 
     function fvm_sync(ctx) bind(C, name="fvm_sync") result(rc)
       import :: c_int, c_ptr

@@ -3,6 +3,7 @@ declares; argument checks that need no GPU return error codes, never abort."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -77,6 +78,40 @@ def test_request_struct_layout_matches_header(tmp_path):
     out = dict(line.split() for line in subprocess.run([str(exe)], check=True,
                                                        capture_output=True, text=True)
                .stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(fvmgpu.Request)
+    for f in fields:
+        assert int(out[f]) == getattr(fvmgpu.Request, f).offset, f
+
+
+def test_fortran_request_layout(tmp_path):
+    """The Fortran module's bind(C) fvm_request has the C struct's size and
+    field offsets (compared with the ctypes structure, itself checked against
+    the C compiler above)."""
+    import shutil
+    fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler")
+    mod = os.path.join(ROOT, "fvm-3d-f90_amd", "fortran", "fvm_gpu_mod.f90")
+    fields = ["n", "spacing", "recon", "kappa", "ref_compat", "bc", "t_end", "q0", "borrow",
+              "ic_kind", "ic_nparams", "ic_params", "ic_origin"]
+    lines = ["program layout", "  use iso_c_binding", "  use fvm_gpu", "  implicit none",
+             "  type(fvm_request), target :: r",
+             "  integer(c_intptr_t) :: b",
+             "  b = transfer(c_loc(r), b)",
+             "  print '(a, i0)', 'size ', c_sizeof(r)"]
+    for f in fields:
+        lines.append(f"  print '(a, i0)', '{f} ', transfer(c_loc(r%{f}), b) - b")
+    lines.append("end program layout")
+    src = tmp_path / "layout.f90"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([fc, "-O0", "-module-dir", str(tmp_path), "-c", mod, "-o", str(tmp_path / "m.o")],
+                   check=True, capture_output=True)
+    lib = os.path.join(ROOT, "fvm-3d-f90_amd", "lib")   # the module's helpers name the library
+    subprocess.run([fc, "-O0", "-I", str(tmp_path), str(src), str(tmp_path / "m.o"), "-o", str(exe),
+                    "-L" + lib, "-lfvmgpu", "-Wl,-rpath," + lib], check=True, capture_output=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.strip().splitlines())
     assert int(out["size"]) == ctypes.sizeof(fvmgpu.Request)
     for f in fields:
         assert int(out[f]) == getattr(fvmgpu.Request, f).offset, f
