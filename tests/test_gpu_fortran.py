@@ -69,3 +69,29 @@ def test_fortran_driver_output_snapshots(tmp_path):
             q = cases.to_conserved(p)
         pv = np.moveaxis(cases.to_primitive(q), -1, 0)
         assert np.array_equal(arr, pv[[0, 1, 2, 4]])
+
+
+def test_fortran_serving_equals_python_serving(tmp_path):
+    """The Fortran serving example (bind(C) fvm_request, initial states
+    computed on the device) gives, bit for bit, the states the Python binding
+    gets for the same requests submitted with uploaded initial states."""
+    exe = os.path.join(ROOT, "fvm-3d-f90_amd", "bin", "fvm_serve")
+    assert os.path.exists(exe), "build the Fortran examples: make -C fvm-3d-f90_amd"
+    prefix = tmp_path / "served"
+    print("case fortran serving example, 3 Riemann requests", flush=True)
+    r = subprocess.run([exe, str(prefix)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    with fvmgpu.Server(max_active=16) as srv:
+        ids = {}
+        for n in (41, 64, 101):
+            p, dx = cases.riemann2d(n, seed=0)
+            ids[n] = srv.submit(cases.to_conserved(p), (dx, dx, 1.0), 0.01, 0.5, recon=O.MUSCL,
+                                limiter=O.LIM_MINMOD, flux=O.ROE, rk=O.RK_SSP3)
+        while srv.poll(16, 1)[2] < 3:
+            pass
+        for n, i in ids.items():
+            q, info = srv.result(i)
+            assert info["status"] == 2
+            f = np.fromfile(f"{prefix}_{n}.bin", dtype=np.float64).reshape(5, 1, n, n)
+            assert np.array_equal(f, q)
