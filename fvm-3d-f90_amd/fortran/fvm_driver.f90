@@ -17,7 +17,11 @@
 !          max_step = 0 (0: no limit), print_every = 1,
 !          output_file = "" (raw dump of the final q with 3 ghost layers),
 !          snapshot_prefix = "" (write the &Output variables at the output
-!          times of &Time_Stepping to <prefix>_<step>.fvmsnap) /
+!          times of &Time_Stepping to <prefix>_<step>.fvmsnap),
+!          restart_file = "" (resume from this conserved-state snapshot: its
+!          time and step count carry on), checkpoint_file = "" (write the
+!          conserved state there at the end: the reference's .rst file,
+!          src/solve/class_output.f90:194-195, as a snapshot) /
 !
 !=============================================================================80
 program fvm_driver
@@ -47,6 +51,7 @@ program fvm_driver
   character(len=16) :: numerics = "exact"
   real(dp) :: cfl = 0.5_dp
   character(len=256) :: output_file = "", snapshot_prefix = ""
+  character(len=256) :: restart_file = "", checkpoint_file = ""
 
   namelist /General/ name, dimension, dimensionless, equation, gasModel
   namelist /Geometry/ type, geoType, geoSize, gridType, gridSize, writeGrid
@@ -56,7 +61,8 @@ program fvm_driver
   namelist /Time_Stepping/ time_scheme, timestep_set, fix_timestep, total_time, &
                            output_time, absolute_residual, relative_residual
   namelist /Output/ format, Density, VelocityX, VelocityY, VelocityZ, Pressure
-  namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file, snapshot_prefix
+  namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file, snapshot_prefix, &
+                 restart_file, checkpoint_file
 
   character(len=512) :: arg, casefile
   integer :: unit, ios, i, j, k, ni, nj, nk, step, nan_flag, eq
@@ -65,7 +71,7 @@ program fvm_driver
   type(c_ptr) :: ctx, blk
   real(dp) :: regions(11, 4), time_out
   integer(c_int) :: vars(5), nvars
-  integer(c_int64_t) :: snaps(4)
+  integer(c_int64_t) :: snaps(4), step0
   integer :: nsnap
   logical :: writing
   integer(c_int) :: rc, recon_c, lim_c, flux_c, rk_c, mode
@@ -165,8 +171,15 @@ program fvm_driver
   write(*, '(a8, a14, 5a14)') "Step", "Time", "R_rho", "R_momx", "R_momy", "R_energy", "R_total"
   time = 0.0_dp
   step = 0
+  if (len_trim(restart_file) > 0) then
+    call fvm_check(fvm_restart_load(ctx, trim(restart_file) // c_null_char), "restart")
+    call fvm_check(fvm_time(ctx, time, step0), "time")
+    step = int(step0)
+    write(*, '(a, a, a, i0, a, es14.6)') "restarted from ", trim(restart_file), " at step ", &
+      step, ", t = ", time
+  end if
   converged = .false.
-  if (writing) call write_output()          ! the initial flowfield
+  if (writing .and. step == 0) call write_output()          ! the initial flowfield
   do while (.not. converged)
     step = step + 1
     if (trim(timestep_set) == "CFL") then
@@ -199,6 +212,16 @@ program fvm_driver
   do i = 1, nsnap
     call fvm_check(fvm_snapshot_release(ctx, snaps(i)), "snapshot write")
   end do
+  if (len_trim(checkpoint_file) > 0) then
+    block
+      integer(c_int) :: cons(5)
+      integer(c_int64_t) :: sid
+      cons = [FVM_VAR_RHO, FVM_VAR_RHOU, FVM_VAR_RHOV, FVM_VAR_RHOW, FVM_VAR_RHOE]
+      call fvm_check(fvm_snapshot_begin(ctx, 5_c_int, cons, sid), "checkpoint")
+      call fvm_check(fvm_snapshot_save(ctx, sid, trim(checkpoint_file) // c_null_char), "checkpoint")
+      call fvm_check(fvm_snapshot_release(ctx, sid), "checkpoint write")
+    end block
+  end if
   call fvm_check(fvm_download(blk, c_loc(q), int(NG, c_int)), "download")
   if (len_trim(output_file) > 0) then
     open(newunit=unit, file=trim(output_file), access="stream", form="unformatted", &

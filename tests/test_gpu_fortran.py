@@ -95,3 +95,34 @@ def test_fortran_serving_equals_python_serving(tmp_path):
             assert info["status"] == 2
             f = np.fromfile(f"{prefix}_{n}.bin", dtype=np.float64).reshape(5, 1, n, n)
             assert np.array_equal(f, q)
+
+
+def test_fortran_checkpoint_restart(tmp_path):
+    """The driver's checkpoint_file / restart_file: 10 steps, a checkpoint,
+    then a second run restarting from it to step 20, equal bit for bit to one
+    run of 20 steps (time and step count carried over)."""
+    base = open(CASE).read()
+
+    def run(tag, **gpu):
+        text = base.replace('"fortran_riemann2d_101.bin"', f'"{tmp_path / tag}.bin"')
+        text = text.replace("max_step    = 20", f"max_step    = {gpu.pop('max_step')}")
+        extra = "".join(f'\n  {k} = {v}' for k, v in gpu.items())
+        text = text.replace("print_every = 5", "print_every = 5" + extra)
+        nml = tmp_path / f"{tag}.nml"
+        nml.write_text(text)
+        r = subprocess.run([DRIVER, f"--input={nml}"], capture_output=True, text=True, timeout=120)
+        print(r.stdout[-600:], r.stderr[-600:])
+        assert r.returncode == 0
+        raw = open(tmp_path / f"{tag}.bin", "rb").read()
+        steps = int(np.frombuffer(raw[:20], dtype=np.int32)[4])
+        t = float(np.frombuffer(raw[20:28], dtype=np.float64)[0])
+        return steps, t, np.frombuffer(raw[28:], dtype=np.float64)
+
+    print("case fortran checkpoint and restart", flush=True)
+    ck = tmp_path / "state.fvmsnap"
+    s1, t1, _ = run("first", max_step=10, checkpoint_file=f'"{ck}"')
+    assert s1 == 10
+    s2, t2, q2 = run("second", max_step=20, restart_file=f'"{ck}"')
+    s3, t3, q3 = run("whole", max_step=20)
+    assert s2 == s3 == 20 and t2 == t3
+    assert np.array_equal(q2, q3)
