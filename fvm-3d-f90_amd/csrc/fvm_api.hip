@@ -347,6 +347,8 @@ static_assert(11 * kInitMaxRegions == FVM_IC_MAX_PARAMS, "region parameter capac
 struct InitSpec {
   int kind, nregions;
   double origin[3];
+  int at_index;                    // 1: x = dx (first[0] + i), ... (origin unused)
+  long long first[3];
   double gamma;
   // regions: per region xmin, xmax, ymin, ymax, zmin, zmax, rho, u, v, w, p;
   // vortex: length, beta, xc, yc, u_inf, v_inf, w_inf; TGV: Mach number
@@ -414,8 +416,12 @@ __global__ __launch_bounds__(256) void init_kernel(double* __restrict__ q, Block
     const int j = (int)((e / g.nx) % g.ny);
     const int k = (int)(e / ((int64_t)g.nx * g.ny));
     double p[5];
-    init_state(s, s.origin[0] + g.dx * (double)i, s.origin[1] + g.dy * (double)j,
-               s.origin[2] + g.dz * (double)k, p);
+    if (s.at_index)   // one product, as the reference's grid x = dx (i - 1)
+      init_state(s, g.dx * (double)(s.first[0] + i), g.dy * (double)(s.first[1] + j),
+                 g.dz * (double)(s.first[2] + k), p);
+    else
+      init_state(s, s.origin[0] + g.dx * (double)i, s.origin[1] + g.dy * (double)j,
+                 s.origin[2] + g.dz * (double)k, p);
     const double rho = p[0], u = p[1], v = p[2], w = p[3];
     const double en = p[4] / (s.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
     const int64_t ci = gidx(g, i, j, k);
@@ -1893,6 +1899,28 @@ void launch_init(const fvm_block* b, const InitSpec& s, hipStream_t stream) {
 }  // namespace
 
 extern "C" {
+
+int fvm_block_init_at(fvm_block* b, int kind, int nparams, const double* params,
+                      const int64_t first[3]) {
+  if (!b) return fail(FVM_EINVAL, "null block");
+  fvm_ctx* ctx = b->ctx;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called (gamma)");
+  InitSpec s;
+  if (int rc = make_init_spec(b->dim, kind, nparams, params, nullptr, ctx->P.gamma, &s)) return rc;
+  if (first) {
+    for (int e = 0; e < 3; ++e)
+      if (first[e] < 0 || first[e] > (int64_t(1) << 40)) return fail(FVM_EINVAL, "first vertex out of range");
+    s.at_index = 1;
+    for (int e = 0; e < 3; ++e) s.first[e] = first[e];
+  }
+  if (int rc = set_device(ctx)) return rc;
+  launch_init(b, s, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  b->uploaded = true;
+  ctx->ghosts_dirty = true;
+  return FVM_OK;
+}
 
 int fvm_block_init(fvm_block* b, int kind, int nparams, const double* params, const double origin[3]) {
   if (!b) return fail(FVM_EINVAL, "null block");

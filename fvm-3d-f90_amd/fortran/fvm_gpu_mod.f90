@@ -46,7 +46,7 @@ module fvm_gpu
 
   public :: fvm_ctx_create, fvm_ctx_destroy, fvm_scheme_set, fvm_set_numerics
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc, fvm_block_set_bc_state
-  public :: fvm_block_init, fvm_cut_uniform, fvm_block_launch_tile
+  public :: fvm_block_init, fvm_block_init_at, fvm_cut_uniform, fvm_block_launch_tile
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
@@ -238,6 +238,17 @@ module fvm_gpu
       integer(c_int), intent(out) :: tile_id
       integer(c_int) :: rc
     end function fvm_block_launch_tile
+
+    ! a block whose first vertex is vertex first (0-based) of a larger grid
+    function fvm_block_init_at(blk, kind, nparams, params, first) bind(C, name="fvm_block_init_at") &
+        result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: blk
+      integer(c_int), value :: kind, nparams
+      real(c_double), intent(in) :: params(*)
+      integer(c_int64_t), intent(in) :: first(3)
+      integer(c_int) :: rc
+    end function fvm_block_init_at
 
     ! prim = (rho, u, v, w, p) of an inflow / farfield face
     function fvm_block_set_bc_state(blk, face, kind, prim) bind(C, name="fvm_block_set_bc_state") &

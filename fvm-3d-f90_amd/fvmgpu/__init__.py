@@ -57,7 +57,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
            "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform",
-           "fvm_block_launch_tile")
+           "fvm_block_launch_tile", "fvm_block_init_at")
 
 
 class FvmError(RuntimeError):
@@ -124,6 +124,7 @@ def load_library(path=LIB_PATH):
         "fvm_restart_load": ([vp, ctypes.c_char_p], ci),
         "fvm_block_set_bc_state": ([vp, ci, ci, ctypes.POINTER(cd)], ci),
         "fvm_block_init": ([vp, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(cd)], ci),
+        "fvm_block_init_at": ([vp, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(i64)], ci),
         "fvm_cut_uniform": ([i64, i64, ci, ctypes.POINTER(i64)], ci),
         "fvm_block_launch_tile": ([vp, ctypes.POINTER(ci)], ci),
         "fvm_server_create": ([ci, cd, ci, c_void_pp], ci),
@@ -206,6 +207,13 @@ class Block:
         pv = (ctypes.c_double * max(1, len(params)))(*(float(x) for x in params))
         ov = None if origin is None else (ctypes.c_double * 3)(*(float(x) for x in origin))
         _check(_lib.fvm_block_init(self._h, kind, len(params), pv, ov))
+
+    def init_at(self, kind, params, first=(0, 0, 0)):
+        """Initial state of a block whose first vertex is vertex `first` of a
+        larger grid (coordinates dx * (first + i), one product)."""
+        pv = (ctypes.c_double * max(1, len(params)))(*(float(x) for x in params))
+        fv = (ctypes.c_int64 * 3)(*(int(x) for x in first))
+        _check(_lib.fvm_block_init_at(self._h, kind, len(params), pv, fv))
 
     def launch_tile(self):
         """Launch-shape id (the FVM_TILE / FVM_TILE2 value) of this block."""

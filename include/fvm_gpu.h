@@ -228,6 +228,13 @@ int fvm_set_launch_mode(fvm_ctx *ctx, int mode);
 #define FVM_IC_MAX_PARAMS 176   /* 16 regions of 11 values */
 int fvm_block_init(fvm_block *blk, int kind, int nparams, const double *params,
                    const double origin[3]);
+/* The same for a block that is one piece of a larger grid whose first vertex
+ * is vertex first[] (0-based) of that grid: x = dx (first[0] + i), ... one
+ * product as the reference's x = dx (i - 1) (src/base/method_gridgen.f90:
+ * 120-131), so the pieces of a cut (fvm_cut_uniform) get the whole grid's
+ * initial state bit for bit.  first may be NULL (0, 0, 0). */
+int fvm_block_init_at(fvm_block *blk, int kind, int nparams, const double *params,
+                      const int64_t first[3]);
 
 /* Host <-> device copies of the conserved state (host layout above).  Upload
  * also fills the device ghost layers (ClassFlowField%initial's ghost update,
