@@ -21,7 +21,10 @@
 !          restart_file = "" (resume from this conserved-state snapshot: its
 !          time and step count carry on), checkpoint_file = "" (write the
 !          conserved state there at the end: the reference's .rst file,
-!          src/solve/class_output.f90:194-195, as a snapshot) /
+!          src/solve/class_output.f90:194-195, as a snapshot),
+!          ncut = 1, 1, 1 (run the grid as ncut(1) x ncut(2) x ncut(3) blocks
+!          of the reference's uniform cut, joined across their shared vertex
+!          planes; output_file still holds the whole grid) /
 !
 !=============================================================================80
 program fvm_driver
@@ -52,6 +55,7 @@ program fvm_driver
   real(dp) :: cfl = 0.5_dp
   character(len=256) :: output_file = "", snapshot_prefix = ""
   character(len=256) :: restart_file = "", checkpoint_file = ""
+  integer :: ncut(3) = [1, 1, 1]
 
   namelist /General/ name, dimension, dimensionless, equation, gasModel
   namelist /Geometry/ type, geoType, geoSize, gridType, gridSize, writeGrid
@@ -62,13 +66,17 @@ program fvm_driver
                            output_time, absolute_residual, relative_residual
   namelist /Output/ format, Density, VelocityX, VelocityY, VelocityZ, Pressure
   namelist /GPU/ device, numerics, cfl, max_step, print_every, output_file, snapshot_prefix, &
-                 restart_file, checkpoint_file
+                 restart_file, checkpoint_file, ncut
 
   character(len=512) :: arg, casefile
   integer :: unit, ios, i, j, k, ni, nj, nk, step, nan_flag, eq
   real(dp) :: dx, dy, dz, gamma, time, dt, dt_used, res(6), res0(6)
   real(dp), allocatable, target :: q(:,:,:,:)
-  type(c_ptr) :: ctx, blk
+  type(c_ptr) :: ctx
+  type(c_ptr), allocatable :: pieces(:,:,:)
+  integer(c_int64_t), allocatable :: px(:), py(:), pz(:)
+  integer(c_int64_t) :: first(3)
+  integer :: a, b, c
   real(dp) :: regions(11, 4), time_out
   integer(c_int) :: vars(5), nvars
   integer(c_int64_t) :: snaps(4), step0
@@ -151,9 +159,42 @@ program fvm_driver
   call fvm_check(fvm_scheme_set(ctx, recon_c, lim_c, flux_c, rk_c, kappa, entropy_fix, &
                                 gamma, 1_c_int), "scheme_set")
   call fvm_check(fvm_set_numerics(ctx, mode), "set_numerics")
-  call fvm_check(fvm_block_create(ctx, int(ni, c_int64_t), int(nj, c_int64_t), &
-                                  int(nk, c_int64_t), dx, dy, dz, blk), "block_create")
-  call fvm_check(fvm_block_init(blk, FVM_IC_REGIONS, 44_c_int, regions, c_null_ptr), "init")
+
+!...The grid as one block, or cut into ncut(1) x ncut(2) x ncut(3) pieces
+!   (ClassGeomBlock%cut_uniform, src/base/defs_block_geom.f90:757-769) that
+!   share their interface vertex planes; each piece gets the initial state
+!   at its offset in the whole grid
+  if (any(ncut < 1) .or. (nk == 1 .and. ncut(3) /= 1)) then
+    write(*, '(a)') "invalid ncut"
+    error stop 2
+  end if
+  allocate(px(0:ncut(1)), py(0:ncut(2)), pz(0:ncut(3)))
+  call fvm_check(fvm_cut_uniform(1_c_int64_t, int(ni, c_int64_t), int(ncut(1), c_int), px), "cut i")
+  call fvm_check(fvm_cut_uniform(1_c_int64_t, int(nj, c_int64_t), int(ncut(2), c_int), py), "cut j")
+  if (nk == 1) then
+    pz = 1
+  else
+    call fvm_check(fvm_cut_uniform(1_c_int64_t, int(nk, c_int64_t), int(ncut(3), c_int), pz), "cut k")
+  end if
+  allocate(pieces(ncut(1), ncut(2), ncut(3)))
+  do c = 1, ncut(3)
+    do b = 1, ncut(2)
+      do a = 1, ncut(1)
+        call fvm_check(fvm_block_create(ctx, px(a) - px(a-1) + 1, py(b) - py(b-1) + 1, &
+                                        pz(c) - pz(c-1) + 1, dx, dy, dz, pieces(a, b, c)), &
+                       "block_create")
+        first = [px(a-1) - 1, py(b-1) - 1, pz(c-1) - 1]
+        call fvm_check(fvm_block_init_at(pieces(a, b, c), FVM_IC_REGIONS, 44_c_int, regions, &
+                                         first), "init")
+        if (a > 1) call fvm_check(fvm_block_connect(pieces(a-1, b, c), 1_c_int, &
+                                                    pieces(a, b, c), 1_c_int), "connect i")
+        if (b > 1) call fvm_check(fvm_block_connect(pieces(a, b-1, c), 3_c_int, &
+                                                    pieces(a, b, c), 1_c_int), "connect j")
+        if (c > 1) call fvm_check(fvm_block_connect(pieces(a, b, c-1), 5_c_int, &
+                                                    pieces(a, b, c), 1_c_int), "connect k")
+      end do
+    end do
+  end do
 
 !...Output variables of &Output (src/solve/class_output.f90:337-374)
   nvars = 0
@@ -167,7 +208,8 @@ program fvm_driver
   time_out = 0.0_dp
 
 !...Time loop (src/process/solver_sub.f90:196-206)
-  write(*, '(a, a, a, i0, a, i0, a, i0)') "case ", trim(name), ": ", ni, " x ", nj, " x ", nk
+  write(*, '(a, a, a, i0, a, i0, a, i0, a, i0)') "case ", trim(name), ": ", ni, " x ", nj, " x ", nk, &
+    ", blocks ", size(pieces)
   write(*, '(a8, a14, 5a14)') "Step", "Time", "R_rho", "R_momx", "R_momy", "R_energy", "R_total"
   time = 0.0_dp
   step = 0
@@ -222,7 +264,13 @@ program fvm_driver
       call fvm_check(fvm_snapshot_release(ctx, sid), "checkpoint write")
     end block
   end if
-  call fvm_check(fvm_download(blk, c_loc(q), int(NG, c_int)), "download")
+  do c = 1, ncut(3)
+    do b = 1, ncut(2)
+      do a = 1, ncut(1)
+        call gather_piece(a, b, c)
+      end do
+    end do
+  end do
   if (len_trim(output_file) > 0) then
     open(newunit=unit, file=trim(output_file), access="stream", form="unformatted", &
          status="replace", action="write")
@@ -271,5 +319,30 @@ contains
     snaps(nsnap) = sid
     time_out = time_out + output_time
   end subroutine write_output
+
+  !> Download piece (a, b, c) with its ghost layers and place it in the
+  !! whole-grid array q: its vertices, and its ghost layers on the faces of
+  !! the whole grid (ghosts across a cut belong to the neighbour's vertices)
+  subroutine gather_piece(a, b, c)
+    integer, intent(in) :: a, b, c
+    real(dp), allocatable, target :: w(:,:,:,:)
+    integer :: n(3), lo(3), hi(3), off(3), kg
+    n = [int(px(a) - px(a-1)) + 1, int(py(b) - py(b-1)) + 1, int(pz(c) - pz(c-1)) + 1]
+    kg = NG
+    if (nk == 1) kg = 0
+    allocate(w(n(1) + 2*NG, n(2) + 2*NG, n(3) + 2*kg, 5))
+    call fvm_check(fvm_download(pieces(a, b, c), c_loc(w), int(NG, c_int)), "download")
+    off = [int(px(a-1)) - 1, int(py(b-1)) - 1, int(pz(c-1)) - 1]
+    lo = [NG + 1, NG + 1, kg + 1]
+    hi = [NG + n(1), NG + n(2), kg + n(3)]
+    if (a == 1) lo(1) = 1
+    if (b == 1) lo(2) = 1
+    if (c == 1) lo(3) = 1
+    if (a == ncut(1)) hi(1) = n(1) + 2*NG
+    if (b == ncut(2)) hi(2) = n(2) + 2*NG
+    if (c == ncut(3)) hi(3) = n(3) + 2*kg
+    q(lo(1) + off(1):hi(1) + off(1), lo(2) + off(2):hi(2) + off(2), &
+      lo(3) + off(3):hi(3) + off(3), :) = w(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), :)
+  end subroutine gather_piece
 
 end program fvm_driver

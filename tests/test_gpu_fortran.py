@@ -126,3 +126,38 @@ def test_fortran_checkpoint_restart(tmp_path):
     s3, t3, q3 = run("whole", max_step=20)
     assert s2 == s3 == 20 and t2 == t3
     assert np.array_equal(q2, q3)
+
+
+@pytest.mark.parametrize("dim,ncut", [("2D", "3, 2, 1"), ("2D", "2, 4, 1"), ("3D", "2, 2, 2")])
+def test_fortran_driver_cut_equals_single_block(tmp_path, dim, ncut):
+    """The driver's ncut option: the grid run as the reference's uniform cut
+    into blocks joined across their shared vertex planes, each initialised at
+    its offset, gathered back into one array: equal bit for bit to the
+    single-block run (vertices and the ghost layers of the grid's faces)."""
+    base = open(CASE).read()
+    if dim == "3D":
+        base = base.replace('dimension     = "2D"', 'dimension     = "3D"')
+        base = base.replace("gridSize  = 101, 101", "gridSize  = 33, 29, 21")
+
+    def run(tag, extra):
+        text = base.replace('"fortran_riemann2d_101.bin"', f'"{tmp_path / tag}.bin"')
+        text = text.replace("print_every = 5", "print_every = 5" + extra)
+        nml = tmp_path / f"{tag}.nml"
+        nml.write_text(text)
+        r = subprocess.run([DRIVER, f"--input={nml}"], capture_output=True, text=True, timeout=120)
+        print(r.stdout[-600:], r.stderr[-600:])
+        assert r.returncode == 0
+        raw = open(tmp_path / f"{tag}.bin", "rb").read()
+        ni, nj, nk, ng, steps = (int(v) for v in np.frombuffer(raw[:20], dtype=np.int32))
+        kk = nk + 2 * ng if nk > 1 else 1
+        return steps, np.frombuffer(raw[28:], dtype=np.float64).reshape(5, kk, nj + 2 * ng, ni + 2 * ng)
+
+    print(f"case fortran driver {dim} ncut={ncut}", flush=True)
+    s1, q1 = run("whole", "")
+    s2, q2 = run("cut", f"\n  ncut = {ncut}")
+    assert s1 == s2 == 20
+    kin = slice(None) if dim == "2D" else slice(3, -3)
+    assert np.array_equal(q1[:, kin, 3:-3, :], q2[:, kin, 3:-3, :])
+    assert np.array_equal(q1[:, kin, :, 3:-3], q2[:, kin, :, 3:-3])
+    if dim == "3D":
+        assert np.array_equal(q1[:, :, 3:-3, 3:-3], q2[:, :, 3:-3, 3:-3])
