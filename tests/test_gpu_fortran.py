@@ -161,3 +161,30 @@ def test_fortran_driver_cut_equals_single_block(tmp_path, dim, ncut):
     assert np.array_equal(q1[:, kin, :, 3:-3], q2[:, kin, :, 3:-3])
     if dim == "3D":
         assert np.array_equal(q1[:, :, 3:-3, 3:-3], q2[:, :, 3:-3, 3:-3])
+
+
+def test_fortran_driver_3d_equals_oracle(tmp_path):
+    """The driver's 3-D path (dimension = "3D", gridSize of three entries, dz
+    = L / (nk - 1)): the four-quadrant regions (unbounded in z) initialised on
+    the device, 20 steps, bit for bit the oracle on the same grid."""
+    ni, nj, nk = 33, 29, 21
+    out = tmp_path / "q3.bin"
+    text = open(CASE).read().replace('"fortran_riemann2d_101.bin"', f'"{out}"')
+    text = text.replace('dimension     = "2D"', 'dimension     = "3D"')
+    text = text.replace("gridSize  = 101, 101", f"gridSize  = {ni}, {nj}, {nk}")
+    nml = tmp_path / "case3.nml"
+    nml.write_text(text)
+    print(f"case fortran driver 3-D {ni}x{nj}x{nk} vs oracle", flush=True)
+    r = subprocess.run([DRIVER, f"--input={nml}"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    raw = open(out, "rb").read()
+    assert tuple(int(v) for v in np.frombuffer(raw[:20], dtype=np.int32)) == (ni, nj, nk, 3, 20)
+    q = np.frombuffer(raw[28:], dtype=np.float64).reshape(5, nk + 6, nj + 6, ni + 6)
+    (dx, x), (dy, y), (dz, _) = (cases.nodal_coords(n, 1.0) for n in (ni, nj, nk))
+    p = np.zeros((nk, nj, ni, 5))
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    for (xa, xb), (ya, yb), state in cases.REF_QUADRANTS:
+        p[:, (X >= xa) & (X <= xb) & (Y >= ya) & (Y <= yb)] = state
+    o = O.oracle_solve(p, 3, (dx, dy, dz), dt=2e-4, nsteps=20, p_from_q=1)
+    assert np.array_equal(q[:, 3:-3, 3:-3, 3:-3], o["q"][:, 3:-3, 3:-3, 3:-3])
