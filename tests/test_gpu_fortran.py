@@ -188,3 +188,46 @@ def test_fortran_driver_3d_equals_oracle(tmp_path):
         p[:, (X >= xa) & (X <= xb) & (Y >= ya) & (Y <= yb)] = state
     o = O.oracle_solve(p, 3, (dx, dy, dz), dt=2e-4, nsteps=20, p_from_q=1)
     assert np.array_equal(q[:, 3:-3, 3:-3, 3:-3], o["q"][:, 3:-3, 3:-3, 3:-3])
+
+
+@pytest.mark.parametrize("names,codes", [
+    (("WENO", "none_", "Roe", "Runge-Kutta", "fixed"), (O.WENO, O.LIM_NONE, O.ROE, O.RK_SSP3)),
+    (("MUSCL", "VanLeer_", "VanLeer", "Runge-Kutta", "CFL"), (O.MUSCL, O.LIM_VANLEER, O.VANLEER, O.RK_SSP3)),
+    (("MUSCL", "vanAlbada_", "AUSM+", "RK4", "CFL"), (O.MUSCL, O.LIM_VANALBADA, O.AUSMPLUS, O.RK4)),
+    (("MUSCL", "minmod_", "AUSM", "Runge-Kutta", "fixed"), (O.MUSCL, O.LIM_MINMOD, O.AUSM, O.RK_SSP3)),
+    (("simple", "none_", "Roe", "Runge-Kutta", "CFL"), (O.SIMPLE, O.LIM_NONE, O.ROE, O.RK_SSP3)),
+], ids=["weno_roe_fixed", "muscl_vanleer_cfl", "muscl_ausmplus_rk4_cfl", "muscl_ausm_fixed",
+        "simple_roe_cfl"])
+def test_fortran_driver_scheme_names(tmp_path, names, codes):
+    """The case file's scheme names (&Inviscid_Flux reconstruct / limiter /
+    flux_split, &Time_Stepping time_scheme / timestep_set, spelled as in
+    src/defs/type_defs_scheme.f90) select the oracle's scheme: 20 steps of the
+    101^2 Riemann case, fixed dt or CFL 0.5, bit for bit."""
+    recon, lim, flux, rk, tset = names
+    out = tmp_path / "q.bin"
+    text = open(CASE).read().replace('"fortran_riemann2d_101.bin"', f'"{out}"')
+    for old, new in (('reconstruct   = "MUSCL"', f'reconstruct   = "{recon}"'),
+                     ('limiter       = "minmod_"', f'limiter       = "{lim}"'),
+                     ('flux_split    = "Roe"', f'flux_split    = "{flux}"'),
+                     ('time_scheme  = "Runge-Kutta"', f'time_scheme  = "{rk}"'),
+                     ('timestep_set = "fixed"', f'timestep_set = "{tset}"')):
+        assert old in text
+        text = text.replace(old, new)
+    nml = tmp_path / "case.nml"
+    nml.write_text(text)
+    print(f"case fortran driver scheme names {names}", flush=True)
+    r = subprocess.run([DRIVER, f"--input={nml}"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    raw = open(out, "rb").read()
+    assert int(np.frombuffer(raw[:20], dtype=np.int32)[4]) == 20
+    t_end = float(np.frombuffer(raw[20:28], dtype=np.float64)[0])
+    q = np.frombuffer(raw[28:], dtype=np.float64).reshape(5, 1, 107, 107)
+    p, dx = cases.riemann2d(101, seed=0)
+    c_recon, c_lim, c_flux, c_rk = codes
+    kw = dict(cfl=0.5, dt=0.0) if tset == "CFL" else dict(dt=2e-4)
+    o = O.oracle_solve(p, 2, (dx, dx, 1.0), recon=c_recon, limiter=c_lim, flux=c_flux, rk=c_rk,
+                       nsteps=20, p_from_q=1, **kw)
+    assert np.array_equal(q[:, 0, 3:-3, 3:-3], o["q"][:, 0, 3:-3, 3:-3])
+    if tset == "CFL":
+        assert t_end == pytest.approx(float(np.sum(o["dt"])), rel=1e-13)
