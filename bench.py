@@ -182,7 +182,7 @@ def bench_vortex(args):
     blk.upload(q)
     ctx.numerics(numerics_code(fvmgpu, args))
     if not args.no_tune:
-        ctx.autotune(reps=3)
+        ctx.autotune(reps=10)   # shapes within a few per cent: long enough windows to rank them
     tile = blk.launch_shape()
     reg = blk.launch_tile() >= 4          # the register-reconstruction kernel
     if args.warmup:
