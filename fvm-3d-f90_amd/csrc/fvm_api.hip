@@ -145,6 +145,10 @@ struct fvm_ctx {
   double time = 0.0;                      // solution time after the last step
   double t_end = HUGE_VAL;                // CFL steps stop here (halt), +inf: never
   int64_t steps_done = 0;
+  // fvm_stage: the next stage of the step in progress (1: none in progress)
+  // and that step's dt
+  int stage_next = 1;
+  double stage_dt = 0.0;
   // hipGraph replay of whole time steps (fvm_set_graph): one graph per
   // (norm partials wanted) flavour, re-captured when `gen` or the key moves
   bool use_graph = false;
@@ -606,28 +610,48 @@ int alloc_rv(fvm_block* b) {
 // on the context's stream.  Otherwise (serving) one stream-ordered allocation
 // on `alloc_stream` holds the four arrays, zeroed on that stream: no device-wide
 // synchronisation, and the memory returns to the device pool when freed.
-int block_create_impl(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
-                      double dz, hipStream_t alloc_stream, fvm_block** out) {
-  if (int rc = check_ctx(ctx)) return rc;
-  if (!out) return fail(FVM_EINVAL, "null output pointer");
+// Extents and spacing a block accepts (also checked when a served request is
+// submitted, so admission cannot meet a block it refuses).
+int check_block_dims(int64_t ni, int64_t nj, int64_t nk, double dx, double dy, double dz) {
   if (ni < 4 || nj < 4 || nk < 1)
     return fail(FVM_EINVAL, "block needs ni, nj >= 4 and nk >= 1 (got %lld %lld %lld)",
                 (long long)ni, (long long)nj, (long long)nk);
   if (ni > (1 << 24) || nj > (1 << 24) || nk > (1 << 24))
     return fail(FVM_EINVAL, "block extent too large");
   const int dim = nk == 1 ? 2 : 3;
-  if (dim == 3 && nk < 4) return fail(FVM_EINVAL, "3-D block needs nk >= 4");
+  if (dim == 3 && nk < 4) return fail(FVM_EINVAL, "3-D block needs nk >= 4 (got %lld)", (long long)nk);
   if (!(dx > 0) || !(dy > 0) || (dim == 3 && !(dz > 0)))
     return fail(FVM_EINVAL, "vertex spacing must be positive");
+  // kernels address a component with a 32-bit byte offset
+  const int64_t cs = (ni + 2 * kGhost) * (nj + 2 * kGhost) * (dim == 3 ? nk + 2 * kGhost : 1);
+  if (cs > (int64_t(1) << 29) - 64)
+    return fail(FVM_EINVAL, "block of %lld vertices per component (with ghosts) exceeds 2^29; "
+                "split the grid into connected blocks", (long long)cs);
+  return FVM_OK;
+}
+
+// Scheme codes and gamma fvm_scheme_set accepts (also checked at submit).
+int check_scheme(int recon, int limiter, int flux, int rk, double gamma) {
+  if (recon != RECON_SIMPLE && recon != RECON_MUSCL && recon != RECON_WENO)
+    return fail(FVM_EINVAL, "unknown reconstruction code %d", recon);
+  if (recon == RECON_MUSCL && limiter != LIM_NONE && limiter != LIM_MINMOD &&
+      limiter != LIM_VANLEER && limiter != LIM_VANALBADA && limiter != -1)
+    return fail(FVM_EINVAL, "unknown limiter code %d", limiter);
+  if (flux != FLUX_ROE && flux != FLUX_VANLEER && flux != FLUX_AUSM && flux != FLUX_AUSMPLUS)
+    return fail(FVM_EINVAL, "unknown flux code %d", flux);
+  if (rk != RK_SSP3 && rk != RK4) return fail(FVM_EINVAL, "unknown time scheme code %d", rk);
+  if (!(gamma > 1.0)) return fail(FVM_EINVAL, "gamma must exceed 1 (got %g)", gamma);
+  return FVM_OK;
+}
+
+int block_create_impl(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double dx, double dy,
+                      double dz, hipStream_t alloc_stream, fvm_block** out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out) return fail(FVM_EINVAL, "null output pointer");
+  if (int rc = check_block_dims(ni, nj, nk, dx, dy, dz)) return rc;
+  const int dim = nk == 1 ? 2 : 3;
   if (!ctx->blocks.empty() && ctx->blocks[0]->dim != dim)
     return fail(FVM_EINVAL, "all blocks of a context must have the same dimension");
-  {
-    // kernels address a component with a 32-bit byte offset
-    const int64_t cs = (ni + 2 * kGhost) * (nj + 2 * kGhost) * (dim == 3 ? nk + 2 * kGhost : 1);
-    if (cs > (int64_t(1) << 29) - 64)
-      return fail(FVM_EINVAL, "block of %lld vertices per component (with ghosts) exceeds 2^29; "
-                  "split the grid into connected blocks", (long long)cs);
-  }
   if (int rc = set_device(ctx)) return rc;
   fvm_block* b = new fvm_block();
   b->ctx = ctx;
@@ -676,12 +700,17 @@ int block_create_impl(fvm_ctx* ctx, int64_t ni, int64_t nj, int64_t nk, double d
 }
 
 // Host layout (5 components, host_ng ghost layers) -> the block's device
-// layout in `stage` (state_elems doubles).
-int pack_state(const fvm_block* b, const double* q, int host_ng, std::vector<double>& stage) {
+// layout in `stage` (state_elems doubles).  A 2-D block reads one k-plane:
+// host_nk == 0, the host array has that one plane and no k ghosts; host_nk
+// >= 1, it holds host_nk real planes between host_ng ghost planes on each
+// side (the reference's own allocation) and real plane k_plane (1-based) is
+// read.
+int pack_state(const fvm_block* b, const double* q, int host_ng, std::vector<double>& stage,
+               int64_t host_nk = 0, int64_t k_plane = 1) {
   const BlockGeom& g = b->g;
   const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
-  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
-  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : (host_nk > 0 ? host_nk + 2 * host_ng : 1);
+  const int64_t hzoff = b->dim == 3 ? host_ng : (host_nk > 0 ? host_ng + k_plane - 1 : 0);
   const int64_t hcs = hx * hy * hz;
   stage.assign((size_t)b->state_elems, 0.0);
   for (int c = 0; c < 5; ++c) {
@@ -690,7 +719,7 @@ int pack_state(const fvm_block* b, const double* q, int host_ng, std::vector<dou
       if (c == 3) {
         for (int64_t j = 0; j < g.ny; ++j)
           for (int64_t i = 0; i < g.nx; ++i) {
-            const double v = q[c * hcs + (j + host_ng) * hx + i + host_ng];
+            const double v = q[c * hcs + hzoff * hx * hy + (j + host_ng) * hx + i + host_ng];
             if (v != 0.0) return fail(FVM_EINVAL, "2-D block: rho*w must be zero (vertex %lld,%lld)", (long long)i, (long long)j);
           }
         continue;
@@ -708,13 +737,19 @@ int pack_state(const fvm_block* b, const double* q, int host_ng, std::vector<dou
 }
 
 // The block's device layout in `stage` -> host layout (5 components, host_ng
-// ghost layers; face ghosts only, edges and corners zero).
-void unpack_state(const fvm_block* b, const double* stage, double* q, int host_ng) {
+// ghost layers; face ghosts only, edges and corners zero).  A 2-D block with
+// host_nk >= 1 (pack_state) fills every real k-plane with its state and each
+// ghost k-plane with the real vertices of its neighbouring real plane, as
+// the reference's ghost update does along k (src/base/defs_block_data.f90:
+// 226-256; the i/j ghosts of a ghost plane are edges, never written there).
+void unpack_state(const fvm_block* b, const double* stage, double* q, int host_ng,
+                  int64_t host_nk = 0) {
   const BlockGeom& g = b->g;
   const int gl = std::min(host_ng, g.ng);
   const int64_t hx = g.nx + 2 * host_ng, hy = g.ny + 2 * host_ng;
-  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : 1;
-  const int64_t hzoff = b->dim == 3 ? host_ng : 0;
+  const bool planes = b->dim == 2 && host_nk > 0;
+  const int64_t hz = b->dim == 3 ? g.nz + 2 * host_ng : (planes ? host_nk + 2 * host_ng : 1);
+  const int64_t hzoff = b->dim == 3 || planes ? host_ng : 0;
   const int64_t hcs = hx * hy * hz;
   const int gzl = b->dim == 3 ? gl : 0;
   std::memset(q, 0, sizeof(double) * 5 * hcs);
@@ -730,6 +765,23 @@ void unpack_state(const fvm_block* b, const double* stage, double* q, int host_n
           q[c * hcs + (k + hzoff) * hx * hy + (j + host_ng) * hx + (i + host_ng)] =
               stage[dc * g.cs + (k + g.oz) * g.pxy + (j + g.oy) * g.px + (i + g.ox)];
         }
+  }
+  if (!planes) return;
+  // the first real plane (written above) into the other real planes, its
+  // real vertices into the ghost planes
+  const int64_t pl = hx * hy, src = host_ng;
+  for (int c = 0; c < 5; ++c) {
+    double* qc = q + c * hcs;
+    for (int64_t kk = 0; kk < hz; ++kk) {
+      if (kk == src) continue;
+      if (kk >= host_ng && kk < host_ng + host_nk) {
+        std::memcpy(qc + kk * pl, qc + src * pl, pl * sizeof(double));
+      } else {
+        for (int64_t j = 0; j < g.ny; ++j)
+          std::memcpy(qc + kk * pl + (j + host_ng) * hx + host_ng,
+                      qc + src * pl + (j + host_ng) * hx + host_ng, g.nx * sizeof(double));
+      }
+    }
   }
 }
 
@@ -1010,9 +1062,11 @@ int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
 // `norm_mode`: 0 no residual norms, 1 the last stage leaves its norm
 // partials, 2 only if the step reached t_end.  With timing on, each stage
 // launch is bracketed by events (ev index *nev).
-int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
+// Stage st (1-based) of one time step: its buffers, combine, norm and CFL
+// roles, then the launch over every block and the patch refill.
+int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t* nev) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
-  for (int st = 1; st <= nstage; ++st) {
+  {
     const bool last = st == nstage;
     StageArgs a{};
     a.P = ctx->P;
@@ -1046,6 +1100,44 @@ int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_patches(ctx, a.out_buf)) return rc;
   }
+  return FVM_OK;
+}
+
+// Residual norms of a step's last stage (class_iteration.f90:210-248) from
+// the per-tile partials in ctx->h_norm: summed in tile order per block;
+// blocks combined as the reference accumulates them, R = sqrt(R_prev^2 +
+// R_blk^2).  Fills ctx->norms; returns the NaN / divergence flag.
+int combine_norms(fvm_ctx* ctx, int nc) {
+  double res_abs[6] = {0, 0, 0, 0, 0, 0};
+  for (fvm_block* b : ctx->blocks) {
+    double sums[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < b->ntiles; ++w)
+      for (int c = 0; c < nc; ++c) sums[c] += ctx->h_norm[(size_t)(b->tile_base + w) * nc + c];
+    double res2[5] = {0, 0, 0, 0, 0};
+    const double npts = (double)b->g.nx * b->g.ny * b->g.nz;
+    for (int c = 0; c < nc; ++c) {
+      const int e = (nc == 4 && c == 3) ? 4 : c;   // 2-D: energy is equation 5
+      res2[e] = sums[c] / npts;
+    }
+    double res2t = 0.0;
+    for (int e = 0; e < 5; ++e) res2t += res2[e];
+    res2t = res2t / 5.0;
+    for (int e = 0; e < 5; ++e) res_abs[e] = std::sqrt(res_abs[e] * res_abs[e] + res2[e]);
+    res_abs[5] = std::sqrt(res_abs[5] * res_abs[5] + res2t);
+  }
+  int nan = 0;
+  for (int e = 0; e < 6; ++e) {
+    ctx->norms[e] = res_abs[e];
+    if (std::isnan(res_abs[e])) nan = 1;
+  }
+  if (res_abs[5] > 1.0e10) nan = 1;
+  return nan;
+}
+
+int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
+  const int nstage = ctx->rk == RK4 ? 4 : 3;
+  for (int st = 1; st <= nstage; ++st)
+    if (int rc = launch_one_stage(ctx, st, cfl_mode, norm_mode, nev)) return rc;
   return FVM_OK;
 }
 
@@ -1149,15 +1241,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
 int fvm_scheme_set(fvm_ctx* ctx, int recon, int limiter, int flux, int rk, double kappa,
                    double entropy_fix, double gamma, int ref_compat) {
   if (int rc = check_ctx(ctx)) return rc;
-  if (recon != RECON_SIMPLE && recon != RECON_MUSCL && recon != RECON_WENO)
-    return fail(FVM_EINVAL, "unknown reconstruction code %d", recon);
-  if (recon == RECON_MUSCL && limiter != LIM_NONE && limiter != LIM_MINMOD &&
-      limiter != LIM_VANLEER && limiter != LIM_VANALBADA && limiter != -1)
-    return fail(FVM_EINVAL, "unknown limiter code %d", limiter);
-  if (flux != FLUX_ROE && flux != FLUX_VANLEER && flux != FLUX_AUSM && flux != FLUX_AUSMPLUS)
-    return fail(FVM_EINVAL, "unknown flux code %d", flux);
-  if (rk != RK_SSP3 && rk != RK4) return fail(FVM_EINVAL, "unknown time scheme code %d", rk);
-  if (!(gamma > 1.0)) return fail(FVM_EINVAL, "gamma must exceed 1 (got %g)", gamma);
+  if (int rc = check_scheme(recon, limiter, flux, rk, gamma)) return rc;
   ctx->recon = recon;
   ctx->flux = flux;
   ctx->rk = rk;
@@ -1474,12 +1558,43 @@ int fvm_upload(fvm_block* b, const double* q, int host_ng) {
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   b->uploaded = true;
   ctx->ghosts_dirty = true;
+  ctx->stage_next = 1;
   return FVM_OK;
 }
 
-int fvm_download(fvm_block* b, double* q, int host_ng) {
+int fvm_upload_planes(fvm_block* b, const double* q, int host_ng, int64_t host_nk, int64_t k_plane) {
   if (!b || !q) return fail(FVM_EINVAL, "null block or host array");
   if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
+  if (b->dim == 3) {
+    if (host_nk != b->g.nz)
+      return fail(FVM_EINVAL, "3-D block: host_nk %lld must equal nk %d", (long long)host_nk, b->g.nz);
+    return fvm_upload(b, q, host_ng);
+  }
+  if (host_nk < 1 || host_nk > (1 << 20)) return fail(FVM_EINVAL, "host_nk %lld out of range", (long long)host_nk);
+  if (k_plane < 1 || k_plane > host_nk)
+    return fail(FVM_EINVAL, "k_plane %lld outside 1..%lld", (long long)k_plane, (long long)host_nk);
+  fvm_ctx* ctx = b->ctx;
+  if (int rc = set_device(ctx)) return rc;
+  std::vector<double> stage;
+  if (int rc = pack_state(b, q, host_ng, stage, host_nk, k_plane)) return rc;
+  HIP_TRY(hipMemcpyAsync(b->q[0], stage.data(), b->state_elems * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  b->uploaded = true;
+  ctx->ghosts_dirty = true;
+  ctx->stage_next = 1;
+  return FVM_OK;
+}
+
+int fvm_download(fvm_block* b, double* q, int host_ng) { return fvm_download_planes(b, q, host_ng, 0); }
+
+int fvm_download_planes(fvm_block* b, double* q, int host_ng, int64_t host_nk) {
+  if (!b || !q) return fail(FVM_EINVAL, "null block or host array");
+  if (host_ng < 0 || host_ng > 16) return fail(FVM_EINVAL, "host_ng %d out of range", host_ng);
+  if (b->dim == 3 && host_nk != 0 && host_nk != b->g.nz)
+    return fail(FVM_EINVAL, "3-D block: host_nk %lld must equal nk %d", (long long)host_nk, b->g.nz);
+  if (b->dim == 2 && (host_nk < 0 || host_nk > (1 << 20)))
+    return fail(FVM_EINVAL, "host_nk %lld out of range", (long long)host_nk);
   fvm_ctx* ctx = b->ctx;
   if (int rc = set_device(ctx)) return rc;
   if ((ctx->ghosts_dirty || (ctx->ghosts_stale && host_ng > 0)) && b->uploaded) {
@@ -1496,7 +1611,7 @@ int fvm_download(fvm_block* b, double* q, int host_ng) {
   HIP_TRY(hipMemcpyAsync(stage.data(), b->q[0], b->state_elems * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  unpack_state(b, stage.data(), q, host_ng);
+  unpack_state(b, stage.data(), q, host_ng, b->dim == 2 ? host_nk : 0);
   return FVM_OK;
 }
 
@@ -1507,6 +1622,8 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   if (nsteps < 0) return fail(FVM_EINVAL, "nsteps must be >= 0");
   const bool cfl_mode = !(dt > 0.0);
   if (cfl_mode && !(cfl > 0.0)) return fail(FVM_EINVAL, "dt <= 0 needs cfl > 0");
+  if (ctx->stage_next != 1)
+    return fail(FVM_ESTATE, "a step begun with fvm_stage is at stage %d", ctx->stage_next);
   for (fvm_block* b : ctx->blocks)
     if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
   if (int rc = set_device(ctx)) return rc;
@@ -1611,29 +1728,7 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     }
     ctx->timing_step += nsteps;
   }
-  double res_abs[6] = {0, 0, 0, 0, 0, 0};
-  for (fvm_block* b : ctx->blocks) {
-    double sums[5] = {0, 0, 0, 0, 0};
-    for (int w = 0; w < b->ntiles; ++w)
-      for (int c = 0; c < nc; ++c) sums[c] += ctx->h_norm[(size_t)(b->tile_base + w) * nc + c];
-    double res2[5] = {0, 0, 0, 0, 0};
-    const double npts = (double)b->g.nx * b->g.ny * b->g.nz;
-    for (int c = 0; c < nc; ++c) {
-      const int e = (nc == 4 && c == 3) ? 4 : c;   // 2-D: energy is equation 5
-      res2[e] = sums[c] / npts;
-    }
-    double res2t = 0.0;
-    for (int e = 0; e < 5; ++e) res2t += res2[e];
-    res2t = res2t / 5.0;
-    for (int e = 0; e < 5; ++e) res_abs[e] = std::sqrt(res_abs[e] * res_abs[e] + res2[e]);
-    res_abs[5] = std::sqrt(res_abs[5] * res_abs[5] + res2t);
-  }
-  int nan = 0;
-  for (int e = 0; e < 6; ++e) {
-    ctx->norms[e] = res_abs[e];
-    if (std::isnan(res_abs[e])) nan = 1;
-  }
-  if (res_abs[5] > 1.0e10) nan = 1;
+  int nan = combine_norms(ctx, nc);
   if (cfl_mode && ctx->h_slot.nan) nan = 1;
   if (ran == 0) {
     // the run had already reached its end time: nothing new to report
@@ -1643,6 +1738,59 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
   ctx->nan_flag = nan;
   ctx->have_norms = true;
   if (cfl_mode && ctx->h_slot.nan) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
+  return FVM_OK;
+}
+
+// One Runge-Kutta stage, the body of the stage loop of
+// ClassTimeDiscret%Runge_Kutta_sub (src/solve/class_time_discret.f90:
+// 172-195): residual of the stage input, combine, ghosts.  Stage 1 takes the
+// current u^n as the step's base state (reset_q0, :170); the last stage
+// writes u^(n+1), advances the solution time by dt and leaves the norms.
+int fvm_stage(fvm_ctx* ctx, int s, double dt) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
+  const int nstage = ctx->rk == RK4 ? 4 : 3;
+  if (s < 1 || s > nstage) return fail(FVM_EINVAL, "stage %d out of range 1..%d", s, nstage);
+  if (s != ctx->stage_next) return fail(FVM_ESTATE, "stage %d called, stage %d expected", s, ctx->stage_next);
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FVM_EINVAL, "fvm_stage needs a fixed dt > 0");
+  if (s > 1 && dt != ctx->stage_dt)
+    return fail(FVM_EINVAL, "dt %.17g differs from the step's dt %.17g", dt, ctx->stage_dt);
+  for (fvm_block* b : ctx->blocks)
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
+  if (int rc = set_device(ctx)) return rc;
+  const int nc = ctx->blocks[0]->nc;
+  const size_t need = (size_t)ctx->total_tiles * nc;
+  if (s == 1) {
+    ctx->have_norms = false;
+    if (int rc = sync_desc(ctx)) return rc;
+    if (ctx->ghosts_dirty) {
+      if (int rc = fill_ghosts_all(ctx, 0)) return rc;
+      ctx->ghosts_dirty = false;
+    }
+    if (int rc = ensure_norm_capacity(ctx, (size_t)ctx->total_tiles * nc)) return rc;
+    hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot, dt);
+    HIP_TRY(hipGetLastError());
+    ctx->stage_dt = dt;
+    ctx->ghosts_stale = true;
+  } else if (ctx->desc_dirty) {
+    return fail(FVM_ESTATE, "blocks or launch shapes changed within a step");
+  }
+  if (int rc = launch_one_stage(ctx, s, false, s == nstage ? 1 : 0, nullptr)) return rc;
+  if (s < nstage) {
+    ctx->stage_next = s + 1;
+    HIP_TRY(hipGetLastError());
+    return FVM_OK;
+  }
+  ctx->stage_next = 1;
+  ctx->h_norm.resize(need);
+  HIP_TRY(hipMemcpyAsync(ctx->h_norm.data(), ctx->d_norm, need * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->time = ctx->time + dt;
+  ctx->steps_done += 1;
+  ctx->nan_flag = combine_norms(ctx, nc);
+  ctx->have_norms = true;
   return FVM_OK;
 }
 
@@ -1712,6 +1860,8 @@ int fvm_norms(fvm_ctx* ctx, double out[6], int* nan_flag) {
 int fvm_autotune(fvm_ctx* ctx, int reps) {
   if (int rc = check_ctx(ctx)) return rc;
   if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (ctx->stage_next != 1)
+    return fail(FVM_ESTATE, "a step begun with fvm_stage is at stage %d", ctx->stage_next);
   if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
   for (fvm_block* b : ctx->blocks)
     if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
@@ -1919,6 +2069,7 @@ int fvm_block_init_at(fvm_block* b, int kind, int nparams, const double* params,
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   b->uploaded = true;
   ctx->ghosts_dirty = true;
+  ctx->stage_next = 1;
   return FVM_OK;
 }
 
@@ -1934,6 +2085,7 @@ int fvm_block_init(fvm_block* b, int kind, int nparams, const double* params, co
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   b->uploaded = true;
   ctx->ghosts_dirty = true;
+  ctx->stage_next = 1;
   return FVM_OK;
 }
 
@@ -2223,7 +2375,9 @@ struct Job {
   int steps = 0;
   double t_final = 0;
   double cell_updates = 0;
-  int status = 0;                  // 0 pending, 1 active, 2 done, -1 diverged, -2 not staged
+  int status = 0;                  // 0 pending, 1 active, 2 done, -1 diverged, -2 not staged,
+                                   // -3 refused at admission (err says why)
+  std::string err;
   int slot = -1;
   int64_t first_batch = 0;         // first batch that advanced this solve (0: none yet)
   fvm_ctx* group = nullptr;
@@ -2408,6 +2562,77 @@ int server_sync_desc(fvm_server* s, fvm_ctx* g, int parity) {
   return FVM_OK;
 }
 
+// One request's admission: its group, its block (state from the device pool
+// on the copy stream), the initial state, a slot.  On failure nothing of it
+// is left in the group and the slot is not taken.
+int server_admit_one(fvm_server* s, Job* j) {
+  fvm_ctx* g = nullptr;
+  if (int rc = server_group(s, j->req, &g)) return rc;
+  fvm_block* b = nullptr;
+  const fvm_request& r = j->req;
+  if (int rc = block_create_impl(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2],
+                                 s->cstream, &b))
+    return rc;
+  auto drop = [&](int rc) {
+    g->blocks.erase(std::remove(g->blocks.begin(), g->blocks.end(), b), g->blocks.end());
+    g->desc_dirty = true;
+    free_block_state(b, s->cstream);
+    delete b;
+    return rc;
+  };
+  // one tile shape per group so its blocks share a launch (WENO: 16 x 16,
+  // the shape the autotuner picks for it on 32-128 vertex blocks; otherwise
+  // 32 x 8); long k-marches (the group's launch has tiles enough without
+  // splitting each block)
+  if (b->dim == 3) plan_launch(b, r.recon == RECON_WENO ? 3 : 1, s->chunk3);
+  else plan_launch(b, -1, s->chunk2);
+  for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
+    if (r.bc[f] == FVM_BC_PERIODIC)
+      if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return drop(rc);
+  // the compact state goes to the stage array q[1] (unused until the first
+  // step) and a kernel scatters it into q[0]; the host copies nothing.  An
+  // initial-state spec is computed into q[0] instead.
+  const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
+  hipError_t e = hipSuccess;
+  if (r.ic_kind != 0) {
+    InitSpec spec;
+    if (int rc = make_init_spec(b->dim, r.ic_kind, r.ic_nparams, r.ic_params, r.ic_origin, r.gamma, &spec))
+      return drop(rc);
+    launch_init(b, spec, s->cstream);
+  } else if (b->dim == 3) {
+    e = hipMemcpyAsync(b->q[1], j->q0, 5 * ncell * sizeof(double), hipMemcpyHostToDevice, s->cstream);
+  } else {   // 2-D: rho, rho u, rho v, E (rho w is zero)
+    e = hipMemcpyAsync(b->q[1], j->q0, 3 * ncell * sizeof(double), hipMemcpyHostToDevice, s->cstream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->q[1] + 3 * ncell, j->q0 + 4 * ncell, ncell * sizeof(double),
+                         hipMemcpyHostToDevice, s->cstream);
+  }
+  if (e == hipSuccess && r.ic_kind == 0)
+    hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[1], b->q[0], b->g,
+                       b->nc, 1);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) return drop(fail(FVM_EHIP, "staging a served state: %s", hipGetErrorString(e)));
+  b->uploaded = true;
+  const int slot = s->free_slots.back();
+  s->free_slots.pop_back();
+  SolveSlot& init = s->h_init[slot];
+  init = SolveSlot{};
+  init.t_end = r.t_end;
+  init.cfl = r.cfl;
+  b->dt_dev = &s->d_slots[slot].dt;
+  b->smax_dev = &s->d_slots[slot].smax;
+  b->halt_dev = &s->d_slots[slot].halt;
+  g->desc_dirty = true;
+  j->slot = slot;
+  j->group = g;
+  j->blk = b;
+  j->status = 1;
+  j->t_admit = now_s(s);
+  s->used += (double)j->bytes;
+  s->active.push_back(j);
+  return FVM_OK;
+}
+
 // Admit pending jobs in arrival order while a slot and the HBM budget allow.
 // Allocation, zeroing and upload of a solve's state run on the copy stream;
 // its group's stream waits for them before the solve's first step.
@@ -2425,62 +2650,18 @@ int server_admit(fvm_server* s) {
     }
     if (s->free_slots.empty() || s->used + (double)j->bytes > s->budget) break;
     s->pending.pop_front();
-    fvm_ctx* g = nullptr;
-    if (int rc = server_group(s, j->req, &g)) return rc;
-    fvm_block* b = nullptr;
-    const fvm_request& r = j->req;
-    if (int rc = block_create_impl(g, r.n[0], r.n[1], r.n[2], r.spacing[0], r.spacing[1], r.spacing[2],
-                                   s->cstream, &b))
-      return rc;
-    // one tile shape per group so its blocks share a launch (WENO: 16 x 16,
-    // the shape the autotuner picks for it on 32-128 vertex blocks; otherwise
-    // 32 x 8); long k-marches (the group's launch has tiles enough without
-    // splitting each block)
-    if (b->dim == 3) plan_launch(b, r.recon == RECON_WENO ? 3 : 1, s->chunk3);
-    else plan_launch(b, -1, s->chunk2);
-    for (int f = 0; f < (r.n[2] == 1 ? 4 : 6); ++f)
-      if (r.bc[f] == FVM_BC_PERIODIC)
-        if (int rc = fvm_block_set_bc(b, f, FVM_BC_PERIODIC)) return rc;
-    // the compact state goes to the stage array q[1] (unused until the first
-    // step) and a kernel scatters it into q[0]; the host copies nothing.  An
-    // initial-state spec is computed into q[0] instead.
-    const int64_t ncell = r.n[0] * r.n[1] * r.n[2];
-    if (r.ic_kind != 0) {
-      InitSpec spec;
-      if (int rc = make_init_spec(b->dim, r.ic_kind, r.ic_nparams, r.ic_params, r.ic_origin, r.gamma, &spec))
-        return rc;
-      launch_init(b, spec, s->cstream);
-    } else if (b->dim == 3) {
-      HIP_TRY(hipMemcpyAsync(b->q[1], j->q0, 5 * ncell * sizeof(double), hipMemcpyHostToDevice,
-                             s->cstream));
-    } else {   // 2-D: rho, rho u, rho v, E (rho w is zero)
-      HIP_TRY(hipMemcpyAsync(b->q[1], j->q0, 3 * ncell * sizeof(double), hipMemcpyHostToDevice,
-                             s->cstream));
-      HIP_TRY(hipMemcpyAsync(b->q[1] + 3 * ncell, j->q0 + 4 * ncell, ncell * sizeof(double),
-                             hipMemcpyHostToDevice, s->cstream));
+    if (server_admit_one(s, j) != FVM_OK) {
+      // this request alone fails (its group or block could not be made: out
+      // of device memory, say); the ones admitted before it still start
+      j->status = -3;
+      j->err = g_err;
+      j->t_done = now_s(s);
+      if (j->q0) {
+        pin_put(s, j->q0, j->q0_cap);
+        j->q0 = nullptr;
+      }
+      continue;
     }
-    if (r.ic_kind == 0)
-      hipLaunchKernelGGL(compact_kernel, dim3(1024), dim3(256), 0, s->cstream, b->q[1], b->q[0], b->g,
-                         b->nc, 1);
-    HIP_TRY(hipGetLastError());
-    b->uploaded = true;
-    const int slot = s->free_slots.back();
-    s->free_slots.pop_back();
-    SolveSlot& init = s->h_init[slot];
-    init = SolveSlot{};
-    init.t_end = r.t_end;
-    init.cfl = r.cfl;
-    b->dt_dev = &s->d_slots[slot].dt;
-    b->smax_dev = &s->d_slots[slot].smax;
-    b->halt_dev = &s->d_slots[slot].halt;
-    g->desc_dirty = true;
-    j->slot = slot;
-    j->group = g;
-    j->blk = b;
-    j->status = 1;
-    j->t_admit = now_s(s);
-    s->used += (double)j->bytes;
-    s->active.push_back(j);
     admitted.push_back(j);
   }
   if (admitted.empty()) return FVM_OK;
@@ -2723,7 +2904,12 @@ int fvm_server_destroy(fvm_server* s) {
 int fvm_server_submit(fvm_server* s, const fvm_request* r, int64_t* id) {
   if (!s || !r) return fail(FVM_EINVAL, "null server or request");
   if (!r->q0 && r->ic_kind == 0) return fail(FVM_EINVAL, "null initial state");
-  if (r->n[0] < 4 || r->n[1] < 4 || r->n[2] < 1) return fail(FVM_EINVAL, "bad grid size");
+  // everything admission checks, so a request it would refuse is refused here
+  if (int rc = check_block_dims(r->n[0], r->n[1], r->n[2], r->spacing[0], r->spacing[1], r->spacing[2]))
+    return rc;
+  if (int rc = check_scheme(r->recon, r->limiter, r->flux, r->rk, r->gamma)) return rc;
+  if (r->numerics != FVM_NUMERICS_EXACT && r->numerics != FVM_NUMERICS_FAST)
+    return fail(FVM_EINVAL, "unknown numerics mode %d", r->numerics);
   if (!(r->t_end > 0.0) || !(r->cfl > 0.0)) return fail(FVM_EINVAL, "t_end and cfl must be positive");
   for (int f = 0; f < 6; ++f) {
     const BcClass cls = bc_class(r->bc[f]);
@@ -2817,6 +3003,8 @@ int fvm_server_result(fvm_server* s, int64_t id, double* q_out, double info[8]) 
   if (j->status == 0 || j->status == 1) return fail(FVM_ESTATE, "request %lld not finished", (long long)id);
   if (j->status == -2)
     return fail(FVM_ENOMEM, "request %lld: no pinned host memory to stage its state", (long long)id);
+  if (j->status == -3)
+    return fail(FVM_EINVAL, "request %lld refused at admission: %s", (long long)id, j->err.c_str());
   if (j->status < 0) return fail(FVM_ENAN, "request %lld diverged (NaN or non-positive dt)", (long long)id);
   if (j->res) {   // first call: wait for the download, keep the state, release the pinned copy
     HIP_TRY(hipEventSynchronize(j->res_ev));

@@ -48,6 +48,7 @@ module fvm_gpu
   public :: fvm_block_create, fvm_block_destroy, fvm_block_set_bc, fvm_block_set_bc_state
   public :: fvm_block_init, fvm_block_init_at, fvm_cut_uniform, fvm_block_launch_tile
   public :: fvm_upload, fvm_download, fvm_step, fvm_norms, fvm_sync
+  public :: fvm_upload_planes, fvm_download_planes, fvm_stage
   public :: fvm_block_connect, fvm_conn_add, fvm_viscous_set, fvm_set_time, fvm_time, fvm_autotune
   public :: fvm_set_launch_mode, fvm_set_graph
   public :: fvm_snapshot_begin, fvm_snapshot_ready, fvm_snapshot_read, fvm_snapshot_save
@@ -146,6 +147,38 @@ module fvm_gpu
       integer(c_int), value :: host_ng
       integer(c_int) :: rc
     end function fvm_download
+
+    !> ClassBlockData%q as the reference allocates it (defs_block_data.f90:
+    !> 137-141), q(ni+2ng, nj+2ng, host_nk+2ng, 5); a 2-D block reads real
+    !> plane k_plane and writes every plane back
+    function fvm_upload_planes(blk, q, host_ng, host_nk, k_plane) &
+        bind(C, name="fvm_upload_planes") result(rc)
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: blk
+      type(c_ptr), value :: q
+      integer(c_int), value :: host_ng
+      integer(c_int64_t), value :: host_nk, k_plane
+      integer(c_int) :: rc
+    end function fvm_upload_planes
+
+    function fvm_download_planes(blk, q, host_ng, host_nk) &
+        bind(C, name="fvm_download_planes") result(rc)
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: blk
+      type(c_ptr), value :: q
+      integer(c_int), value :: host_ng
+      integer(c_int64_t), value :: host_nk
+      integer(c_int) :: rc
+    end function fvm_download_planes
+
+    !> One Runge-Kutta stage (the stage loop of class_time_discret.f90:172-195)
+    function fvm_stage(ctx, s, dt) bind(C, name="fvm_stage") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: s
+      real(c_double), value :: dt
+      integer(c_int) :: rc
+    end function fvm_stage
 
     function fvm_step(ctx, nsteps, dt, cfl, dt_last) bind(C, name="fvm_step") result(rc)
       import :: c_int, c_ptr, c_double

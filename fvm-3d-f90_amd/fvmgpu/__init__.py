@@ -57,7 +57,8 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_set_graph", "fvm_conn_add", "fvm_snapshot_begin", "fvm_snapshot_ready",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
            "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform",
-           "fvm_block_launch_tile", "fvm_block_init_at")
+           "fvm_block_launch_tile", "fvm_block_init_at", "fvm_upload_planes",
+           "fvm_download_planes", "fvm_stage")
 
 
 class FvmError(RuntimeError):
@@ -102,6 +103,9 @@ def load_library(path=LIB_PATH):
         "fvm_block_set_bc": ([vp, ci, ci], ci),
         "fvm_upload": ([vp, vp, ci], ci),
         "fvm_download": ([vp, vp, ci], ci),
+        "fvm_upload_planes": ([vp, vp, ci, i64, i64], ci),
+        "fvm_download_planes": ([vp, vp, ci, i64], ci),
+        "fvm_stage": ([vp, ci, cd], ci),
         "fvm_step": ([vp, ci, cd, cd, ctypes.POINTER(cd)], ci),
         "fvm_norms": ([vp, ctypes.POINTER(cd), ctypes.POINTER(ci)], ci),
         "fvm_sync": ([vp], ci),
@@ -232,6 +236,20 @@ class Block:
         _check(_lib.fvm_download(self._h, q.ctypes.data, ng))
         return q
 
+    def upload_planes(self, q, ng, nk_host, k_plane=1):
+        """q: (5, nk_host + 2ng, nj + 2ng, ni + 2ng), the reference's own
+        allocation of ClassBlockData%q; a 2-D block reads real plane k_plane."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        want = (5, nk_host + 2 * ng, self.nj + 2 * ng, self.ni + 2 * ng)
+        if q.shape != want:
+            raise FvmError(f"upload shape {q.shape} != {want}")
+        _check(_lib.fvm_upload_planes(self._h, q.ctypes.data, ng, int(nk_host), int(k_plane)))
+
+    def download_planes(self, ng, nk_host):
+        q = np.zeros((5, nk_host + 2 * ng, self.nj + 2 * ng, self.ni + 2 * ng))
+        _check(_lib.fvm_download_planes(self._h, q.ctypes.data, ng, int(nk_host)))
+        return q
+
 
 class Context:
     """A device context: one stream and the blocks created on it."""
@@ -318,6 +336,10 @@ class Context:
         out = ctypes.c_double(0.0)
         _check(_lib.fvm_step(self._h, int(nsteps), float(dt), float(cfl), ctypes.byref(out)))
         return out.value
+
+    def stage(self, s, dt):
+        """Runge-Kutta stage s (1-based) of the step in progress, fixed dt."""
+        _check(_lib.fvm_stage(self._h, int(s), float(dt)))
 
     def viscous(self, mu_ref, t_ref=1.0, sutherland_s=0.0, prandtl=0.72, r_gas=1.0):
         """Central viscous fluxes with Sutherland's law; mu_ref <= 0: off."""

@@ -242,6 +242,22 @@ int fvm_block_init_at(fvm_block *blk, int kind, int nparams, const double *param
 int fvm_upload(fvm_block *blk, const double *q, int host_ng);
 int fvm_download(fvm_block *blk, double *q, int host_ng);
 
+/* The same copies for a host array in the reference's own allocation,
+ * ClassBlockData%q (src/base/defs_block_data.f90:137-141):
+ * q(ni + 2 host_ng, nj + 2 host_ng, host_nk + 2 host_ng, 5), host_nk the
+ * reference's real k-planes.  Its 2-D grid has two identical k-planes
+ * (gsize(3) = 2, src/base/method_gridgen.f90:98-99), so its q holds 2 + 6
+ * planes; the device block is the 2-D block (nk == 1).  Upload reads real
+ * plane k_plane (1-based among the host_nk real planes; its rho*w must be
+ * zero).  Download writes the state into every real plane and the real
+ * vertices of the boundary planes into the host_ng ghost planes on each
+ * side (the reference's k ghost update, defs_block_data.f90:226-256; edges
+ * and corners zero).  A 3-D block needs host_nk == nk and then equals
+ * fvm_upload / fvm_download (host_nk 0 is accepted by download too). */
+int fvm_upload_planes(fvm_block *blk, const double *q, int host_ng, int64_t host_nk,
+                      int64_t k_plane);
+int fvm_download_planes(fvm_block *blk, double *q, int host_ng, int64_t host_nk);
+
 /* nsteps full time steps on every block of the context
  * (ClassTimeDiscret%solve, src/solve/class_time_discret.f90:125-156).
  *   dt > 0         fixed time step (the reference's "fixed" timestep_set)
@@ -251,6 +267,18 @@ int fvm_download(fvm_block *blk, double *q, int host_ng);
  * dt_last (nullable) receives the last step's dt.  Residual norms of the
  * last stage of the last step are kept for fvm_norms. */
 int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
+
+/* One Runge-Kutta stage s (1..3 SSP-RK3, 1..4 RK4) with a fixed dt > 0: the
+ * body of the stage loop of ClassTimeDiscret%Runge_Kutta_sub
+ * (src/solve/class_time_discret.f90:172-195: residual, combine, q -> p and
+ * ghosts).  Stages are called in order with the same dt; stage 1 takes the
+ * current state as the step's base state (reset_q0, :170), the last stage
+ * writes u^(n+1), advances the solution time by dt and leaves the norms for
+ * fvm_norms.  The stage values live in device work arrays, so between
+ * stages fvm_download returns u^n.  Results equal fvm_step's bit for bit.
+ * Out of order, or fvm_step / fvm_autotune while a step is in progress:
+ * FVM_ESTATE; an upload or device initial state abandons the step. */
+int fvm_stage(fvm_ctx *ctx, int s, double dt);
 
 /* Central second-order viscous fluxes with Sutherland's law (no reference
  * counterpart: viscous_flux_sub, src/solve/class_space_discret.f90:671-675,

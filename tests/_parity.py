@@ -38,6 +38,28 @@ def run_oracle(p, spacing, nsteps, dt=0.0, cfl=0.0, bc=None, recon=O.MUSCL,
     return q, r
 
 
+NG = 3   # ghost layers of the reference's arrays (src/base/defs_ghost_cell.f90:42)
+
+
+def reference_q_2d(p):
+    """The reference's q(ni+6, nj+6, 8, 5) (as a C array [5][8][nj+6][ni+6])
+    after flowfield%initial: conserved state on both real planes, face ghosts
+    copied from the boundary vertex in i, j and k (update_ghost_cell,
+    src/base/defs_block_data.f90:226-256), edges and corners never written."""
+    q = cases.to_conserved(p)[:, 0]                      # (5, nj, ni)
+    nj, ni = q.shape[1:]
+    H = np.zeros((5, 2 + 2 * NG, nj + 2 * NG, ni + 2 * NG))
+    for kk in range(2 + 2 * NG):
+        H[:, kk, NG:NG + nj, NG:NG + ni] = q
+    for kk in (NG, NG + 1):
+        for g in range(NG):
+            H[:, kk, NG:NG + nj, g] = q[:, :, 0]
+            H[:, kk, NG:NG + nj, NG + ni + g] = q[:, :, -1]
+            H[:, kk, g, NG:NG + ni] = q[:, 0, :]
+            H[:, kk, NG + nj + g, NG:NG + ni] = q[:, -1, :]
+    return H
+
+
 def rel_linf(a, b):
     """Relative L-inf per conserved component, normalised by max|b| with an
     absolute floor of 1 (rho*w is identically zero in 2-D)."""

@@ -231,3 +231,31 @@ def test_fortran_driver_scheme_names(tmp_path, names, codes):
     assert np.array_equal(q[:, 0, 3:-3, 3:-3], o["q"][:, 0, 3:-3, 3:-3])
     if tset == "CFL":
         assert t_end == pytest.approx(float(np.sum(o["dt"])), rel=1e-13)
+
+
+def test_integration_snippet_reference_layout(tmp_path):
+    """The integration notes' snippet, compiled verbatim (bin/integration_example),
+    on the reference's own 2-D case: ClassBlockData%q allocated as the
+    reference allocates it, q(n+6, n+6, 2+6, 5), the case file's scheme values
+    through ClassSpaceDiscret%input, ten fixed steps.  Every real plane equals
+    the oracle bit for bit; the ghost planes hold the real vertices."""
+    from _parity import NG, reference_q_2d
+    exe = os.path.join(ROOT, "fvm-3d-f90_amd", "bin", "integration_example")
+    assert os.path.exists(exe), "build it: make -C fvm-3d-f90_amd"
+    n, nsteps, dt = 41, 10, 2e-4
+    p, dx = cases.riemann2d(n, seed=0)
+    H = reference_q_2d(p)
+    fin, fout = tmp_path / "q0.bin", tmp_path / "q1.bin"
+    H.tofile(fin)
+    print(f"case integration snippet riemann2d {n}x{n}x2, {nsteps} steps", flush=True)
+    r = subprocess.run([exe, str(n), str(nsteps), repr(dt), str(fin), str(fout)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    q = np.fromfile(fout, dtype=np.float64).reshape(H.shape)
+    o = O.oracle_solve(p, 2, (dx, dx, 1.0), dt=dt, nsteps=nsteps, p_from_q=1)
+    want = o["q"][:, 0, NG:-NG, NG:-NG]
+    for kk in range(2 + 2 * NG):
+        assert np.array_equal(q[:, kk, NG:-NG, NG:-NG], want), f"plane {kk}"
+    norms = [float(x) for x in r.stdout.split("norms")[1].split()]
+    np.testing.assert_allclose(norms, o["res"][-1], rtol=1e-12, atol=1e-300)
