@@ -545,62 +545,21 @@ def bench_tgv(args):
 # ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
-# end times per case kind.  At 50 arrivals / s the served run ends soon after
-# the last arrival (the rate is then set by the arrivals); the capacity run
-# queues every request at t = 0 and measures what the GPU sustains.
-SERVE_T_END = {"riemann2d": 0.1, "vortex3d": 1.0, "blob3d": 0.1}
-
-
-def serving_requests(fvmgpu, cases, seed, nreq, rate):
-    """The request stream: arrival times and fvmgpu.Server.submit arguments."""
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    arrivals -= arrivals[0]
-    schemes = (dict(recon=fvmgpu.MUSCL, limiter=fvmgpu.LIM_MINMOD, flux=fvmgpu.ROE,
-                    rk=fvmgpu.RK_SSP3, ref_compat=1),
-               dict(recon=fvmgpu.WENO, limiter=fvmgpu.LIM_NONE, flux=fvmgpu.AUSMPLUS,
-                    rk=fvmgpu.RK_SSP3, ref_compat=0))
-    reqs = []
-    for i in range(nreq):
-        kind = rng.integers(3)            # 0: Riemann 2-D, 1: vortex 3-D, 2: blob 3-D
-        sch = dict(schemes[rng.integers(2)])
-        if kind == 0:
-            n = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
-            # the reference's quadrant states (Lax-Liu configuration 3): the
-            # drawn states of other seeds include near-vacuum rarefactions in
-            # which Roe's linearisation loses positivity
-            p, dx = cases.riemann2d(n, seed=0)
-            r = dict(q0=cases.to_conserved(p), spacing=(dx, dx, 1.0),
-                     t_end=SERVE_T_END["riemann2d"], cfl=0.5,
-                     label=f"riemann2d {n}^2")
-        else:
-            n = int(round(np.exp(rng.uniform(np.log(32), np.log(128)))))
-            if kind == 1:
-                p, h, _ = cases.isentropic_vortex(n, seed=1000 + i)
-                r = dict(q0=cases.to_conserved(p), spacing=(h, h, h),
-                         t_end=SERVE_T_END["vortex3d"], cfl=0.5,
-                         periodic=(True, True, True), label=f"vortex3d {n}^3")
-            else:
-                x = np.linspace(0.0, 1.0, n)
-                p = cases.blob(x, x, x, seed=1000 + i)
-                h = 1.0 / (n - 1)
-                r = dict(q0=cases.to_conserved(p), spacing=(h, h, h),
-                         t_end=SERVE_T_END["blob3d"], cfl=0.5,
-                         label=f"blob3d {n}^3")
-        r.update(sch)
-        reqs.append(r)
-    return arrivals, reqs
+# The request stream is fvmgpu.cases.serving_requests (end times
+# cases.SERVE_T_END).  At 50 arrivals / s the served run ends soon after the
+# last arrival (the rate is then set by the arrivals); the capacity run queues
+# every request at t = 0 and measures what the GPU sustains.
 
 
 def bench_serving(args):
     import fvmgpu
     from fvmgpu import cases
     seed = 0 if args.seed is None else args.seed
-    arrivals, reqs = serving_requests(fvmgpu, cases, seed, args.requests, args.rate)
+    arrivals, reqs = cases.serving_requests(seed, args.requests, args.rate)
     numerics = numerics_code(fvmgpu, args)
 
     def submit_args(r):
-        a = {k: v for k, v in r.items() if k != "label"}
+        a = {k: v for k, v in r.items() if k not in ("label", "p0")}
         a["numerics"] = numerics
         return a
 
@@ -707,7 +666,7 @@ def bench_serving(args):
     fifo = np.array(fifo)
     config = {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.rate}/s (seed {seed}); "
                           "riemann2d 64^2..512^2, vortex3d / blob3d 32^3..128^3, log-uniform "
-                          f"sizes, end times {SERVE_T_END}, MUSCL-minmod + Roe or WENO5 + "
+                          f"sizes, end times {cases.SERVE_T_END}, MUSCL-minmod + Roe or WENO5 + "
                           "AUSM+, SSP-RK3, CFL 0.5, FP64",
               "requests": len(reqs), "total_steps": steps, "cell_updates": cell_updates,
               "makespan_s": makespan, "p50_time_to_solution_s": float(np.percentile(lat, 50)),

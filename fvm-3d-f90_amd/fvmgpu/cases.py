@@ -224,3 +224,54 @@ def kinetic_energy(q):
     (5, nk, nj, ni) on a periodic box (each vertex one cell)."""
     rho = q[0]
     return float(np.mean(0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / rho))
+
+
+# ---------------------------------------------------------------------------
+# The mixed serving workload: Poisson arrivals of independent solves, 2-D
+# four-quadrant Riemann cases of 64^2-512^2 vertices and 3-D vortex / blob
+# cases of 32^3-128^3, sizes log-uniform, scheme MUSCL-minmod + Roe or textbook
+# WENO5 + AUSM+ (SSP-RK3, CFL 0.5, FP64), each to a fixed end time.
+SERVE_T_END = {"riemann2d": 0.1, "vortex3d": 1.0, "blob3d": 0.1}
+
+# scheme codes (src/defs/type_defs_scheme.f90:11-36; 1303: AUSM+)
+_MUSCL_ROE = dict(recon=201, limiter=251, flux=300, rk=101, ref_compat=1)
+_WENO_AUSMPLUS = dict(recon=202, limiter=250, flux=1303, rk=101, ref_compat=0)
+
+
+def serving_requests(seed, nreq, rate, t_end=None):
+    """(arrival times, requests) of the serving workload: each request a dict
+    of Server.submit arguments plus a `label` and the primitive state `p0`
+    that q0 was converted from.  t_end: a dict of per-kind end
+    times replacing SERVE_T_END (e.g. a short run for tests)."""
+    ends = dict(SERVE_T_END)
+    ends.update(t_end or {})
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    reqs = []
+    for i in range(nreq):
+        kind = rng.integers(3)            # 0: Riemann 2-D, 1: vortex 3-D, 2: blob 3-D
+        sch = dict((_MUSCL_ROE, _WENO_AUSMPLUS)[rng.integers(2)])
+        if kind == 0:
+            n = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
+            # the reference's quadrant states (Lax-Liu configuration 3): the
+            # drawn states of other seeds include near-vacuum rarefactions in
+            # which Roe's linearisation loses positivity
+            p, dx = riemann2d(n, seed=0)
+            r = dict(q0=to_conserved(p), p0=p, spacing=(dx, dx, 1.0), t_end=ends["riemann2d"],
+                     cfl=0.5, label=f"riemann2d {n}^2")
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(32), np.log(128)))))
+            if kind == 1:
+                p, h, _ = isentropic_vortex(n, seed=1000 + i)
+                r = dict(q0=to_conserved(p), p0=p, spacing=(h, h, h), t_end=ends["vortex3d"],
+                         cfl=0.5, periodic=(True, True, True), label=f"vortex3d {n}^3")
+            else:
+                x = np.linspace(0.0, 1.0, n)
+                p = blob(x, x, x, seed=1000 + i)
+                h = 1.0 / (n - 1)
+                r = dict(q0=to_conserved(p), p0=p, spacing=(h, h, h), t_end=ends["blob3d"],
+                         cfl=0.5, label=f"blob3d {n}^3")
+        r.update(sch)
+        reqs.append(r)
+    return arrivals, reqs

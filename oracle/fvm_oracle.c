@@ -831,9 +831,31 @@ static void residual_norms(int64_t nreal, const double *rhs, double *res)
     res[NEQ] = sqrt(0.0 * 0.0 + res2t);
 }
 
+static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
+                      double t_end, int nsteps, const double *p_real, double *q_out,
+                      double *rhs_out, double *res_out, double *dt_out, int *steps_out,
+                      double *t_out);
+
 int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
               int nsteps, const double *p_real, double *q_out, double *rhs_out,
               double *res_out, double *dt_out)
+{
+    return solve_core(n, c, dt_fix, cfl, INFINITY, nsteps, p_real, q_out, rhs_out, res_out,
+                      dt_out, NULL, NULL);
+}
+
+int orc_solve_to(const int64_t n[3], const orc_config *c, double cfl, double t_end,
+                 int max_steps, const double *p_real, double *q_out, int *steps_out,
+                 double *t_out)
+{
+    return solve_core(n, c, 0.0, cfl, t_end, max_steps, p_real, q_out, NULL, NULL, NULL,
+                      steps_out, t_out);
+}
+
+static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
+                      double t_end, int nsteps, const double *p_real, double *q_out,
+                      double *rhs_out, double *res_out, double *dt_out, int *steps_out,
+                      double *t_out)
 {
     blk_t b;
     for (int d = 0; d < 3; ++d) b.n[d] = n[d];
@@ -864,10 +886,17 @@ int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl
     fill_ghosts(&b, c->bc, c->dim);
 
     int nstage = c->rk == ORC_RK4 ? 4 : 3;
-    for (int step = 0; step < nsteps; ++step) {
+    double t = 0.0;
+    int step = 0;
+    for (; step < nsteps; ++step) {
+        /* an end time: stop once t is within 1e-14 |t_end| of it, clip the
+         * last CFL step to it, t accumulated step by step */
+        if (isfinite(t_end) && !(t < t_end - 1.0e-14 * fabs(t_end))) break;
         gather_real(&b, q0);
         double dt = dt_fix;
         if (dt_fix <= 0.0) dt = orc_cfl_dt(n, c->dim, q0, c->gamma, c->dr, cfl);
+        if (isfinite(t_end) && dt > t_end - t) dt = t_end - t;
+        t = t + dt;
         if (dt_out) dt_out[step] = dt;
         for (int s = 1; s <= nstage; ++s) {
             memset(rhs, 0, sizeof(double) * NEQ * nreal);
@@ -885,6 +914,8 @@ int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl
         }
         if (res_out) residual_norms(nreal, rhs, res_out + (NEQ + 1) * step);
     }
+    if (steps_out) *steps_out = step;
+    if (t_out) *t_out = t;
     if (q_out) memcpy(q_out, b.q, sizeof(double) * NEQ * cs);
     if (rhs_out) memcpy(rhs_out, rhs, sizeof(double) * NEQ * nreal);
     free(b.q); free(b.p); free(q0); free(qm); free(acc); free(rhs);

@@ -92,6 +92,16 @@ int orc_solve(const int64_t n[3], const orc_config *cfg, double dt, double cfl,
               int nsteps, const double *p_real, double *q_out, double *rhs_out,
               double *res_out, double *dt_out);
 
+/* CFL steps to an end time, as the device time slot of a served solve takes
+ * them (no reference counterpart: the reference's CFL time step is
+ * unimplemented, src/solve/class_time_discret.f90:94-107): per step
+ * dt = min(cfl / smax, t_end - t), t = t + dt, until t >= t_end - 1e-14
+ * |t_end| or max_steps steps; steps_out and t_out report them.  Outputs as
+ * orc_solve's q_out. */
+int orc_solve_to(const int64_t n[3], const orc_config *cfg, double cfl, double t_end,
+                 int max_steps, const double *p_real, double *q_out, int *steps_out,
+                 double *t_out);
+
 /* Global CFL time step of a state (no reference counterpart):
  *   dt = cfl / max_v sum_d (|u_d| + c) / dr_d,  c = sqrt(gamma p / rho)   */
 double orc_cfl_dt(const int64_t n[3], int dim, const double *q_real,

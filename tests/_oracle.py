@@ -59,6 +59,9 @@ def oracle():
         lib.orc_flux_ausmplus.argtypes = [i64, ci, cd, P, P, P, P]
         lib.orc_solve.argtypes = [P, ctypes.POINTER(OrcConfig), cd, cd, ci, P, P, P, P, P]
         lib.orc_solve.restype = ci
+        lib.orc_solve_to.argtypes = [P, ctypes.POINTER(OrcConfig), cd, cd, ci, P, P,
+                                     ctypes.POINTER(ci), ctypes.POINTER(cd)]
+        lib.orc_solve_to.restype = ci
         lib.orc_cfl_dt.argtypes = [P, ci, P, cd, P, cd]
         lib.orc_cfl_dt.restype = cd
         lib.orc_rk4_stage.argtypes = [ci, cd, i64, P, P, P, P]
@@ -166,6 +169,25 @@ def oracle_solve(p_real, dim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=
                             _ptr(q), _ptr(rhs), _ptr(res), _ptr(dts))
     assert rc == 0
     return {"q": q, "rhs": rhs, "res": res[:nsteps], "dt": dts[:nsteps]}
+
+
+def oracle_solve_to(p_real, dim, dr, cfl, t_end, max_steps=100000, recon=MUSCL,
+                    limiter=LIM_MINMOD, flux=ROE, rk=RK_SSP3, kappa=0.33333, entfix=0.2,
+                    gamma=1.4, bc=(BC_EXTRAP,) * 6, weno_textbook=0):
+    """CFL steps to t_end (the last one clipped), as a served solve takes
+    them.  Returns (q (5, nk+2gz, nj+6, ni+6), steps, final time)."""
+    p_real = np.ascontiguousarray(p_real, dtype=np.float64)
+    nk, nj, ni, _ = p_real.shape
+    gz = 3 if dim == 3 else 0
+    n = np.array([ni, nj, nk], dtype=np.int64)
+    cfg = OrcConfig(dim, recon, limiter, flux, rk, weno_textbook, 1, (ci * 6)(*bc), kappa,
+                    entfix, gamma, (cd * 3)(*dr), 0.0, 0.0, 0.0, 0.0, 0.0)
+    q = np.zeros((5, nk + 2 * gz, nj + 6, ni + 6))
+    steps, t = ci(0), cd(0.0)
+    rc = oracle().orc_solve_to(_ptr(n), ctypes.byref(cfg), cfl, t_end, max_steps, _ptr(p_real),
+                               _ptr(q), ctypes.byref(steps), ctypes.byref(t))
+    assert rc == 0
+    return q, steps.value, t.value
 
 
 def ref_solve(p_real, idim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, kappa=0.33333,
