@@ -69,3 +69,70 @@ def test_fast_1000_steps_vortex():
     for mode in out:
         for c in range(5):
             assert abs(out[mode][0][c].sum() - q0[c].sum()) <= 1e-11 * np.abs(q0[c]).sum() + 1e-12
+
+
+def _vortex128_run(mode, q0, h, marks):
+    out, done = [], 0
+    with fvmgpu.Context() as ctx:
+        ctx.scheme(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+        ctx.numerics(mode)
+        b = ctx.block((128, 128, 128), (h, h, h))
+        b.set_periodic()
+        b.upload(q0)
+        for m in marks:
+            ctx.step(m - done, cfl=0.8)
+            done = m
+            out.append(b.download())
+    return out
+
+
+def test_vortex128_amplifies_rounding_differences():
+    """The benchmark workload (128^3 vortex, MUSCL-vanAlbada + Roe, RK4, CFL
+    0.8) amplifies any rounding-level difference: the Yee vortex's vorticity
+    changes sign at r = sqrt 2 (a shielded vortex, unstable to azimuthal
+    perturbations), and at 128^3 the scheme's dissipation no longer damps it.
+    Measured: the exact family (bit for bit the oracle) started from a state
+    one ulp away in rho*E differs from itself by more than 1e-10 after 256
+    steps, and the fast family (FMA contraction, refined reciprocals) differs
+    from the exact family by no more than that same exact-family spread at
+    1000 steps.  So at this size the 1000-step 1e-9 gate is met only by
+    arithmetic bit-identical to the oracle: the exact family, which is what
+    the benchmark reports; the fast family is as close as the problem's own
+    conditioning under one-ulp input changes allows."""
+    p, h, _ = cases.isentropic_vortex(128, seed=0)
+    q0 = cases.to_conserved(p)
+    q1 = q0.copy()
+    q1[4] = np.nextafter(q1[4], np.inf)
+    marks = [16, 256, 1000]
+    print("case vortex128 conditioning: exact, exact+1ulp, fast; 1000 steps", flush=True)
+    ex = _vortex128_run(fvmgpu.NUMERICS_EXACT, q0, h, marks)
+    ex1 = _vortex128_run(fvmgpu.NUMERICS_EXACT, q1, h, marks)
+    fa = _vortex128_run(fvmgpu.NUMERICS_FAST, q0, h, marks)
+    spread = [rel_linf(a, b) for a, b in zip(ex1, ex)]
+    fast = [rel_linf(a, b) for a, b in zip(fa, ex)]
+    print("exact one-ulp spread", spread, "fast vs exact", fast)
+    assert spread[0] < 1e-13 and spread[1] > 1e-10            # exponential growth
+    assert fast[0] < 1e-13
+    assert fast[2] <= max(spread[1:]) * 1.5 + 1e-12
+    # conservation to round-off in both families
+    for q in (ex[2], fa[2]):
+        for c in range(5):
+            assert abs(q[c].sum() - q0[c].sum()) <= 1e-11 * np.abs(q0[c]).sum() + 1e-12
+
+
+def test_exact_1000_steps_equals_oracle():
+    """The exact family against the oracle over the full 1000 steps (not only
+    step by step), on a grid the host oracle runs in seconds: the 32^3 vortex
+    with the benchmark's scheme and CFL steps; bit for bit, including every
+    CFL dt."""
+    p, h, _ = cases.isentropic_vortex(32, seed=7)
+    print("case exact vs oracle 1000 steps vortex32", flush=True)
+    scheme = dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+    with fvmgpu.Context() as ctx:
+        qg, norms, _, dt_last = run_gpu(ctx, p, (h, h, h), 1000, cfl=0.8, bc=(1,) * 6, **scheme)
+    O.set_threads(16)
+    qo, r = run_oracle(p, (h, h, h), 1000, cfl=0.8, bc=(1,) * 6, **scheme)
+    print("rel linf", rel_linf(qg, qo))
+    assert dt_last == r["dt"][-1]
+    assert np.array_equal(qg, qo)
+    np.testing.assert_allclose(norms, r["res"][-1], rtol=1e-12, atol=1e-300)
