@@ -789,7 +789,8 @@ void unpack_state(const fvm_block* b, const double* stage, double* q, int host_n
 // block after block).
 // Descriptor entries of the context's blocks into h[0 .. blocks), tile bases
 // and the tile total.
-void fill_desc(fvm_ctx* ctx, BlockDesc* h) {
+void fill_desc(fvm_ctx* ctx, BlockDesc* h, const double* dt_ptr = nullptr) {
+  if (!dt_ptr) dt_ptr = ctx->d_dt;
   int base = 0;
   ctx->has_conn = false;
   for (size_t i = 0; i < ctx->blocks.size(); ++i) {
@@ -802,7 +803,7 @@ void fill_desc(fvm_ctx* ctx, BlockDesc* h) {
     for (int f = 0; f < 6; ++f) { d.nbr[f] = b->nbr[f]; d.nbr_shared[f] = b->nbr_shared[f]; }
     d.tiles_x = b->tiles_x; d.tiles_y = b->tiles_y; d.tiles_z = b->tiles_z; d.chunk = b->chunk;
     d.tile_base = base;
-    d.dt_dev = b->dt_dev ? b->dt_dev : ctx->d_dt;
+    d.dt_dev = b->dt_dev ? b->dt_dev : dt_ptr;
     d.smax = b->smax_dev;
     d.halt = b->halt_dev;
     d.rv = ctx->visc_mu_ref > 0.0 ? b->rv : nullptr;
@@ -937,18 +938,25 @@ int ensure_norm_capacity(fvm_ctx* ctx, size_t doubles) {
   return FVM_OK;
 }
 
-// Refill the ghost layers of state array `buf` of every block.
-int fill_ghosts_all(fvm_ctx* ctx, int buf) {
-  if (int rc = sync_desc(ctx)) return rc;
+// Refill the face ghost layers of state array `buf` of every block, reading
+// the block table `tab` (the context's, or the no-allocation scratch's).
+int fill_ghosts_tab(fvm_ctx* ctx, const BlockDesc* tab, int buf) {
   for (size_t i = 0; i < ctx->blocks.size(); ++i) {
     fvm_block* b = ctx->blocks[i];
     for (int d = 0; d < b->dim; ++d)
       for (int side = 0; side < 2; ++side) {
-        hipLaunchKernelGGL(ghost_fill_kernel, dim3(256), dim3(256), 0, ctx->stream, ctx->d_desc, (int)i,
+        hipLaunchKernelGGL(ghost_fill_kernel, dim3(256), dim3(256), 0, ctx->stream, tab, (int)i,
                            buf, b->nc, d, side);
         HIP_TRY(hipGetLastError());
       }
   }
+  return FVM_OK;
+}
+
+// Refill the ghost layers of state array `buf` of every block.
+int fill_ghosts_all(fvm_ctx* ctx, int buf) {
+  if (int rc = sync_desc(ctx)) return rc;
+  if (int rc = fill_ghosts_tab(ctx, ctx->d_desc, buf)) return rc;
   return launch_patches(ctx, buf);
 }
 
@@ -1039,10 +1047,10 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
   return FVM_OK;
 }
 
-int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
+int launch_stage_all(fvm_ctx* ctx, StageArgs a, const BlockDesc* tab = nullptr) {
   if (ctx->visc_mu_ref > 0.0)
     if (int rc = launch_viscous_all(ctx, a.in_buf)) return rc;
-  a.blocks = ctx->d_desc;
+  a.blocks = tab ? tab : ctx->d_desc;
   a.nblocks = (int)ctx->blocks.size();
   if (ctx->launch_mode != FVM_LAUNCH_PER_BLOCK && batchable(ctx)) {
     a.tile0 = 0;
@@ -1057,14 +1065,24 @@ int launch_stage_all(fvm_ctx* ctx, StageArgs a) {
   return FVM_OK;
 }
 
-// One time step of every block: the RK stages (in CFL mode the first one
-// computes dt from the device time slot and the second commits the step).
+// Device memory a time step works in: the block table, the per-tile norm
+// partials and the time slot -- the context's own, or the static scratch of
+// the no-allocation entry point.
+struct StepMem {
+  const BlockDesc* desc;
+  double* norm;
+  SolveSlot* slot;
+};
+
+StepMem ctx_mem(fvm_ctx* ctx) { return StepMem{ctx->d_desc, ctx->d_norm, ctx->d_slot}; }
+
+// Stage st (1-based) of one time step: its buffers, combine, norm and CFL
+// roles, then the launch over every block and the patch refill.
 // `norm_mode`: 0 no residual norms, 1 the last stage leaves its norm
 // partials, 2 only if the step reached t_end.  With timing on, each stage
 // launch is bracketed by events (ev index *nev).
-// Stage st (1-based) of one time step: its buffers, combine, norm and CFL
-// roles, then the launch over every block and the patch refill.
-int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t* nev) {
+int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t* nev,
+                     const StepMem& mem) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   {
     const bool last = st == nstage;
@@ -1087,32 +1105,43 @@ int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t*
         default: a.in_buf = 2; a.out_buf = 0; a.stage = ST_SSP3; break;
       }
     }
-    a.norm_part = (last && norm_mode) ? ctx->d_norm : nullptr;
+    a.norm_part = (last && norm_mode) ? mem.norm : nullptr;
     a.norm_if_end = last && norm_mode == 2 ? 1 : 0;
-    a.smax = (last && cfl_mode) ? ctx->d_smax : nullptr;
-    a.halt = cfl_mode ? &ctx->d_slot->halt : nullptr;
-    a.slot = cfl_mode ? ctx->d_slot : nullptr;
+    a.smax = (last && cfl_mode) ? &mem.slot->smax : nullptr;
+    a.halt = cfl_mode ? &mem.slot->halt : nullptr;
+    a.slot = cfl_mode ? mem.slot : nullptr;
     a.cfl_dt = cfl_mode && st == 1 ? 1 : 0;
     a.commit = cfl_mode && st == 2 ? 1 : 0;
     const bool ev = nev != nullptr;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
-    if (int rc = launch_stage_all(ctx, a)) return rc;
+    if (int rc = launch_stage_all(ctx, a, mem.desc)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_patches(ctx, a.out_buf)) return rc;
   }
   return FVM_OK;
 }
 
-// Residual norms of a step's last stage (class_iteration.f90:210-248) from
-// the per-tile partials in ctx->h_norm: summed in tile order per block;
-// blocks combined as the reference accumulates them, R = sqrt(R_prev^2 +
-// R_blk^2).  Fills ctx->norms; returns the NaN / divergence flag.
-int combine_norms(fvm_ctx* ctx, int nc) {
-  double res_abs[6] = {0, 0, 0, 0, 0, 0};
-  for (fvm_block* b : ctx->blocks) {
-    double sums[5] = {0, 0, 0, 0, 0};
+// Per-block sums of the per-tile norm partials in ctx->h_norm, in tile order
+// (block_sums: nc per block, context order).
+void sum_norm_partials(fvm_ctx* ctx, int nc, double* block_sums) {
+  for (size_t bi = 0; bi < ctx->blocks.size(); ++bi) {
+    fvm_block* b = ctx->blocks[bi];
+    double* sums = block_sums + bi * nc;
+    for (int c = 0; c < nc; ++c) sums[c] = 0.0;
     for (int w = 0; w < b->ntiles; ++w)
       for (int c = 0; c < nc; ++c) sums[c] += ctx->h_norm[(size_t)(b->tile_base + w) * nc + c];
+  }
+}
+
+// Residual norms of a step's last stage (class_iteration.f90:210-248) from
+// the per-block sums of rhs^2; blocks combined as the reference accumulates
+// them, R = sqrt(R_prev^2 + R_blk^2).  Fills ctx->norms; returns the NaN /
+// divergence flag.
+int combine_norms(fvm_ctx* ctx, int nc, const double* block_sums) {
+  double res_abs[6] = {0, 0, 0, 0, 0, 0};
+  for (size_t bi = 0; bi < ctx->blocks.size(); ++bi) {
+    fvm_block* b = ctx->blocks[bi];
+    const double* sums = block_sums + bi * nc;
     double res2[5] = {0, 0, 0, 0, 0};
     const double npts = (double)b->g.nx * b->g.ny * b->g.nz;
     for (int c = 0; c < nc; ++c) {
@@ -1134,10 +1163,12 @@ int combine_norms(fvm_ctx* ctx, int nc) {
   return nan;
 }
 
-int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev) {
+// One time step of every block: the RK stages (in CFL mode the first one
+// computes dt from the device time slot and the second commits the step).
+int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev, const StepMem& mem) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   for (int st = 1; st <= nstage; ++st)
-    if (int rc = launch_one_stage(ctx, st, cfl_mode, norm_mode, nev)) return rc;
+    if (int rc = launch_one_stage(ctx, st, cfl_mode, norm_mode, nev, mem)) return rc;
   return FVM_OK;
 }
 
@@ -1157,7 +1188,7 @@ int step_graph(fvm_ctx* ctx, bool cfl_mode, int norm_mode, hipGraphExec_t* out) 
     sg.exec = nullptr;
   }
   HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = launch_step(ctx, cfl_mode, norm_mode, nullptr);
+  const int rc = launch_step(ctx, cfl_mode, norm_mode, nullptr, ctx_mem(ctx));
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
   if (rc) {
@@ -1175,6 +1206,60 @@ int step_graph(fvm_ctx* ctx, bool cfl_mode, int norm_mode, hipGraphExec_t* out) 
   sg.cfl_mode = cfl_mode;
   *out = sg.exec;
   return FVM_OK;
+}
+
+// ---- memory of the no-allocation step (fvm_step_noalloc) -------------------
+// One static device array per GPU holds what fvm_step keeps in context
+// allocations: the time slot, the block table, per-block norm sums and the
+// per-tile norm partials.  Host code never uses the array's name as an
+// address: fvm_ctx_create resolves its address on each device once
+// (hipGetSymbolAddress) and the entry point passes that pointer on.
+constexpr int kScratchBlocks = 64;          // blocks of a context
+constexpr int kScratchTiles = 65536;        // stage tiles over all its blocks
+constexpr size_t scratch_round(size_t n) { return (n + 255) / 256 * 256; }
+constexpr size_t kScratchDescOff = scratch_round(sizeof(SolveSlot));
+constexpr size_t kScratchSumOff = kScratchDescOff + scratch_round(kScratchBlocks * sizeof(BlockDesc));
+constexpr size_t kScratchNormOff = kScratchSumOff + scratch_round(kScratchBlocks * 5 * sizeof(double));
+constexpr size_t kScratchBytes = kScratchNormOff + (size_t)kScratchTiles * 5 * sizeof(double);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+// With `device` current: the device address of its copy of `scratch`.
+int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kScratchDevices || g_scratch_ptr[device]) return FVM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(FVM_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_scratch_ptr[device] = (unsigned char*)p;
+  return FVM_OK;
+}
+
+unsigned char* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kScratchDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// Per-block sums of the per-tile norm partials, tiles in order, one thread
+// per (block, component): the additions sum_norm_partials makes on the host.
+__global__ void norm_block_sum_kernel(const BlockDesc* __restrict__ tab, int nblocks, int total_tiles,
+                                      int nc, const double* __restrict__ part, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nblocks * nc) return;
+  const int b = e / nc, c = e % nc;
+  const int t0 = tab[b].tile_base, t1 = b + 1 < nblocks ? tab[b + 1].tile_base : total_tiles;
+  double s = 0.0;
+  for (int w = t0; w < t1; ++w) s += part[(int64_t)w * nc + c];
+  out[e] = s;
 }
 
 }  // namespace
@@ -1199,6 +1284,10 @@ int fvm_ctx_create(int device, fvm_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cu = prop.multiProcessorCount;
+  if (int rc = scratch_setup(device)) {
+    delete ctx;
+    return rc;
+  }
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc(&ctx->d_slot, sizeof(SolveSlot)));
   HIP_TRY(hipMemset(ctx->d_slot, 0, sizeof(SolveSlot)));
@@ -1688,7 +1777,7 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
       HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     } else {
       const size_t before = nev;
-      if (int rc = launch_step(ctx, cfl_mode, norm_mode, timed ? &nev : nullptr)) return rc;
+      if (int rc = launch_step(ctx, cfl_mode, norm_mode, timed ? &nev : nullptr, ctx_mem(ctx))) return rc;
       for (size_t e = before; e < nev; e += 2) ev_step.push_back(s);
     }
   }
@@ -1728,7 +1817,9 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     }
     ctx->timing_step += nsteps;
   }
-  int nan = combine_norms(ctx, nc);
+  std::vector<double> block_sums(ctx->blocks.size() * nc);
+  sum_norm_partials(ctx, nc, block_sums.data());
+  int nan = combine_norms(ctx, nc, block_sums.data());
   if (cfl_mode && ctx->h_slot.nan) nan = 1;
   if (ran == 0) {
     // the run had already reached its end time: nothing new to report
@@ -1776,7 +1867,7 @@ int fvm_stage(fvm_ctx* ctx, int s, double dt) {
   } else if (ctx->desc_dirty) {
     return fail(FVM_ESTATE, "blocks or launch shapes changed within a step");
   }
-  if (int rc = launch_one_stage(ctx, s, false, s == nstage ? 1 : 0, nullptr)) return rc;
+  if (int rc = launch_one_stage(ctx, s, false, s == nstage ? 1 : 0, nullptr, ctx_mem(ctx))) return rc;
   if (s < nstage) {
     ctx->stage_next = s + 1;
     HIP_TRY(hipGetLastError());
@@ -1789,8 +1880,104 @@ int fvm_stage(fvm_ctx* ctx, int s, double dt) {
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->time = ctx->time + dt;
   ctx->steps_done += 1;
-  ctx->nan_flag = combine_norms(ctx, nc);
+  std::vector<double> block_sums(ctx->blocks.size() * nc);
+  sum_norm_partials(ctx, nc, block_sums.data());
+  ctx->nan_flag = combine_norms(ctx, nc, block_sums.data());
   ctx->have_norms = true;
+  return FVM_OK;
+}
+
+// fvm_step without any allocation: the step's table, time slot and norm
+// partials live in the device's static scratch array (see its header entry).
+int fvm_step_noalloc(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->scheme_set) return fail(FVM_ESTATE, "fvm_scheme_set has not been called");
+  if (ctx->blocks.empty()) return fail(FVM_ESTATE, "no blocks");
+  if (nsteps < 0) return fail(FVM_EINVAL, "nsteps must be >= 0");
+  const bool cfl_mode = !(dt > 0.0);
+  if (cfl_mode && !(cfl > 0.0)) return fail(FVM_EINVAL, "dt <= 0 needs cfl > 0");
+  if (ctx->stage_next != 1)
+    return fail(FVM_ESTATE, "a step begun with fvm_stage is at stage %d", ctx->stage_next);
+  if ((int)ctx->blocks.size() > kScratchBlocks)
+    return fail(FVM_EINVAL, "fvm_step_noalloc takes at most %d blocks (context has %zu)", kScratchBlocks,
+                ctx->blocks.size());
+  if (!ctx->patches.empty())
+    return fail(FVM_EINVAL, "fvm_step_noalloc does not take connection patches or wall, symmetry, "
+                "inflow or farfield faces");
+  for (fvm_block* b : ctx->blocks)
+    if (!b->uploaded) return fail(FVM_ESTATE, "block state not uploaded");
+  unsigned char* base = scratch_ptr(ctx->device);
+  if (!base) return fail(FVM_ESTATE, "no scratch memory on device %d", ctx->device);
+  if (int rc = set_device(ctx)) return rc;
+  ctx->have_norms = false;
+  if (nsteps == 0) return FVM_OK;
+  const StepMem mem{(const BlockDesc*)(base + kScratchDescOff), (double*)(base + kScratchNormOff),
+                    (SolveSlot*)base};
+  double* d_sums = (double*)(base + kScratchSumOff);
+  // the block table, staged on the stack (the copy is complete before return)
+  BlockDesc h_desc[kScratchBlocks];
+  fill_desc(ctx, h_desc, &mem.slot->dt);
+  if (ctx->total_tiles > kScratchTiles)
+    return fail(FVM_EINVAL, "fvm_step_noalloc takes at most %d stage tiles (the blocks have %d)",
+                kScratchTiles, ctx->total_tiles);
+  const int nb = (int)ctx->blocks.size();
+  const int nc = ctx->blocks[0]->nc;
+  HIP_TRY(hipMemcpyAsync((void*)mem.desc, h_desc, nb * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                         ctx->stream));
+  if (ctx->ghosts_dirty) {
+    if (int rc = fill_ghosts_tab(ctx, mem.desc, 0)) return rc;
+    ctx->ghosts_dirty = false;
+  }
+  if (cfl_mode) {
+    ctx->h_slot = SolveSlot{};
+    ctx->h_slot.t = ctx->time;
+    ctx->h_slot.t_end = ctx->t_end;
+    ctx->h_slot.cfl = cfl;
+    HIP_TRY(hipMemcpyAsync(mem.slot, &ctx->h_slot, sizeof(SolveSlot), hipMemcpyHostToDevice, ctx->stream));
+    for (fvm_block* b : ctx->blocks) {
+      if (b->nc == 5)
+        hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g,
+                           ctx->P.gamma, &mem.slot->smax);
+      else
+        hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g,
+                           ctx->P.gamma, &mem.slot->smax);
+      HIP_TRY(hipGetLastError());
+    }
+  } else {
+    hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, mem.slot, dt);
+    HIP_TRY(hipGetLastError());
+  }
+  const bool may_halt = cfl_mode && std::isfinite(ctx->t_end);
+  ctx->ghosts_stale = true;
+  for (int st = 0; st < nsteps; ++st) {
+    const int norm_mode = st == nsteps - 1 ? 1 : (may_halt ? 2 : 0);
+    if (int rc = launch_step(ctx, cfl_mode, norm_mode, nullptr, mem)) return rc;
+  }
+  hipLaunchKernelGGL(norm_block_sum_kernel, dim3((nb * nc + 63) / 64), dim3(64), 0, ctx->stream, mem.desc,
+                     nb, ctx->total_tiles, nc, mem.norm, d_sums);
+  HIP_TRY(hipGetLastError());
+  double h_sums[kScratchBlocks * 5];
+  HIP_TRY(hipMemcpyAsync(h_sums, d_sums, nb * nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (cfl_mode)
+    HIP_TRY(hipMemcpyAsync(&ctx->h_slot, mem.slot, sizeof(SolveSlot), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  double dt_host = dt;
+  int ran = nsteps;
+  if (cfl_mode) {
+    dt_host = ctx->h_slot.dt;
+    ran = ctx->h_slot.steps;
+    ctx->time = ctx->h_slot.t;
+  } else {
+    for (int st = 0; st < nsteps; ++st) ctx->time = ctx->time + dt;
+  }
+  ctx->steps_done += ran;
+  if (dt_last) *dt_last = dt_host;
+  int nan = combine_norms(ctx, nc, h_sums);
+  if (cfl_mode && ctx->h_slot.nan) nan = 1;
+  if (ran == 0) return FVM_OK;
+  ctx->nan_flag = nan;
+  ctx->have_norms = true;
+  if (cfl_mode && ctx->h_slot.nan) return fail(FVM_ENAN, "CFL wave speed is NaN or zero");
   return FVM_OK;
 }
 

@@ -58,7 +58,7 @@ EXPORTS = ("fvm_last_error", "fvm_version", "fvm_ctx_create", "fvm_ctx_destroy",
            "fvm_snapshot_read", "fvm_snapshot_save", "fvm_snapshot_release", "fvm_restart_load",
            "fvm_block_set_bc_state", "fvm_block_init", "fvm_cut_uniform",
            "fvm_block_launch_tile", "fvm_block_init_at", "fvm_upload_planes",
-           "fvm_download_planes", "fvm_stage")
+           "fvm_download_planes", "fvm_stage", "fvm_step_noalloc")
 
 
 class FvmError(RuntimeError):
@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
         "fvm_upload_planes": ([vp, vp, ci, i64, i64], ci),
         "fvm_download_planes": ([vp, vp, ci, i64], ci),
         "fvm_stage": ([vp, ci, cd], ci),
+        "fvm_step_noalloc": ([vp, ci, cd, cd, ctypes.POINTER(cd)], ci),
         "fvm_step": ([vp, ci, cd, cd, ctypes.POINTER(cd)], ci),
         "fvm_norms": ([vp, ctypes.POINTER(cd), ctypes.POINTER(ci)], ci),
         "fvm_sync": ([vp], ci),
@@ -335,6 +336,13 @@ class Context:
         """Advance every block; dt > 0 fixed, else CFL.  Returns the last dt."""
         out = ctypes.c_double(0.0)
         _check(_lib.fvm_step(self._h, int(nsteps), float(dt), float(cfl), ctypes.byref(out)))
+        return out.value
+
+    def step_noalloc(self, nsteps=1, dt=0.0, cfl=0.0):
+        """fvm_step_noalloc: step() with no allocation (static scratch)."""
+        out = ctypes.c_double(0.0)
+        _check(_lib.fvm_step_noalloc(self._h, int(nsteps), float(dt), float(cfl),
+                                     ctypes.byref(out)))
         return out.value
 
     def stage(self, s, dt):

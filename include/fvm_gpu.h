@@ -280,6 +280,20 @@ int fvm_step(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
  * FVM_ESTATE; an upload or device initial state abandons the step. */
 int fvm_stage(fvm_ctx *ctx, int s, double dt);
 
+/* fvm_step for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The same steps and results as fvm_step (bit
+ * for bit), but the step's own device memory -- block table, time slot and
+ * residual-norm partials, which fvm_step allocates in the context on first
+ * use -- is a static array of 2.7 MB per GPU, held until the process ends;
+ * the call allocates nothing (device or host).  Limits: at most 64 blocks
+ * and 65536 stage tiles per context (a 128^3 block at the default launch
+ * shape has 512 tiles), no connection patches or wall / symmetry / inflow /
+ * farfield faces; beyond them FVM_EINVAL.  Stage timing and hipGraph replay
+ * do not apply.  The array is shared by every call on a GPU: calls that
+ * overlap on one GPU (other threads, other contexts' streams) race. */
+int fvm_step_noalloc(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt_last);
+
 /* Central second-order viscous fluxes with Sutherland's law (no reference
  * counterpart: viscous_flux_sub, src/solve/class_space_discret.f90:671-675,
  * is an empty stub and its call at src/solve/class_time_discret.f90:182 is
