@@ -43,6 +43,56 @@ __device__ __forceinline__ double frcp(double x) {
   }
 }
 
+// ---- exact family: IEEE FP64 division and square root, shortened -----------
+// The compiler expands an IEEE FP64 division a / b into two v_div_scale, a
+// v_rcp estimate, four FMAs (two Newton steps on the reciprocal), a multiply,
+// an FMA (the remainder), v_div_fmas and v_div_fixup.  When v_div_scale does
+// not rescale -- a numerator of 0 or of magnitude at least 2^-969, a normal
+// divisor with a normal reciprocal, a normal quotient, operand exponents
+// less than 768 apart -- and the divisor is finite and non-zero, the two
+// scale steps and the fix-up return their operand unchanged and v_div_fmas is
+// a plain FMA, so the other eight instructions alone give the IEEE quotient
+// bit for bit.  Every divisor on this path is positive (densities, sound
+// speeds, spacings, sums of squares plus an epsilon, Harten's delta, rho_l +
+// rho_b, gamma); taking the remainder negated, b q0 - a, gives a zero
+// quotient the sign IEEE gives it for a positive divisor.  The reciprocal
+// part depends on the divisor alone, so a divisor used several times is
+// inverted once (Recip) and each quotient costs three instructions.
+struct Recip { double b = 0.0, r = 0.0; };
+__device__ __forceinline__ Recip erecip(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  return Recip{b, r};
+}
+__device__ __forceinline__ double ediv(double a, const Recip& d) {
+  const double q0 = a * d.r;
+  const double rem = fma(d.b, q0, -a);
+  return fma(-rem, d.r, q0);
+}
+__device__ __forceinline__ double ediv(double a, double b) { return ediv(a, erecip(b)); }
+
+// IEEE FP64 square root: the compiler's sequence -- the reciprocal square
+// root estimate, one Goldschmidt step and two Newton corrections -- without
+// its rescaling of operands below 2^-767 and its select for zero and
+// infinity, which leave every finite operand from 2^-767 up unchanged: bit
+// for bit sqrt there (the operands here are products of densities and
+// squared sound speeds).
+__device__ __forceinline__ double esqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double h = y * 0.5;
+  double g = x * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  double d = fma(-g, g, x);
+  h = fma(h, r, h);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
 // Square root.  exact: IEEE sqrt.  fast: the hardware reciprocal square
 // root estimate refined by one Goldschmidt iteration and a final Newton
 // correction (relative error near one ulp for the positive normal operands it
@@ -128,11 +178,12 @@ __device__ __forceinline__ void cons_to_prim(const double q[NC], double p[NC], d
     }
     return;
   }
-  u = q[1] / rho;
-  v = q[2] / rho;
+  const Recip ir = erecip(rho);
+  u = ediv(q[1], ir);
+  v = ediv(q[2], ir);
   p[0] = rho; p[1] = u; p[2] = v;
   if constexpr (NC == 5) {
-    const double w = q[3] / rho;
+    const double w = ediv(q[3], ir);
     p[3] = w;
     p[4] = (gamma - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w));
   } else {
@@ -161,12 +212,12 @@ __device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3, do
   } else if constexpr (LIM == LIM_VANLEER) {
     const double eps = 1.0e-6;
     if constexpr (kFast) Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p * frcp(fabs(D_m) + fabs(D_p) + eps);
-    else Db_m = (fsign1(D_m) + fsign1(D_p)) * D_m * D_p / (fabs(D_m) + fabs(D_p) + eps);
+    else Db_m = ediv((fsign1(D_m) + fsign1(D_p)) * D_m * D_p, fabs(D_m) + fabs(D_p) + eps);
     Db_p = Db_m;
   } else if constexpr (LIM == LIM_VANALBADA) {
     const double eps = 1.0e-6;
     if constexpr (kFast) Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) * frcp(D_m * D_m + D_p * D_p + eps);
-    else Db_m = (D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps)) / (D_m * D_m + D_p * D_p + eps);
+    else Db_m = ediv(D_m * (D_p * D_p + eps) + D_p * (D_m * D_m + eps), D_m * D_m + D_p * D_p + eps);
     Db_p = Db_m;
   } else {
     Db_m = D_m;
@@ -243,11 +294,11 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
     const double b0 = c_0 * (E1 * E2), b1 = c_1 * (E0 * E2), b2 = c_2 * (E0 * E1);
     return (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
   }
-  const double a0 = c_0 / (e0 * e0);
-  const double a1 = c_1 / (e1 * e1);
-  const double a2 = c_2 / (e2 * e2);
-  const double as = a0 + a1 + a2;
-  const double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+  const double a0 = ediv(c_0, e0 * e0);
+  const double a1 = ediv(c_1, e1 * e1);
+  const double a2 = ediv(c_2, e2 * e2);
+  const Recip ras = erecip(a0 + a1 + a2);
+  const double w0 = ediv(a0, ras), w1 = ediv(a1, ras), w2 = ediv(a2, ras);
   return w0 * q0 + w1 * q1 + w2 * q2;
 }
 
@@ -296,7 +347,11 @@ __device__ __forceinline__ void weno_js_pair(double u0, double u1, double u2, do
 // ---- Harten fix, src/scheme/scheme_FDS_Roe.f90:167-177 ----------------------
 __device__ __forceinline__ double entropy_correct(double lambda, double delta) {
   if constexpr (kFast) return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) * frcp(delta);
-  return (lambda > delta) ? lambda : 0.5 * (lambda * lambda + delta * delta) / delta;
+  return (lambda > delta) ? lambda : ediv(0.5 * (lambda * lambda + delta * delta), delta);
+}
+// The same with delta's reciprocal formed by the caller (exact family).
+__device__ __forceinline__ double entropy_correct(double lambda, const Recip& delta) {
+  return (lambda > delta.b) ? lambda : ediv(0.5 * (lambda * lambda + delta.b * delta.b), delta);
 }
 
 // Face state as primitive components along the sweep direction D (0,1,2):
@@ -326,9 +381,9 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     H_l = P.g_gm1 * p_l * (rho_r * rs2) + 0.5 * ke_l;
     H_r = P.g_gm1 * p_r * (rho_l * rs2) + 0.5 * ke_r;
   } else {
-    H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-    H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-    rho_b = sqrt(rho_l * rho_r);
+    H_l = ediv(P.g_gm1 * p_l, rho_l) + 0.5 * ke_l;
+    H_r = ediv(P.g_gm1 * p_r, rho_r) + 0.5 * ke_r;
+    rho_b = esqrt(rho_l * rho_r);
   }
   const double drho = rho_r - rho_l, du = u_r - u_l, dv = v_r - v_l, dw = w_r - w_l;
   const double dpress = p_r - p_l, dVl = Vc_r - Vc_l;
@@ -343,10 +398,11 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     w_b = (NC == 5) ? w_l + wr * dw : 0.0;
     H_b = H_l + wr * (H_r - H_l);
   } else {
-    u_b = (rho_l * u_l + rho_b * u_r) / den;
-    v_b = (rho_l * v_l + rho_b * v_r) / den;
-    w_b = (NC == 5) ? (rho_l * w_l + rho_b * w_r) / den : 0.0;
-    H_b = (rho_l * H_l + rho_b * H_r) / den;
+    const Recip rden = erecip(den);
+    u_b = ediv(rho_l * u_l + rho_b * u_r, rden);
+    v_b = ediv(rho_l * v_l + rho_b * v_r, rden);
+    w_b = (NC == 5) ? ediv(rho_l * w_l + rho_b * w_r, rden) : 0.0;
+    H_b = ediv(rho_l * H_l + rho_b * H_r, rden);
   }
   const double Vl_b = (D == 0) ? u_b : (D == 1) ? v_b : w_b;
   double q2_b;
@@ -355,7 +411,7 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   const double c2_b = (gamma - 1.0) * (H_b - 0.5 * q2_b);
   double c_b, rc_b = 0.0;
   if constexpr (kFast) fsqrt_rsqrt(c2_b, c_b, rc_b);
-  else c_b = sqrt(c2_b);
+  else c_b = esqrt(c2_b);
   const double delta = P.entfix * fabs(Vl_b + c_b);
   double lam1, lam2, lam5;
   if constexpr (kFast) {
@@ -366,9 +422,10 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     lam2 = a2 > delta ? a2 : (a2 * a2 + dd) * hrd;
     lam5 = a5 > delta ? a5 : (a5 * a5 + dd) * hrd;
   } else {
-    lam1 = entropy_correct(fabs(Vl_b - c_b), delta);
-    lam2 = entropy_correct(fabs(Vl_b), delta);
-    lam5 = entropy_correct(fabs(Vl_b + c_b), delta);
+    const Recip rdelta = erecip(delta);
+    lam1 = entropy_correct(fabs(Vl_b - c_b), rdelta);
+    lam2 = entropy_correct(fabs(Vl_b), rdelta);
+    lam5 = entropy_correct(fabs(Vl_b + c_b), rdelta);
   }
   double d1, d2, d5;
   if constexpr (kFast) {
@@ -377,9 +434,10 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     d2 = lam2 * (drho - dpress * ic2);
     d5 = lam5 * (dpress + rho_b * c_b * dVl) * hic2;
   } else {
-    d1 = lam1 * (dpress - rho_b * c_b * dVl) / (2.0 * c_b * c_b);
-    d2 = lam2 * (drho - dpress / (c_b * c_b));
-    d5 = lam5 * (dpress + rho_b * c_b * dVl) / (2.0 * c_b * c_b);
+    const Recip r2c2 = erecip(2.0 * c_b * c_b);
+    d1 = ediv(lam1 * (dpress - rho_b * c_b * dVl), r2c2);
+    d2 = lam2 * (drho - ediv(dpress, c_b * c_b));
+    d5 = ediv(lam5 * (dpress + rho_b * c_b * dVl), r2c2);
   }
   const double lr = lam2 * rho_b;
   // mass
@@ -461,23 +519,25 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
   double q2_l, q2_r;
   if constexpr (NC == 5) { q2_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; q2_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { q2_l = vl[0] * vl[0] + vl[1] * vl[1]; q2_r = vr[0] * vr[0] + vr[1] * vr[1]; }
-  const double c_l = kFast ? fsqrt(gamma * p_l * frcp(rho_l)) : sqrt(gamma * p_l / rho_l);
-  const double c_r = kFast ? fsqrt(gamma * p_r * frcp(rho_r)) : sqrt(gamma * p_r / rho_r);
-  double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
-  double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
+  const double c_l = kFast ? fsqrt(gamma * p_l * frcp(rho_l)) : esqrt(ediv(gamma * p_l, rho_l));
+  const double c_r = kFast ? fsqrt(gamma * p_r * frcp(rho_r)) : esqrt(ediv(gamma * p_r, rho_r));
+  double M_l = kFast ? Vc_l * frcp(c_l) : ediv(Vc_l, c_l);
+  double M_r = kFast ? Vc_r * frcp(c_r) : ediv(Vc_r, c_r);
   M_l = (M_l >= 1.0) ? M_l : (M_l <= -1.0) ? 0.0 : 0.25 * (M_l + 1.0) * (M_l + 1.0);
   M_r = (M_r >= 1.0) ? 0.0 : (M_r <= -1.0) ? M_r : -(0.25 * (M_r - 1.0) * (M_r - 1.0));
   const double fp0 = rho_l * c_l * M_l;
   const double fm0 = rho_r * c_r * M_r;
   F[0] = fp0 + fm0;
   const double gp = -Vc_l + 2.0 * c_l, gm = -Vc_r - 2.0 * c_r;
-  const double rg = frcp(gamma);
+  const double rg = kFast ? frcp(gamma) : 0.0;
+  const Recip rgam = kFast ? Recip{gamma, 0.0} : erecip(gamma);
+  // (0.0 / gamma is the reference's +0 for the tangential components)
 #pragma unroll
   for (int m = 0; m < NC - 2; ++m) {
     double a, b;
     if (m == D && kFast) { a = fp0 * (gp * rg + vl[m]); b = fm0 * (gm * rg + vr[m]); }
-    else if (m == D) { a = fp0 * (gp / gamma + vl[m]); b = fm0 * (gm / gamma + vr[m]); }
-    else { a = fp0 * (0.0 / gamma + vl[m]); b = fm0 * (0.0 / gamma + vr[m]); }
+    else if (m == D) { a = fp0 * (ediv(gp, rgam) + vl[m]); b = fm0 * (ediv(gm, rgam) + vr[m]); }
+    else { a = fp0 * (0.0 + vl[m]); b = fm0 * (0.0 + vr[m]); }
     F[1 + m] = a + b;
   }
   const double tl = (gamma - 1.0) * Vc_l + 2.0 * c_l;
@@ -489,8 +549,9 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
               + fm0 * (tr * tr * rden + (q2_r - Vc_r * Vc_r) * 0.5);
     return;
   }
-  F[NC - 1] = fp0 * (tl * tl / den + (q2_l - Vc_l * Vc_l) / 2.0)
-            + fm0 * (tr * tr / den + (q2_r - Vc_r * Vc_r) / 2.0);
+  const Recip rden = erecip(den);
+  F[NC - 1] = fp0 * (ediv(tl * tl, rden) + (q2_l - Vc_l * Vc_l) / 2.0)
+            + fm0 * (ediv(tr * tr, rden) + (q2_r - Vc_r * Vc_r) / 2.0);
 }
 
 // ---- "AUSM" as the reference implements it, src/scheme/scheme_AUSM.f90 -------
@@ -509,12 +570,14 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   if constexpr (NC == 5) { ke_l = vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]; ke_r = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]; }
   else { ke_l = vl[0] * vl[0] + vl[1] * vl[1]; ke_r = vr[0] * vr[0] + vr[1] * vr[1]; }
   const double ir_l = kFast ? frcp(rho_l) : 0.0, ir_r = kFast ? frcp(rho_r) : 0.0;
-  const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-  const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-  const double c_l = kFast ? fsqrt(gamma * p_l * ir_l) : sqrt(gamma * p_l / rho_l);
-  const double c_r = kFast ? fsqrt(gamma * p_r * ir_r) : sqrt(gamma * p_r / rho_r);
-  const double M_l = kFast ? Vc_l * frcp(c_l) : Vc_l / c_l;
-  const double M_r = kFast ? Vc_r * frcp(c_r) : Vc_r / c_r;
+  const Recip er_l = kFast ? Recip{rho_l, 0.0} : erecip(rho_l);
+  const Recip er_r = kFast ? Recip{rho_r, 0.0} : erecip(rho_r);
+  const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : ediv(P.g_gm1 * p_l, er_l) + 0.5 * ke_l;
+  const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : ediv(P.g_gm1 * p_r, er_r) + 0.5 * ke_r;
+  const double c_l = kFast ? fsqrt(gamma * p_l * ir_l) : esqrt(ediv(gamma * p_l, er_l));
+  const double c_r = kFast ? fsqrt(gamma * p_r * ir_r) : esqrt(ediv(gamma * p_r, er_r));
+  const double M_l = kFast ? Vc_l * frcp(c_l) : ediv(Vc_l, c_l);
+  const double M_r = kFast ? Vc_r * frcp(c_r) : ediv(Vc_r, c_r);
   // Mach split, beta = 0: the beta term adds an exact +0 for |M| < 1
   const double mp = (fabs(M_l) >= 1.0) ? 0.5 * (M_l + 1.0 * fabs(M_l))
                                        : 1.0 * (0.25 * ((M_l + 1.0) * (M_l + 1.0)));
@@ -525,7 +588,7 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   const double delta = 0.1;
   const double Mabs = (aMn > delta) ? aMn
                     : (kFast ? 5.0 * (aMn * aMn + delta * delta)
-                             : 0.5 * (aMn * aMn + delta * delta) / delta);
+                             : ediv(0.5 * (aMn * aMn + delta * delta), delta));
   // pressure split, alpha = 0: the alpha term adds an exact zero for |M| < 1
   const double pp = ((fabs(M_l) >= 1.0) ? 0.5 * (1.0 + 1.0 * fsign1(M_l))
                                         : 0.25 * ((M_l + 1.0) * (M_l + 1.0)) * (2.0 - 1.0 * M_l)) * p_l;
@@ -581,16 +644,18 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
     M_l = Vc_l * rc;
     M_r = Vc_r * rc;
   } else {
-    H_l = P.g_gm1 * p_l / rho_l + 0.5 * ke_l;
-    H_r = P.g_gm1 * p_r / rho_r + 0.5 * ke_r;
-    const double cs2_l = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_l;
-    const double cs2_r = 2.0 * (gamma - 1.0) / (gamma + 1.0) * H_r;
-    const double cs_l = sqrt(cs2_l), cs_r = sqrt(cs2_r);
-    const double ct_l = cs2_l / fmax(cs_l, Vc_l);
-    const double ct_r = cs2_r / fmax(cs_r, -Vc_r);
+    H_l = ediv(P.g_gm1 * p_l, rho_l) + 0.5 * ke_l;
+    H_r = ediv(P.g_gm1 * p_r, rho_r) + 0.5 * ke_r;
+    const double k2 = ediv(2.0 * (gamma - 1.0), gamma + 1.0);
+    const double cs2_l = k2 * H_l;
+    const double cs2_r = k2 * H_r;
+    const double cs_l = esqrt(cs2_l), cs_r = esqrt(cs2_r);
+    const double ct_l = ediv(cs2_l, fmax(cs_l, Vc_l));
+    const double ct_r = ediv(cs2_r, fmax(cs_r, -Vc_r));
     c12 = fmin(ct_l, ct_r);
-    M_l = Vc_l / c12;
-    M_r = Vc_r / c12;
+    const Recip rc12 = erecip(c12);
+    M_l = ediv(Vc_l, rc12);
+    M_r = ediv(Vc_r, rc12);
   }
   double m4p, m4m, p5p, p5m;
   if (fabs(M_l) >= 1.0) {

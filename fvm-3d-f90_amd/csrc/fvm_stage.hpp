@@ -328,9 +328,9 @@ __device__ __forceinline__ double wave_speed(const double q[NC], const BlockGeom
     if constexpr (NC == 5) s = s + (fabs(p[3]) + a) * g.rdz;
     return s;
   }
-  const double a = sqrt(gamma * p[NC - 1] / p[0]);
-  double s = (fabs(p[1]) + a) / g.dx + (fabs(p[2]) + a) / g.dy;
-  if constexpr (NC == 5) s = s + (fabs(p[3]) + a) / g.dz;
+  const double a = esqrt(ediv(gamma * p[NC - 1], p[0]));
+  double s = ediv(fabs(p[1]) + a, g.dx) + ediv(fabs(p[2]) + a, g.dy);
+  if constexpr (NC == 5) s = s + ediv(fabs(p[3]) + a, g.dz);
   return s;
 }
 
@@ -416,11 +416,18 @@ __device__ __forceinline__ BlockGeom load_geom(const G& s) {
   return g;
 }
 
+// The block owning `tile`: the last block whose first tile is at or before
+// it (tile bases ascend in table order); a binary search, so a launch over
+// hundreds of served solves costs a few table reads per workgroup.
 template <class Tab>
 __device__ __forceinline__ int block_of_tile(Tab tab, int nblocks, int tile) {
-  int b = 0;
-  while (b + 1 < nblocks && tab[b + 1].tile_base <= tile) ++b;
-  return b;
+  int lo = 0, hi = nblocks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].tile_base <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // Workgroup reduction helpers (deterministic order).
@@ -581,6 +588,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
       __syncthreads();
       if (owner && j < g.ny) {
         const int64_t ci = gidx(g, c, j, 0);
+        const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
         double out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
@@ -589,8 +597,8 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
             r = -(Fx[m][t] - Fx[m][t - 1]) * g.rdx;
             r = r - (Fn[m] - Fc[m]) * g.rdy;
           } else {
-            r = 0.0 - (Fx[m][t] - Fx[m][t - 1]) / g.dx;
-            r = r - (Fn[m] - Fc[m]) / g.dy;
+            r = 0.0 - ediv(Fx[m][t] - Fx[m][t - 1], ix);
+            r = r - ediv(Fn[m] - Fc[m], iy);
           }
           if (m > 0 && rvp) r = r + rvp[(int64_t)(m - 1) * g.cs + ci];
           const double q0 = q0p[ci + m * g.cs];
@@ -770,6 +778,7 @@ __global__ __launch_bounds__(TX * TY) void stage2d_tile_kernel(StageArgs a) {
     for (int m = 0; m < NC; ++m) nrm[m] = 0.0;
     if (owner) {
       const uint32_t cidx = gidx32(g, c, r, 0);
+      const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
       double rr[NC], out[NC];
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
@@ -778,8 +787,8 @@ __global__ __launch_bounds__(TX * TY) void stage2d_tile_kernel(StageArgs a) {
         if constexpr (kFast) {
           rr[m] = -(fxd * g.rdx) - fyd * g.rdy;
         } else {
-          rr[m] = 0.0 - fxd / g.dx;
-          rr[m] = rr[m] - fyd / g.dy;
+          rr[m] = 0.0 - ediv(fxd, ix);
+          rr[m] = rr[m] - ediv(fyd, iy);
         }
       }
       if (rvp) {
@@ -1111,6 +1120,8 @@ void stage3d_kernel(StageArgs a) {
       // ---- D: update
       if (owner) {
         const uint32_t cidx = gidx32(g, c, r, k);
+        const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
+        const Recip iz = kFast ? Recip{} : erecip(g.dz);
         double rr[NC], out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
@@ -1119,9 +1130,9 @@ void stage3d_kernel(StageArgs a) {
           if constexpr (kFast) {
             rr[m] = -(fxd * g.rdx) - fyd * g.rdy - (Fn[m] - Fc[m]) * g.rdz;
           } else {
-            rr[m] = 0.0 - fxd / g.dx;
-            rr[m] = rr[m] - fyd / g.dy;
-            rr[m] = rr[m] - (Fn[m] - Fc[m]) / g.dz;
+            rr[m] = 0.0 - ediv(fxd, ix);
+            rr[m] = rr[m] - ediv(fyd, iy);
+            rr[m] = rr[m] - ediv(Fn[m] - Fc[m], iz);
           }
           if constexpr (kPrefetchRK != 3) {
             if (m == 0) rk_prefetch<NC, 3 - kPrefetchRK>(a.stage, true, cidx, g.cs, q0p, qin, acc, pq0, px);
@@ -1378,6 +1389,8 @@ void stage3d_reg_kernel(StageArgs a) {
       for (int m = 0; m < NC; ++m) rr[m] = 0.0;
       if (owner) {
         const uint32_t cidx = gidx32(g, c, r, k);
+        const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
+        const Recip iz = kFast ? Recip{} : erecip(g.dz);
         // operands of the RK combine (loaded here: registers are the limit
         // during the face tasks, and the other waves cover the latency)
         double pq0[NC], px[NC], out[NC];
@@ -1390,9 +1403,9 @@ void stage3d_reg_kernel(StageArgs a) {
           if constexpr (kFast) {
             rr[m] = -(fxd * g.rdx) - fyd * g.rdy - fzd * g.rdz;
           } else {
-            rr[m] = 0.0 - fxd / g.dx;
-            rr[m] = rr[m] - fyd / g.dy;
-            rr[m] = rr[m] - fzd / g.dz;
+            rr[m] = 0.0 - ediv(fxd, ix);
+            rr[m] = rr[m] - ediv(fyd, iy);
+            rr[m] = rr[m] - ediv(fzd, iz);
           }
         }
         if (rvp) {
