@@ -1538,6 +1538,11 @@ int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
     return fail(FVM_EINVAL, "a face already has connection patches");
   drop_face_patches(ctx, a, face);
   drop_face_patches(ctx, b, bface);
+  // a periodic direction cut open by this connection: the opposite face of
+  // either block no longer wraps onto this one (as set_face_kind does when a
+  // periodic face gets another kind)
+  if (a->g.bc[face ^ 1] == BC_PERIODIC) a->g.bc[face ^ 1] = BC_EXTRAP;
+  if (b->g.bc[bface ^ 1] == BC_PERIODIC) b->g.bc[bface ^ 1] = BC_EXTRAP;
   const int ia = block_index(ctx, a), ib = block_index(ctx, b);
   a->g.bc[face] = BC_CONN; a->nbr[face] = ib; a->nbr_shared[face] = shared ? 1 : 0;
   b->g.bc[bface] = BC_CONN; b->nbr[bface] = ia; b->nbr_shared[bface] = shared ? 1 : 0;

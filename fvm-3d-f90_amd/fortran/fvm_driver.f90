@@ -71,6 +71,7 @@ program fvm_driver
   character(len=512) :: arg, casefile
   integer :: unit, ios, i, j, k, ni, nj, nk, step, nan_flag, eq
   real(dp) :: dx, dy, dz, gamma, time, dt, dt_used, res(6), res0(6)
+  logical :: have_res0
   real(dp), allocatable, target :: q(:,:,:,:)
   type(c_ptr) :: ctx
   type(c_ptr), allocatable :: pieces(:,:,:)
@@ -219,7 +220,15 @@ program fvm_driver
     step = int(step0)
     write(*, '(a, a, a, i0, a, es14.6)') "restarted from ", trim(restart_file), " at step ", &
       step, ", t = ", time
+    ! the output times already passed (written by the run that saved the
+    ! restart file), accumulated as write_output accumulates them
+    if (writing) then
+      do while (time_out <= time + 1.0e-9_dp)
+        time_out = time_out + output_time
+      end do
+    end if
   end if
+  have_res0 = .false.
   converged = .false.
   if (writing .and. step == 0) call write_output()          ! the initial flowfield
   do while (.not. converged)
@@ -237,7 +246,10 @@ program fvm_driver
       if (abs(time - time_out) <= 1.0e-9_dp) call write_output()
     end if
     call fvm_check(fvm_norms(ctx, res, nan_flag), "norms")
-    if (step == 1) res0 = res
+    if (.not. have_res0) then   ! relative residuals against this run's first step
+      res0 = res
+      have_res0 = .true.
+    end if
     if (nan_flag /= 0) then
       write(*, '(a, es12.4)') "Error: ABSRES ", res(6)
       error stop 3

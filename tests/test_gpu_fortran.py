@@ -119,13 +119,24 @@ def test_fortran_checkpoint_restart(tmp_path):
         return steps, t, np.frombuffer(raw[28:], dtype=np.float64)
 
     print("case fortran checkpoint and restart", flush=True)
+    base = base.replace("output_time  = 0.01", "output_time  = 1.0E-3")   # every 5 steps
     ck = tmp_path / "state.fvmsnap"
     s1, t1, _ = run("first", max_step=10, checkpoint_file=f'"{ck}"')
     assert s1 == 10
-    s2, t2, q2 = run("second", max_step=20, restart_file=f'"{ck}"')
-    s3, t3, q3 = run("whole", max_step=20)
+    s2, t2, q2 = run("second", max_step=20, restart_file=f'"{ck}"',
+                     snapshot_prefix=f'"{tmp_path / "second"}"')
+    s3, t3, q3 = run("whole", max_step=20, snapshot_prefix=f'"{tmp_path / "whole"}"')
     assert s2 == s3 == 20 and t2 == t3
     assert np.array_equal(q2, q3)
+    # output snapshots after the restart continue at the uninterrupted run's
+    # output times (steps 15 and 20), and equal its snapshots
+    for step in (15, 20):
+        a = fvmgpu.read_snapshot_file(str(tmp_path / f"second_{step:06d}.fvmsnap"))
+        b = fvmgpu.read_snapshot_file(str(tmp_path / f"whole_{step:06d}.fvmsnap"))
+        assert a["time"] == b["time"] and a["step"] == b["step"] == step
+        assert np.array_equal(a["blocks"][0][2], b["blocks"][0][2])
+    for step in (0, 5, 10):   # already written before the checkpoint
+        assert not (tmp_path / f"second_{step:06d}.fvmsnap").exists()
 
 
 @pytest.mark.parametrize("dim,ncut", [("2D", "3, 2, 1"), ("2D", "2, 4, 1"), ("3D", "2, 2, 2")])

@@ -206,3 +206,28 @@ def test_cut_uniform_equals_single_block(ncut, periodic):
             out[:, z0:z0 + q.shape[1], y0:y0 + q.shape[2], x0:x0 + q.shape[3]] = q
     qo, _ = run_oracle(p, sp, 4, dt=2e-3, bc=bc, **scheme)
     assert np.array_equal(out, qo)
+
+
+def test_connect_cuts_a_periodic_direction_open():
+    """A block made periodic in x and then connected on its i+ face: its i-
+    face no longer wraps onto itself but extrapolates (the same result as
+    setting it so explicitly), so the face table stays consistent."""
+    p = cases.smooth_3d((10, 12, 30), seed=8)
+    sp = (1 / 29, 1 / 11, 1 / 9)
+    q = cases.to_conserved(p)
+    out = []
+    for periodic_first in (True, False):
+        with fvmgpu.Context() as ctx:
+            ctx.scheme(recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE, rk=O.RK_SSP3)
+            a = ctx.block((16, 12, 10), sp)
+            b = ctx.block((15, 12, 10), sp)
+            a.upload(q[:, :, :, :16])
+            b.upload(q[:, :, :, 15:])
+            if periodic_first:
+                a.set_bc(0, fvmgpu.BC_PERIODIC)
+            a.connect(1, b, shared=True)
+            ctx.step(3, dt=1e-3)
+            out.append((a.download(), b.download()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    qo, _ = run_oracle(p, sp, 3, dt=1e-3, recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE)
+    assert np.array_equal(out[0][0], qo[:, :, :, :16])
