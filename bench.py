@@ -5,6 +5,9 @@ Default workload: 3-D Euler isentropic vortex, 128^3 vertices on a periodic
 with the van Albada limiter + Roe flux, classical RK4, CFL 0.8, FP64.
 One bench step = one full RK4 time step (4 fused stage launches + the CFL
 dt update) over the whole grid; the state is resident in HBM throughout.
+Setup before the W warm-up steps: the launch-shape autotuner and 0.5 s of
+untimed steps while the GPU's clocks settle.  Numerics: the exact family
+(bit for bit the oracle), the one that meets the 1000-step tolerance here.
 
 Metric: FP64 cell-updates per second per RK stage
         = vertices x stages x steps / wall time of the steps.
@@ -202,6 +205,7 @@ def bench_vortex(args):
         ctx.autotune(reps=getattr(args, "tune_reps", 10))
     tile = blk.launch_shape()
     reg = blk.launch_tile() >= 4          # the register-reconstruction kernel
+    settle(ctx, cfl)
     if args.warmup:
         ctx.step(args.warmup, cfl=cfl)
     ctx.sync()
@@ -268,6 +272,18 @@ def bench_vortex(args):
                                      bc=(O.BC_PERIODIC,) * 6),
             ncell * 4 * cs, f"{cs} RK4 step(s) of the {n}^3 vortex workload")
     return line(args, value, steps, sec, config, roof, cpu)
+
+
+def settle(ctx, cfl, seconds=0.5):
+    """Untimed steps until the GPU has run the workload for `seconds`: a
+    fresh box ramps its clocks over the first fraction of a second under
+    load, and a 20-step timed window would otherwise measure the ramp.  Part
+    of the setup, like the autotuner; the state advances, the timed region
+    is unchanged."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        ctx.step(8, cfl=cfl)
+        ctx.sync()
 
 
 def timed_with_output(fvmgpu, cases, ctx, blk, steps, cfl, every, mode):

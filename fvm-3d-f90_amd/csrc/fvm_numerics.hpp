@@ -294,9 +294,9 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
     const double b0 = c_0 * (E1 * E2), b1 = c_1 * (E0 * E2), b2 = c_2 * (E0 * E1);
     return (b0 * q0 + b1 * q1 + b2 * q2) * frcp(b0 + b1 + b2);
   }
-  const double a0 = ediv(c_0, e0 * e0);
-  const double a1 = ediv(c_1, e1 * e1);
-  const double a2 = ediv(c_2, e2 * e2);
+  const double a0 = c_0 / (e0 * e0);
+  const double a1 = c_1 / (e1 * e1);
+  const double a2 = c_2 / (e2 * e2);
   const Recip ras = erecip(a0 + a1 + a2);
   const double w0 = ediv(a0, ras), w1 = ediv(a1, ras), w2 = ediv(a2, ras);
   return w0 * q0 + w1 * q1 + w2 * q2;

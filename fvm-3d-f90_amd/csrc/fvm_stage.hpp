@@ -866,14 +866,14 @@ void stage3d_kernel(StageArgs a) {
   __shared__ double LRx[NLR][NC * LRX];
   __shared__ double LRy[NLR][NC * LRY];
   __shared__ double red[NT / 64];
-  __shared__ double Nrm[NC * NT];     // per-thread rhs^2 sums (norm stage only)
+  __shared__ double NrmW[(NT / 64) * NC];   // per-wave rhs^2 sums (norm stage only)
   double* Fx = Vs;                 // [NC][FXN] after phase B
   double* Fy = Vs + NC * FXN;      // [NC][FYN]
 
   const Params& P = a.P;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  const int wave = t / 64;
+  const int wave = t / 64, lane = t % 64;
   const bool want_smax = a.smax != nullptr || a.block_smax;
 
   double dt_step = 0.0;
@@ -966,12 +966,7 @@ void stage3d_kernel(StageArgs a) {
       }
     }
 
-    if (want_norm) {
-#pragma unroll
-      for (int m = 0; m < NC; ++m) Nrm[m * NT + t] = 0.0;
-    }
-#pragma unroll
-
+    if (want_norm && t < (NT / 64) * NC) NrmW[t] = 0.0;
 
     for (int k = k0; k < k1; ++k) {
       // ---- A: queue shift (centre k+1), plane k into LDS, next loads
@@ -1118,11 +1113,14 @@ void stage3d_kernel(StageArgs a) {
       __syncthreads();
 
       // ---- D: update
+      double rr[NC];
+#pragma unroll
+      for (int m = 0; m < NC; ++m) rr[m] = 0.0;
       if (owner) {
         const uint32_t cidx = gidx32(g, c, r, k);
         const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
         const Recip iz = kFast ? Recip{} : erecip(g.dz);
-        double rr[NC], out[NC];
+        double out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
           const double fxd = Fx[m * FXN + ty * FXW + tx + 1] - Fx[m * FXN + ty * FXW + tx];
@@ -1143,12 +1141,16 @@ void stage3d_kernel(StageArgs a) {
           for (int m = 1; m < NC; ++m) rr[m] = rr[m] + ld_comp(rvp, g.cs, m - 1, cidx);
         }
         rk_apply_all<NC>(a.stage, kc, pq0, px, rr, out, acc, g.cs, cidx);
-        if (want_norm) {
-#pragma unroll
-          for (int m = 0; m < NC; ++m) Nrm[m * NT + t] += rr[m] * rr[m];
-        }
         store_with_ghosts<NC, 3, CONN>(g, tab, B, a.out_buf, qout, c, r, k, out);
         if (want_smax) smax = nan_max(smax, wave_speed<NC>(out, g, P.gamma));
+      }
+      if (want_norm) {   // uniform branch: each wave's sum of rhs^2 over its plane
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          double v = rr[m] * rr[m];
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+          if (lane == 0) NrmW[wave * NC + m] += v;
+        }
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) Fc[m] = Fn[m];
@@ -1156,11 +1158,15 @@ void stage3d_kernel(StageArgs a) {
     }
 
     if (want_norm) {
+      if (t == 0) {
 #pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        const double s = wg_sum<NT>(Nrm[m * NT + t], red);
-        if (t == 0) a.norm_part[(int64_t)tile * NC + m] = s;
+        for (int m = 0; m < NC; ++m) {
+          double s = 0.0;
+          for (int w = 0; w < NT / 64; ++w) s += NrmW[w * NC + m];
+          a.norm_part[(int64_t)tile * NC + m] = s;
+        }
       }
+      __syncthreads();   // the next tile clears the sums
     }
     if (want_smax) {
       const double s = wg_max<NT>(smax, red);

@@ -36,6 +36,25 @@ constexpr bool kFastV = true;
 constexpr bool kFastV = false;
 #endif
 
+// Reciprocals of the divisors every vertex and face shares (exact family):
+// the spacings, four times the spacings and T_ref, formed once per thread.
+struct Rec {
+  Recip dr[3], dr4[3], tref;
+};
+__device__ __forceinline__ Rec make_rec(const BlockGeom& g, const ViscParams& V) {
+  Rec R;
+  if constexpr (!kFastV) {
+    const double dr[3] = {g.dx, g.dy, g.dz};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      R.dr[e] = erecip(dr[e]);
+      R.dr4[e] = erecip(4.0 * dr[e]);
+    }
+    R.tref = erecip(V.t_ref);
+  }
+  return R;
+}
+
 // index of a stencil point along one direction: periodic wrap or clamp
 __device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
   if (x < 0) return lo_bc == BC_PERIODIC ? ((x % n) + n) % n : 0;
@@ -46,7 +65,7 @@ __device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
 // u, v, w, T, mu of the conserved state at vertex (i, j, k) (already mapped)
 template <int NC>
 __device__ __forceinline__ void prims(const double* __restrict__ q, const BlockGeom& g, int i, int j,
-                                      int k, const ViscParams& V, double out[5]) {
+                                      int k, const ViscParams& V, const Rec& R, double out[5]) {
   const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
   const double rho = q[c];
   if constexpr (kFastV) {
@@ -65,18 +84,19 @@ __device__ __forceinline__ void prims(const double* __restrict__ q, const BlockG
     out[4] = V.mu_ref * tr * fsqrt(tr) * V.sfac * frcp(t + V.s);
     return;
   }
-  const double u = q[c + g.cs] / rho, v = q[c + 2 * g.cs] / rho;
-  const double w = NC == 5 ? q[c + 3 * g.cs] / rho : 0.0;
+  const Recip ir = erecip(rho);
+  const double u = ediv(q[c + g.cs], ir), v = ediv(q[c + 2 * g.cs], ir);
+  const double w = NC == 5 ? ediv(q[c + 3 * g.cs], ir) : 0.0;
   const double e = q[c + (NC - 1) * g.cs];
   // class_flow_field.f90:383-396 operation order
   const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
-  const double t = p / (rho * V.r_gas);
-  const double tr = t / V.t_ref;
+  const double t = ediv(p, rho * V.r_gas);
+  const double tr = ediv(t, R.tref);
   out[0] = u;
   out[1] = v;
   out[2] = w;
   out[3] = t;
-  out[4] = V.mu_ref * tr * sqrt(tr) * (V.t_ref + V.s) / (t + V.s);
+  out[4] = ediv(V.mu_ref * tr * esqrt(tr) * (V.t_ref + V.s), t + V.s);
 }
 
 // Face flux in direction D between vertex A (offset O along D from the
@@ -84,7 +104,7 @@ __device__ __forceinline__ void prims(const double* __restrict__ q, const BlockG
 // at an offset from the centre vertex.
 template <int DIM, int D, int O, class At>
 __device__ __forceinline__ void face(const At& at, const BlockGeom& g, const ViscParams& V,
-                                     double F[4]) {
+                                     const Rec& R, double F[4]) {
   constexpr int a0 = D == 0 ? O : 0, a1 = D == 1 ? O : 0, a2 = D == 2 ? O : 0;
   constexpr int b0 = a0 + (D == 0), b1 = a1 + (D == 1), b2 = a2 + (D == 2);
   const double dr[3] = {g.dx, g.dy, g.dz};
@@ -97,7 +117,7 @@ __device__ __forceinline__ void face(const At& at, const BlockGeom& g, const Vis
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const double diff = at(m, b0, b1, b2) - at(m, a0, a1, a2);
-    gq[D][m] = kFastV ? diff * rdr[D] : diff / dr[D];
+    gq[D][m] = kFastV ? diff * rdr[D] : ediv(diff, R.dr[D]);
   }
 #pragma unroll
   for (int e = 0; e < DIM; ++e) {
@@ -107,7 +127,7 @@ __device__ __forceinline__ void face(const At& at, const BlockGeom& g, const Vis
     for (int m = 0; m < 4; ++m) {
       const double sum = (at(m, a0 + s0, a1 + s1, a2 + s2) - at(m, a0 - s0, a1 - s1, a2 - s2)) +
                          (at(m, b0 + s0, b1 + s1, b2 + s2) - at(m, b0 - s0, b1 - s1, b2 - s2));
-      gq[e][m] = kFastV ? sum * (0.25 * rdr[e]) : sum / (4.0 * dr[e]);
+      gq[e][m] = kFastV ? sum * (0.25 * rdr[e]) : ediv(sum, R.dr4[e]);
     }
   }
   const double mu = 0.5 * (at(4, a0, a1, a2) + at(4, b0, b1, b2));
@@ -130,23 +150,23 @@ __device__ __forceinline__ void face(const At& at, const BlockGeom& g, const Vis
 
 template <int DIM, class At>
 __device__ __forceinline__ void residual(const At& at, const BlockGeom& g, const ViscParams& V,
-                                         double acc[4]) {
+                                         const Rec& R, double acc[4]) {
   double Fp[4], Fm[4];
-  face<DIM, 0, 0>(at, g, V, Fp);
-  face<DIM, 0, -1>(at, g, V, Fm);
+  face<DIM, 0, 0>(at, g, V, R, Fp);
+  face<DIM, 0, -1>(at, g, V, R, Fm);
 #pragma unroll
-  for (int m = 0; m < 4; ++m) acc[m] = kFastV ? (Fp[m] - Fm[m]) * g.rdx : (Fp[m] - Fm[m]) / g.dx;
-  face<DIM, 1, 0>(at, g, V, Fp);
-  face<DIM, 1, -1>(at, g, V, Fm);
+  for (int m = 0; m < 4; ++m) acc[m] = kFastV ? (Fp[m] - Fm[m]) * g.rdx : ediv(Fp[m] - Fm[m], R.dr[0]);
+  face<DIM, 1, 0>(at, g, V, R, Fp);
+  face<DIM, 1, -1>(at, g, V, R, Fm);
 #pragma unroll
   for (int m = 0; m < 4; ++m)
-    acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + (Fp[m] - Fm[m]) / g.dy;
+    acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + ediv(Fp[m] - Fm[m], R.dr[1]);
   if constexpr (DIM == 3) {
-    face<DIM, 2, 0>(at, g, V, Fp);
-    face<DIM, 2, -1>(at, g, V, Fm);
+    face<DIM, 2, 0>(at, g, V, R, Fp);
+    face<DIM, 2, -1>(at, g, V, R, Fm);
 #pragma unroll
     for (int m = 0; m < 4; ++m)
-      acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdz : acc[m] + (Fp[m] - Fm[m]) / g.dz;
+      acc[m] = kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdz : acc[m] + ediv(Fp[m] - Fm[m], R.dr[2]);
   }
 }
 
@@ -165,6 +185,7 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
   const int i0 = (rem % a.tiles_x) * TX, j0 = (rem / a.tiles_x) * TY;
   const int k0 = tk * a.chunk, k1 = min(k0 + a.chunk, g.nz);
   const int i = i0 + tx, j = j0 + ty;
+  const visc::Rec R = visc::make_rec(g, V);
 
   auto load_plane = [&](int k, int slot) {
     const int mk = visc::vmap(k, g.nz, g.bc[4], g.bc[5]);
@@ -172,7 +193,7 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
       const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
       const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
       double v[5];
-      visc::prims<5>(a.q, g, mi, mj, mk, V, v);
+      visc::prims<5>(a.q, g, mi, mj, mk, V, R, v);
 #pragma unroll
       for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
     }
@@ -193,20 +214,21 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
       // x and y faces as visc::residual; the k- face is the previous
       // plane's k+ face (same points, same arithmetic), carried in registers
       double acc[4], Fp[4], Fm[4];
-      visc::face<3, 0, 0>(at, g, V, Fp);
-      visc::face<3, 0, -1>(at, g, V, Fm);
-#pragma unroll
-      for (int m = 0; m < 4; ++m) acc[m] = visc::kFastV ? (Fp[m] - Fm[m]) * g.rdx : (Fp[m] - Fm[m]) / g.dx;
-      visc::face<3, 1, 0>(at, g, V, Fp);
-      visc::face<3, 1, -1>(at, g, V, Fm);
+      visc::face<3, 0, 0>(at, g, V, R, Fp);
+      visc::face<3, 0, -1>(at, g, V, R, Fm);
 #pragma unroll
       for (int m = 0; m < 4; ++m)
-        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + (Fp[m] - Fm[m]) / g.dy;
-      visc::face<3, 2, 0>(at, g, V, Fp);
-      if (k == k0) visc::face<3, 2, -1>(at, g, V, Fzm);
+        acc[m] = visc::kFastV ? (Fp[m] - Fm[m]) * g.rdx : ediv(Fp[m] - Fm[m], R.dr[0]);
+      visc::face<3, 1, 0>(at, g, V, R, Fp);
+      visc::face<3, 1, -1>(at, g, V, R, Fm);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fm[m]) * g.rdy : acc[m] + ediv(Fp[m] - Fm[m], R.dr[1]);
+      visc::face<3, 2, 0>(at, g, V, R, Fp);
+      if (k == k0) visc::face<3, 2, -1>(at, g, V, R, Fzm);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fzm[m]) * g.rdz : acc[m] + (Fp[m] - Fzm[m]) / g.dz;
+        acc[m] = visc::kFastV ? acc[m] + (Fp[m] - Fzm[m]) * g.rdz : acc[m] + ediv(Fp[m] - Fzm[m], R.dr[2]);
         Fzm[m] = Fp[m];
       }
       const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
@@ -231,11 +253,12 @@ __global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
   const int i0 = (blockIdx.x % a.tiles_x) * TX, j0 = (blockIdx.x / a.tiles_x) * TY;
   const int i = i0 + tx, j = j0 + ty;
+  const visc::Rec R = visc::make_rec(g, V);
   for (int e = t; e < PN; e += NT) {
     const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
     const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
     double v[5];
-    visc::prims<4>(a.q, g, mi, mj, 0, V, v);
+    visc::prims<4>(a.q, g, mi, mj, 0, V, R, v);
 #pragma unroll
     for (int m = 0; m < 5; ++m) S[m][e] = v[m];
   }
@@ -244,7 +267,7 @@ __global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
     const int ctr = (ty + 1) * W + (tx + 1);
     auto at = [&](int m, int dx, int dy, int) -> double { return S[m][ctr + dy * W + dx]; };
     double acc[4];
-    visc::residual<2>(at, g, V, acc);
+    visc::residual<2>(at, g, V, R, acc);
     const int64_t c = (int64_t)(j + g.oy) * g.px + (i + g.ox);
     // 2-D state: rho, rho u, rho v, E -> residual components rho u, rho v, E
     a.rv[c] = acc[0];
