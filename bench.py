@@ -373,8 +373,9 @@ def bench_riemann(args):
     sec = t1 - t0
     ncell = float(n) * n
     value = ncell * 3 * steps / sec
-    roof = roofline(stage_ms, launches, ncell, B_ALG["ssp3_2d"], None,
-                    f"fvm::{args.numerics}::stage2d_kernel<ReconMuscl, 300>")
+    kname = (f"fvm::{args.numerics}::stage2d_kernel<ReconMuscl, 300>" if tile[1] == 1 else
+             f"fvm::{args.numerics}::stage2d_tile_kernel<ReconMuscl, 300, {tile[0]}, {tile[1]}>")
+    roof = roofline(stage_ms, launches, ncell, B_ALG["ssp3_2d"], None, kname)
     config = {"workload": f"riemann-2d {n}x{n} four-quadrant (reference states), MUSCL-minmod "
                           f"+ Roe, SSP-RK3, CFL {cfl}, transmissive, t_end {t_end}, FP64",
               "grid": [n, n, 1], "stages_per_step": 3, "t_final": t, "numerics": args.numerics,

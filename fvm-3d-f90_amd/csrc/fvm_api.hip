@@ -53,6 +53,7 @@ int fail(int code, const char* fmt, ...) {
 constexpr int kGhost = 3;   // ghost layers per side (src/base/defs_ghost_cell.f90:42)
 constexpr int kSnapSlots = 4;   // snapshots in flight per context
 constexpr int kSnapMaxVars = 10;
+const SmaxWays kZeroSmax{};     // uploaded to clear a wave-speed maximum
 
 }  // namespace
 
@@ -70,7 +71,7 @@ struct fvm_block {
   int tiles_x = 0, tiles_y = 0, tiles_z = 0, ntiles = 0, tile_base = 0;
   // independent solve (serving): own dt and CFL accumulator on the device
   const double* dt_dev = nullptr;
-  unsigned long long* smax_dev = nullptr;
+  SmaxWays* smax_dev = nullptr;
   const int* halt_dev = nullptr;
   double* rv = nullptr;                                  // viscous residual (nc - 1 components)
   // the four state arrays are one stream-ordered allocation starting at q[0]
@@ -141,7 +142,7 @@ struct fvm_ctx {
   SolveSlot* d_slot = nullptr;            // device time-stepping state (CFL mode)
   SolveSlot h_slot{};                     // host staging of d_slot
   double* d_dt = nullptr;                 // &d_slot->dt: CFL dt of the current step
-  unsigned long long* d_smax = nullptr;   // &d_slot->smax: wave-speed max of the current state
+  SmaxWays* d_smax = nullptr;             // wave-speed max of the current state (CFL mode)
   double time = 0.0;                      // solution time after the last step
   double t_end = HUGE_VAL;                // CFL steps stop here (halt), +inf: never
   int64_t steps_done = 0;
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(256) void init_kernel(double* __restrict__ q, Block
 // Max over real vertices of sum_d (|u_d| + c) / dr_d.
 template <int NC>
 __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__ q, BlockGeom g,
-                                                     double gamma, unsigned long long* smax) {
+                                                     double gamma, SmaxWays* smax) {
   __shared__ double red[4];
   const int64_t total = (int64_t)g.nx * g.ny * g.nz;
   double s = 0.0;
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__
     s = nan_max(s, w);
   }
   s = wg_max<256>(s, red);
-  if (threadIdx.x == 0) atomicMax(smax, (unsigned long long)__double_as_longlong(s));
+  if (threadIdx.x == 0) smax_add(smax, s);
 }
 
 // Time step of one served solve for the step about to start: dt =
@@ -469,18 +470,18 @@ __global__ __launch_bounds__(256) void cfl_max_kernel(const double* __restrict__
 // the step's last stage collects the next maximum.  Once t reaches t_end the
 // slot halts and the step's stage launches skip the solve.  (A plain
 // context's stages do the same work themselves: stage_clock.)
-__device__ __forceinline__ void advance_slot(SolveSlot& s) {
+__device__ __forceinline__ void advance_slot(SolveSlot& s, SmaxWays* smax) {
   if (s.halt) return;
   // (an infinite t_end never ends: inf - tol * inf would be NaN)
   if (isfinite(s.t_end) && !(s.t < s.t_end - kEndTol * fabs(s.t_end))) { s.halt = 1; return; }
-  const double sm = __longlong_as_double((long long)s.smax);
+  const double sm = smax_get(smax);
   double dt = s.cfl / sm;
   if (!(dt > 0.0) || !(dt < 1.0e300)) s.nan = 1;
   const double rest = s.t_end - s.t;
   if (dt > rest) dt = rest;
   s.dt = dt;
   s.t = s.t + dt;
-  s.smax = 0ull;
+  smax_clear(smax);
   s.steps = s.steps + 1;
 }
 
@@ -490,11 +491,11 @@ __global__ void set_dt_kernel(SolveSlot* slot, double dt) { slot->dt = dt; }
 // The same for the solves of a serving group (slot ids listed in ids).  A
 // solve whose dt is NaN, zero or infinite halts at once: its state stays the
 // last one computed, and no later launch writes it while it is retired.
-__global__ void solve_dt_kernel(SolveSlot* slots, const int* ids, int n) {
+__global__ void solve_dt_kernel(SolveSlot* slots, SmaxWays* smax, const int* ids, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     SolveSlot& sl = slots[ids[i]];
-    advance_slot(sl);
+    advance_slot(sl, smax + ids[i]);
     if (sl.nan) sl.halt = 1;
   }
 }
@@ -1072,9 +1073,10 @@ struct StepMem {
   const BlockDesc* desc;
   double* norm;
   SolveSlot* slot;
+  SmaxWays* smax;
 };
 
-StepMem ctx_mem(fvm_ctx* ctx) { return StepMem{ctx->d_desc, ctx->d_norm, ctx->d_slot}; }
+StepMem ctx_mem(fvm_ctx* ctx) { return StepMem{ctx->d_desc, ctx->d_norm, ctx->d_slot, ctx->d_smax}; }
 
 // Stage st (1-based) of one time step: its buffers, combine, norm and CFL
 // roles, then the launch over every block and the patch refill.
@@ -1107,9 +1109,10 @@ int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t*
     }
     a.norm_part = (last && norm_mode) ? mem.norm : nullptr;
     a.norm_if_end = last && norm_mode == 2 ? 1 : 0;
-    a.smax = (last && cfl_mode) ? &mem.slot->smax : nullptr;
+    a.smax = (last && cfl_mode) ? mem.smax : nullptr;
     a.halt = cfl_mode ? &mem.slot->halt : nullptr;
     a.slot = cfl_mode ? mem.slot : nullptr;
+    a.slot_smax = cfl_mode ? mem.smax : nullptr;
     a.cfl_dt = cfl_mode && st == 1 ? 1 : 0;
     a.commit = cfl_mode && st == 2 ? 1 : 0;
     const bool ev = nev != nullptr;
@@ -1217,7 +1220,8 @@ int step_graph(fvm_ctx* ctx, bool cfl_mode, int norm_mode, hipGraphExec_t* out) 
 constexpr int kScratchBlocks = 64;          // blocks of a context
 constexpr int kScratchTiles = 65536;        // stage tiles over all its blocks
 constexpr size_t scratch_round(size_t n) { return (n + 255) / 256 * 256; }
-constexpr size_t kScratchDescOff = scratch_round(sizeof(SolveSlot));
+constexpr size_t kScratchSmaxOff = scratch_round(sizeof(SolveSlot));
+constexpr size_t kScratchDescOff = kScratchSmaxOff + scratch_round(sizeof(SmaxWays));
 constexpr size_t kScratchSumOff = kScratchDescOff + scratch_round(kScratchBlocks * sizeof(BlockDesc));
 constexpr size_t kScratchNormOff = kScratchSumOff + scratch_round(kScratchBlocks * 5 * sizeof(double));
 constexpr size_t kScratchBytes = kScratchNormOff + (size_t)kScratchTiles * 5 * sizeof(double);
@@ -1291,8 +1295,9 @@ int fvm_ctx_create(int device, fvm_ctx** out) {
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc(&ctx->d_slot, sizeof(SolveSlot)));
   HIP_TRY(hipMemset(ctx->d_slot, 0, sizeof(SolveSlot)));
+  HIP_TRY(hipMalloc(&ctx->d_smax, sizeof(SmaxWays)));
+  HIP_TRY(hipMemset(ctx->d_smax, 0, sizeof(SmaxWays)));
   ctx->d_dt = &ctx->d_slot->dt;
-  ctx->d_smax = &ctx->d_slot->smax;
   *out = ctx;
   g_err.clear();
   return FVM_OK;
@@ -1320,6 +1325,7 @@ int fvm_ctx_destroy(fvm_ctx* ctx) {
     if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   (void)hipFree(ctx->d_desc);
   (void)hipFree(ctx->d_slot);
+  (void)hipFree(ctx->d_smax);
   (void)hipFree(ctx->d_norm);
   (void)hipFree(ctx->d_patch);
   (void)hipStreamDestroy(ctx->stream);
@@ -1743,6 +1749,8 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
     ctx->h_slot.cfl = cfl;
     HIP_TRY(hipMemcpyAsync(ctx->d_slot, &ctx->h_slot, sizeof(SolveSlot), hipMemcpyHostToDevice,
                            ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_smax, &kZeroSmax, sizeof(SmaxWays), hipMemcpyHostToDevice,
+                           ctx->stream));
     for (fvm_block* b : ctx->blocks) {
       if (b->nc == 5)
         hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g, ctx->P.gamma, ctx->d_smax);
@@ -1917,7 +1925,7 @@ int fvm_step_noalloc(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt
   ctx->have_norms = false;
   if (nsteps == 0) return FVM_OK;
   const StepMem mem{(const BlockDesc*)(base + kScratchDescOff), (double*)(base + kScratchNormOff),
-                    (SolveSlot*)base};
+                    (SolveSlot*)base, (SmaxWays*)(base + kScratchSmaxOff)};
   double* d_sums = (double*)(base + kScratchSumOff);
   // the block table, staged on the stack (the copy is complete before return)
   BlockDesc h_desc[kScratchBlocks];
@@ -1939,13 +1947,14 @@ int fvm_step_noalloc(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt
     ctx->h_slot.t_end = ctx->t_end;
     ctx->h_slot.cfl = cfl;
     HIP_TRY(hipMemcpyAsync(mem.slot, &ctx->h_slot, sizeof(SolveSlot), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(mem.smax, &kZeroSmax, sizeof(SmaxWays), hipMemcpyHostToDevice, ctx->stream));
     for (fvm_block* b : ctx->blocks) {
       if (b->nc == 5)
         hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g,
-                           ctx->P.gamma, &mem.slot->smax);
+                           ctx->P.gamma, mem.smax);
       else
         hipLaunchKernelGGL(cfl_max_kernel<4>, dim3(512), dim3(256), 0, ctx->stream, b->q[0], b->g,
-                           ctx->P.gamma, &mem.slot->smax);
+                           ctx->P.gamma, mem.smax);
       HIP_TRY(hipGetLastError());
     }
   } else {
@@ -2067,7 +2076,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
   // scratch word and events, released on every return path
   struct TuneRes {
     hipStream_t s;
-    unsigned long long* smax = nullptr;
+    SmaxWays* smax = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ~TuneRes() {
       (void)hipStreamSynchronize(s);
@@ -2076,12 +2085,12 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
       if (e1) (void)hipEventDestroy(e1);
     }
   } res{ctx->stream};
-  if (hipMalloc(&res.smax, sizeof(unsigned long long)) != hipSuccess) {
+  if (hipMalloc(&res.smax, sizeof(SmaxWays)) != hipSuccess) {
     res.smax = nullptr;
-    return fail(FVM_ENOMEM, "cannot allocate the autotuner's scratch word");
+    return fail(FVM_ENOMEM, "cannot allocate the autotuner's scratch maximum");
   }
-  unsigned long long* tune_smax = res.smax;
-  HIP_TRY(hipMemsetAsync(tune_smax, 0, sizeof(unsigned long long), ctx->stream));
+  SmaxWays* tune_smax = res.smax;
+  HIP_TRY(hipMemsetAsync(tune_smax, 0, sizeof(SmaxWays), ctx->stream));
   HIP_TRY(hipEventCreate(&res.e0));
   HIP_TRY(hipEventCreate(&res.e1));
   hipEvent_t e0 = res.e0, e1 = res.e1;
@@ -2613,11 +2622,13 @@ struct fvm_server {
   std::map<GroupKey, fvm_ctx*> groups;
   std::map<fvm_ctx*, GroupRes> gres;
   SolveSlot* d_slots = nullptr;
+  SmaxWays* d_smax = nullptr;     // [max_active]: each slot's wave-speed maximum
   std::vector<int> free_slots;
   std::vector<SlotHold> held;
   int* d_ids = nullptr;           // [2][max_active] slot ids of a batch, per group segment
   int* h_ids = nullptr;           // pinned staging of d_ids
   SolveSlot* h_init = nullptr;    // pinned [max_active]: initial slot contents
+  SmaxWays* h_zero_smax = nullptr;   // pinned zeros: clears an admitted solve's maximum
   hipStream_t cstream = nullptr;  // uploads, downloads, allocations of solve state
   hipStream_t sstream = nullptr;  // joins the groups' streams at the end of a batch
   hipEvent_t up_ev = nullptr;     // uploads of the latest admissions
@@ -2812,7 +2823,7 @@ int server_admit_one(fvm_server* s, Job* j) {
   init.t_end = r.t_end;
   init.cfl = r.cfl;
   b->dt_dev = &s->d_slots[slot].dt;
-  b->smax_dev = &s->d_slots[slot].smax;
+  b->smax_dev = s->d_smax + slot;
   b->halt_dev = &s->d_slots[slot].halt;
   g->desc_dirty = true;
   j->slot = slot;
@@ -2867,6 +2878,8 @@ int server_admit(fvm_server* s) {
     }
     HIP_TRY(hipMemcpyAsync(s->d_slots + j->slot, s->h_init + j->slot, sizeof(SolveSlot),
                            hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_smax + j->slot, s->h_zero_smax, sizeof(SmaxWays), hipMemcpyHostToDevice,
+                           g->stream));
     fvm_block* b = j->blk;
     if (b->nc == 5)
       hipLaunchKernelGGL(cfl_max_kernel<5>, dim3(256), dim3(256), 0, g->stream, b->q[0], b->g, g->P.gamma, b->smax_dev);
@@ -2901,7 +2914,7 @@ int server_enqueue(fvm_server* s, int nsteps) {
     const int nstage = g->rk == RK4 ? 4 : 3;
     for (int step = 0; step < nsteps; ++step) {
       hipLaunchKernelGGL(solve_dt_kernel, dim3((n + 63) / 64), dim3(64), 0, g->stream, s->d_slots,
-                         did + off, n);
+                         s->d_smax, did + off, n);
       HIP_TRY(hipGetLastError());
       for (int st = 1; st <= nstage; ++st) {
         StageArgs a{};
@@ -3035,9 +3048,11 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
   }
   const size_t ns = (size_t)max_active;
   if (hipMalloc(&s->d_slots, ns * sizeof(SolveSlot)) != hipSuccess ||
+      hipMalloc(&s->d_smax, ns * sizeof(SmaxWays)) != hipSuccess ||
       hipMalloc(&s->d_ids, 2 * ns * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&s->h_ids, 2 * ns * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&s->h_init, ns * sizeof(SolveSlot)) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_zero_smax, sizeof(SmaxWays)) != hipSuccess ||
       hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&s->sstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming) != hipSuccess ||
@@ -3047,6 +3062,8 @@ int fvm_server_create(int device, double hbm_budget_bytes, int max_active, fvm_s
     return fail(FVM_ENOMEM, "cannot allocate the server's slot tables, streams or events");
   }
   HIP_TRY(hipMemset(s->d_slots, 0, ns * sizeof(SolveSlot)));
+  HIP_TRY(hipMemset(s->d_smax, 0, ns * sizeof(SmaxWays)));
+  std::memset(s->h_zero_smax, 0, sizeof(SmaxWays));
   s->stager = std::thread(stager_main, s);
   *out = s;
   return FVM_OK;
@@ -3080,9 +3097,11 @@ int fvm_server_destroy(fvm_server* s) {
   for (auto& kv : s->pin_free) (void)hipHostFree(kv.second);
   for (hipEvent_t e : s->ev_free) (void)hipEventDestroy(e);
   (void)hipFree(s->d_slots);
+  (void)hipFree(s->d_smax);
   (void)hipFree(s->d_ids);
   if (s->h_ids) (void)hipHostFree(s->h_ids);
   if (s->h_init) (void)hipHostFree(s->h_init);
+  if (s->h_zero_smax) (void)hipHostFree(s->h_zero_smax);
   if (s->up_ev) (void)hipEventDestroy(s->up_ev);
   for (hipEvent_t e : s->bev) if (e) (void)hipEventDestroy(e);
   if (s->cstream) (void)hipStreamDestroy(s->cstream);

@@ -152,7 +152,7 @@ __device__ __forceinline__ bool stage_clock(const StageArgs& a, double& dt, bool
       if (lead) S->halt = 1;
       return false;
     }
-    const double sm = __longlong_as_double((long long)S->smax);
+    const double sm = smax_get(a.slot_smax);
     double d = S->cfl / sm;
     const bool bad = !(d > 0.0) || !(d < 1.0e300);
     const double rest = t_end - t;
@@ -166,7 +166,7 @@ __device__ __forceinline__ bool stage_clock(const StageArgs& a, double& dt, bool
   if (a.commit && lead) {
     S->t = S->t + S->dt;
     S->steps = S->steps + 1;
-    S->smax = 0ull;
+    smax_clear(a.slot_smax);
   }
   if (a.norm_if_end && want_norm) {
     const double t = S->t, t_end = S->t_end;
@@ -623,8 +623,7 @@ __global__ __launch_bounds__(64) void stage2d_kernel(StageArgs a) {
     }
     if (want_smax) {
       const double s = wg_max<NT>(smax, red);
-      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
-      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+      if (t == 0) smax_add(a.block_smax ? B.smax : a.smax, s);
     }
   }
 }
@@ -810,8 +809,7 @@ __global__ __launch_bounds__(TX * TY) void stage2d_tile_kernel(StageArgs a) {
     }
     if (want_smax) {
       const double s = wg_max<NT>(smax, red);
-      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
-      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+      if (t == 0) smax_add(a.block_smax ? B.smax : a.smax, s);
     }
     __syncthreads();   // the next tile's plane overwrites the fluxes
   }
@@ -1170,8 +1168,7 @@ void stage3d_kernel(StageArgs a) {
     }
     if (want_smax) {
       const double s = wg_max<NT>(smax, red);
-      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
-      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+      if (t == 0) smax_add(a.block_smax ? B.smax : a.smax, s);
     }
   }
 }
@@ -1445,8 +1442,7 @@ void stage3d_reg_kernel(StageArgs a) {
     }
     if (want_smax) {
       const double s = wg_max<NT>(smax, red);
-      unsigned long long* dst = a.block_smax ? B.smax : a.smax;
-      if (t == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(s));
+      if (t == 0) smax_add(a.block_smax ? B.smax : a.smax, s);
     }
     __syncthreads();   // the next tile's plane and norm sums
   }

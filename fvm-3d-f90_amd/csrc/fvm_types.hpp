@@ -57,6 +57,36 @@ struct PatchDesc {
   double fixed[5];                 // conserved state, device component order
 };
 
+// CFL wave-speed maximum of one solve (bits of a non-negative double, or of
+// a NaN: as unsigned integers those order like the values and NaN wins),
+// spread over kSmaxWays cache lines.  Each workgroup of a launch adds its
+// maximum to line (workgroup id % kSmaxWays) with one device-scope atomic:
+// atomics on one line are serialised (~10 ns each on MI355X, so one line
+// taking a 1250-workgroup launch's atomics adds ~13 us to the launch);
+// readers take the maximum over the lines.
+constexpr int kSmaxWays = 32;
+struct alignas(128) SmaxLine {
+  unsigned long long v;
+  unsigned long long pad[15];
+};
+struct SmaxWays {
+  SmaxLine line[kSmaxWays];
+};
+
+__device__ __forceinline__ void smax_add(SmaxWays* w, double s) {
+  atomicMax(&w->line[blockIdx.x % kSmaxWays].v, (unsigned long long)__double_as_longlong(s));
+}
+__device__ __forceinline__ double smax_get(const SmaxWays* w) {
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (int i = 0; i < kSmaxWays; ++i) m = w->line[i].v > m ? w->line[i].v : m;
+  return __longlong_as_double((long long)m);
+}
+__device__ __forceinline__ void smax_clear(SmaxWays* w) {
+#pragma unroll
+  for (int i = 0; i < kSmaxWays; ++i) w->line[i].v = 0ull;
+}
+
 // One block as the stage kernels see it.
 struct BlockDesc {
   double* q[4];          // state arrays: 0 = u^n, 1, 2 = stage buffers, 3 = RK4 accumulator
@@ -67,7 +97,7 @@ struct BlockDesc {
   int tile_base;         // first tile of this block in the launch's tile list
   const double* dt_dev;            // the block's dt for this step (device memory: the
                                    // context's time slot, or the solve's own in serving)
-  unsigned long long* smax;        // per-block CFL max accumulator, or null
+  SmaxWays* smax;                  // per-block CFL max accumulator, or null
   const int* halt;                 // per-block: if set, the block's tiles are skipped
   const double* rv;                // viscous residual of the stage input (NC-1 components:
                                    // momentum, energy; stride g.cs), or null: inviscid
@@ -107,10 +137,10 @@ constexpr double kEndTol = 1.0e-14;
 
 // Time-stepping state of one solve, kept on the device so a run of steps
 // needs no host round trip: dt = min(cfl / smax, t_end - t), t += dt, and
-// halt once t reaches t_end.
+// halt once t reaches t_end.  The wave-speed maximum smax lives beside it in
+// an SmaxWays.
 struct SolveSlot {
   double t, t_end, cfl, dt;
-  unsigned long long smax;        // CFL wave-speed max (bits of a non-negative double)
   int steps;
   int nan;                        // dt was NaN, zero or infinite
   int halt;                       // t reached t_end: later steps are skipped
@@ -127,7 +157,7 @@ struct StageArgs {
   Params P;
   int stage;
   double* norm_part;              // per-tile sums of rhs^2 (NC each), or null
-  unsigned long long* smax;       // CFL wave-speed max of the output, or null
+  SmaxWays* smax;                 // CFL wave-speed max of the output, or null
   int block_smax;                 // 1: accumulate into each block's own smax instead
   const int* halt;                // if non-null and set, the launch does nothing (run ended)
   // a plain context's time slot in CFL mode (null otherwise): the first stage
@@ -135,6 +165,7 @@ struct StageArgs {
   // step count and clears the CFL maximum (commit); norm_if_end: the last
   // stage leaves norm partials only if the step reached t_end
   SolveSlot* slot;
+  SmaxWays* slot_smax;            // the slot's wave-speed max (with slot)
   int cfl_dt, commit, norm_if_end;
 };
 
