@@ -59,8 +59,9 @@ HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (published)
 B_ALG = {"rk4_3d": 160.0, "ssp3_3d": 8.0 / 3.0 * 40.0, "ssp3_2d": 8.0 / 3.0 * 32.0}
 METRIC = "FP64 cell-updates/s per RK stage"
-# stage launches of every TIMING_EVERY-th step are bracketed by HIP events
-# (a few microseconds each: timing every launch would slow short stages)
+# the stage launches of every TIMING_EVERY-th step are bracketed by one HIP
+# event pair (an event costs the stream a few microseconds: timing every
+# launch would slow short stages)
 TIMING_EVERY = 4
 
 

@@ -1081,10 +1081,8 @@ StepMem ctx_mem(fvm_ctx* ctx) { return StepMem{ctx->d_desc, ctx->d_norm, ctx->d_
 // Stage st (1-based) of one time step: its buffers, combine, norm and CFL
 // roles, then the launch over every block and the patch refill.
 // `norm_mode`: 0 no residual norms, 1 the last stage leaves its norm
-// partials, 2 only if the step reached t_end.  With timing on, each stage
-// launch is bracketed by events (ev index *nev).
-int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t* nev,
-                     const StepMem& mem) {
+// partials, 2 only if the step reached t_end.
+int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, const StepMem& mem) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   {
     const bool last = st == nstage;
@@ -1115,10 +1113,7 @@ int launch_one_stage(fvm_ctx* ctx, int st, bool cfl_mode, int norm_mode, size_t*
     a.slot_smax = cfl_mode ? mem.smax : nullptr;
     a.cfl_dt = cfl_mode && st == 1 ? 1 : 0;
     a.commit = cfl_mode && st == 2 ? 1 : 0;
-    const bool ev = nev != nullptr;
-    if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_stage_all(ctx, a, mem.desc)) return rc;
-    if (ev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
     if (int rc = launch_patches(ctx, a.out_buf)) return rc;
   }
   return FVM_OK;
@@ -1168,10 +1163,15 @@ int combine_norms(fvm_ctx* ctx, int nc, const double* block_sums) {
 
 // One time step of every block: the RK stages (in CFL mode the first one
 // computes dt from the device time slot and the second commits the step).
+// With timing on (nev non-null), one event pair brackets the step's stage
+// launches (ev index *nev): events between launches would stall the stream
+// for microseconds each, a large share of a short stage.
 int launch_step(fvm_ctx* ctx, bool cfl_mode, int norm_mode, size_t* nev, const StepMem& mem) {
   const int nstage = ctx->rk == RK4 ? 4 : 3;
+  if (nev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
   for (int st = 1; st <= nstage; ++st)
-    if (int rc = launch_one_stage(ctx, st, cfl_mode, norm_mode, nev, mem)) return rc;
+    if (int rc = launch_one_stage(ctx, st, cfl_mode, norm_mode, mem)) return rc;
+  if (nev) HIP_TRY(hipEventRecord(ctx->ev[(*nev)++], ctx->stream));
   return FVM_OK;
 }
 
@@ -1769,9 +1769,9 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
 
   const int nstage = ctx->rk == RK4 ? 4 : 3;
   size_t nev = 0;
-  std::vector<int> ev_step;   // step of each timed launch pair
+  std::vector<int> ev_step;   // step of each timed event pair
   if (ctx->timing_every) {
-    const size_t need_ev = 2 * (size_t)((nsteps + ctx->timing_every - 1) / ctx->timing_every + 1) * nstage;
+    const size_t need_ev = 2 * (size_t)((nsteps + ctx->timing_every - 1) / ctx->timing_every + 1);
     while (ctx->ev.size() < need_ev) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -1825,8 +1825,8 @@ int fvm_step(fvm_ctx* ctx, int nsteps, double dt, double cfl, double* dt_last) {
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[e], ctx->ev[e + 1]));
       ctx->stage_ms += ms;
-      ctx->stage_launches += 1;
-      ctx->cell_updates += cells;
+      ctx->stage_launches += nstage;
+      ctx->cell_updates += cells * nstage;
     }
     ctx->timing_step += nsteps;
   }
@@ -1880,7 +1880,7 @@ int fvm_stage(fvm_ctx* ctx, int s, double dt) {
   } else if (ctx->desc_dirty) {
     return fail(FVM_ESTATE, "blocks or launch shapes changed within a step");
   }
-  if (int rc = launch_one_stage(ctx, s, false, s == nstage ? 1 : 0, nullptr, ctx_mem(ctx))) return rc;
+  if (int rc = launch_one_stage(ctx, s, false, s == nstage ? 1 : 0, ctx_mem(ctx))) return rc;
   if (s < nstage) {
     ctx->stage_next = s + 1;
     HIP_TRY(hipGetLastError());

@@ -229,8 +229,11 @@ __device__ __forceinline__ void muscl_vertex(double p1, double p2, double p3, do
     R = p2 - 0.5 * Db_m;
     return;
   }
-  L = p2 + 0.25 * ((1.0 - kappa) * Db_m + (1.0 + kappa) * Db_p);
-  R = p2 - 0.25 * ((1.0 - kappa) * Db_p + (1.0 + kappa) * Db_m);
+  // p2 +/- 0.25 s as one FMA: the product by 0.25 is exact (no slope is
+  // below 2^-1020 in magnitude unless it is zero), so the single rounding of
+  // the FMA is the rounding of the sum, as in the reference
+  L = fma(0.25, (1.0 - kappa) * Db_m + (1.0 + kappa) * Db_p, p2);
+  R = fma(-0.25, (1.0 - kappa) * Db_p + (1.0 + kappa) * Db_m, p2);
 }
 
 // muscl_vertex over the NC components of one vertex (V[0..2]: the vertex and
@@ -279,12 +282,14 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
   const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
   const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
   const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
-  const double t0 = u0 - 2.0 * u1 + u2;
-  const double s0 = u0 - 4.0 * u1 + 3.0 * u2;
-  const double t1 = u1 - 2.0 * u2 + u3;
+  // a - 2b and a - 4b as FMAs: products by 2 and 4 are exact, so each FMA
+  // rounds once, as the separate subtraction does
+  const double t0 = fma(-2.0, u1, u0) + u2;
+  const double s0 = fma(-4.0, u1, u0) + 3.0 * u2;
+  const double t1 = fma(-2.0, u2, u1) + u3;
   const double s1 = textbook ? (u1 - u3) : (u1 + u3);
-  const double t2 = u2 - 2.0 * u3 + u4;
-  const double s2 = 3.0 * u2 - 4.0 * u3 + u4;
+  const double t2 = fma(-2.0, u3, u2) + u4;
+  const double s2 = fma(-4.0, u3, 3.0 * u2) + u4;
   const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
   const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
   const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
@@ -381,8 +386,9 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
     H_l = P.g_gm1 * p_l * (rho_r * rs2) + 0.5 * ke_l;
     H_r = P.g_gm1 * p_r * (rho_l * rs2) + 0.5 * ke_r;
   } else {
-    H_l = ediv(P.g_gm1 * p_l, rho_l) + 0.5 * ke_l;
-    H_r = ediv(P.g_gm1 * p_r, rho_r) + 0.5 * ke_r;
+    // x + 0.5 y as one FMA (exact product, one rounding)
+    H_l = fma(0.5, ke_l, ediv(P.g_gm1 * p_l, rho_l));
+    H_r = fma(0.5, ke_r, ediv(P.g_gm1 * p_r, rho_r));
     rho_b = esqrt(rho_l * rho_r);
   }
   const double drho = rho_r - rho_l, du = u_r - u_l, dv = v_r - v_l, dw = w_r - w_l;
@@ -408,7 +414,7 @@ __device__ __forceinline__ void flux_roe(const double L[NC], const double R[NC],
   double q2_b;
   if constexpr (NC == 5) q2_b = u_b * u_b + v_b * v_b + w_b * w_b;
   else q2_b = u_b * u_b + v_b * v_b;
-  const double c2_b = (gamma - 1.0) * (H_b - 0.5 * q2_b);
+  const double c2_b = (gamma - 1.0) * fma(-0.5, q2_b, H_b);   // H_b - 0.5 q2_b, one rounding
   double c_b, rc_b = 0.0;
   if constexpr (kFast) fsqrt_rsqrt(c2_b, c_b, rc_b);
   else c_b = esqrt(c2_b);
@@ -528,7 +534,7 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
   const double fp0 = rho_l * c_l * M_l;
   const double fm0 = rho_r * c_r * M_r;
   F[0] = fp0 + fm0;
-  const double gp = -Vc_l + 2.0 * c_l, gm = -Vc_r - 2.0 * c_r;
+  const double gp = fma(2.0, c_l, -Vc_l), gm = fma(-2.0, c_r, -Vc_r);   // exact products
   const double rg = kFast ? frcp(gamma) : 0.0;
   const Recip rgam = kFast ? Recip{gamma, 0.0} : erecip(gamma);
   // (0.0 / gamma is the reference's +0 for the tangential components)
@@ -540,8 +546,8 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
     else { a = fp0 * (0.0 + vl[m]); b = fm0 * (0.0 + vr[m]); }
     F[1 + m] = a + b;
   }
-  const double tl = (gamma - 1.0) * Vc_l + 2.0 * c_l;
-  const double tr = (gamma - 1.0) * Vc_r - 2.0 * c_r;
+  const double tl = fma(2.0, c_l, (gamma - 1.0) * Vc_l);
+  const double tr = fma(-2.0, c_r, (gamma - 1.0) * Vc_r);
   const double den = 2.0 * (gamma * gamma - 1.0);
   if constexpr (kFast) {
     const double rden = frcp(den);
@@ -550,8 +556,9 @@ __device__ __forceinline__ void flux_vanleer(const double L[NC], const double R[
     return;
   }
   const Recip rden = erecip(den);
-  F[NC - 1] = fp0 * (ediv(tl * tl, rden) + (q2_l - Vc_l * Vc_l) / 2.0)
-            + fm0 * (ediv(tr * tr, rden) + (q2_r - Vc_r * Vc_r) / 2.0);
+  // x + y / 2 = fma(0.5, y, x): halving is exact
+  F[NC - 1] = fp0 * fma(0.5, q2_l - Vc_l * Vc_l, ediv(tl * tl, rden))
+            + fm0 * fma(0.5, q2_r - Vc_r * Vc_r, ediv(tr * tr, rden));
 }
 
 // ---- "AUSM" as the reference implements it, src/scheme/scheme_AUSM.f90 -------
@@ -572,8 +579,8 @@ __device__ __forceinline__ void flux_ausm(const double L[NC], const double R[NC]
   const double ir_l = kFast ? frcp(rho_l) : 0.0, ir_r = kFast ? frcp(rho_r) : 0.0;
   const Recip er_l = kFast ? Recip{rho_l, 0.0} : erecip(rho_l);
   const Recip er_r = kFast ? Recip{rho_r, 0.0} : erecip(rho_r);
-  const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : ediv(P.g_gm1 * p_l, er_l) + 0.5 * ke_l;
-  const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : ediv(P.g_gm1 * p_r, er_r) + 0.5 * ke_r;
+  const double H_l = kFast ? P.g_gm1 * p_l * ir_l + 0.5 * ke_l : fma(0.5, ke_l, ediv(P.g_gm1 * p_l, er_l));
+  const double H_r = kFast ? P.g_gm1 * p_r * ir_r + 0.5 * ke_r : fma(0.5, ke_r, ediv(P.g_gm1 * p_r, er_r));
   const double c_l = kFast ? fsqrt(gamma * p_l * ir_l) : esqrt(ediv(gamma * p_l, er_l));
   const double c_r = kFast ? fsqrt(gamma * p_r * ir_r) : esqrt(ediv(gamma * p_r, er_r));
   const double M_l = kFast ? Vc_l * frcp(c_l) : ediv(Vc_l, c_l);
@@ -644,8 +651,9 @@ __device__ __forceinline__ void flux_ausmplus(const double L[NC], const double R
     M_l = Vc_l * rc;
     M_r = Vc_r * rc;
   } else {
-    H_l = ediv(P.g_gm1 * p_l, rho_l) + 0.5 * ke_l;
-    H_r = ediv(P.g_gm1 * p_r, rho_r) + 0.5 * ke_r;
+    // x + 0.5 y as one FMA (exact product, one rounding)
+    H_l = fma(0.5, ke_l, ediv(P.g_gm1 * p_l, rho_l));
+    H_r = fma(0.5, ke_r, ediv(P.g_gm1 * p_r, rho_r));
     const double k2 = ediv(2.0 * (gamma - 1.0), gamma + 1.0);
     const double cs2_l = k2 * H_l;
     const double cs2_r = k2 * H_r;

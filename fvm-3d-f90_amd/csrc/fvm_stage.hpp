@@ -152,7 +152,7 @@ __device__ __forceinline__ bool stage_clock(const StageArgs& a, double& dt, bool
       if (lead) S->halt = 1;
       return false;
     }
-    const double sm = smax_get(a.slot_smax);
+    const double sm = smax_get_wave(a.slot_smax);   // every thread of the workgroup is here
     double d = S->cfl / sm;
     const bool bad = !(d > 0.0) || !(d < 1.0e300);
     const double rest = t_end - t;
@@ -865,6 +865,7 @@ void stage3d_kernel(StageArgs a) {
   __shared__ double LRy[NLR][NC * LRY];
   __shared__ double red[NT / 64];
   __shared__ double NrmW[(NT / 64) * NC];   // per-wave rhs^2 sums (norm stage only)
+  __shared__ double RcpD[3];       // exact family: reciprocals of the tile's dx, dy, dz
   double* Fx = Vs;                 // [NC][FXN] after phase B
   double* Fy = Vs + NC * FXN;      // [NC][FYN]
 
@@ -965,6 +966,9 @@ void stage3d_kernel(StageArgs a) {
     }
 
     if (want_norm && t < (NT / 64) * NC) NrmW[t] = 0.0;
+    // the divisors' reciprocals, formed once per tile instead of per plane
+    // (read in D, after this plane's first barrier)
+    if (!kFast && t < 3) RcpD[t] = erecip(t == 0 ? g.dx : (t == 1 ? g.dy : g.dz)).r;
 
     for (int k = k0; k < k1; ++k) {
       // ---- A: queue shift (centre k+1), plane k into LDS, next loads
@@ -1116,8 +1120,9 @@ void stage3d_kernel(StageArgs a) {
       for (int m = 0; m < NC; ++m) rr[m] = 0.0;
       if (owner) {
         const uint32_t cidx = gidx32(g, c, r, k);
-        const Recip ix = kFast ? Recip{} : erecip(g.dx), iy = kFast ? Recip{} : erecip(g.dy);
-        const Recip iz = kFast ? Recip{} : erecip(g.dz);
+        const Recip ix = kFast ? Recip{} : Recip{g.dx, RcpD[0]};
+        const Recip iy = kFast ? Recip{} : Recip{g.dy, RcpD[1]};
+        const Recip iz = kFast ? Recip{} : Recip{g.dz, RcpD[2]};
         double out[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {

@@ -82,6 +82,20 @@ __device__ __forceinline__ double smax_get(const SmaxWays* w) {
   for (int i = 0; i < kSmaxWays; ++i) m = w->line[i].v > m ? w->line[i].v : m;
   return __longlong_as_double((long long)m);
 }
+// The same read by a whole wave (all 64 lanes active): lane l < kSmaxWays
+// loads line l, so the lines are fetched by one vector load instead of
+// kSmaxWays scalar ones, and a butterfly gives every lane the maximum.
+__device__ __forceinline__ double smax_get_wave(const SmaxWays* w) {
+  static_assert(kSmaxWays <= 64 && (kSmaxWays & (kSmaxWays - 1)) == 0, "one line per lane");
+  const int l = threadIdx.x & 63;
+  unsigned long long m = l < kSmaxWays ? w->line[l].v : 0ull;
+#pragma unroll
+  for (int off = kSmaxWays / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  return __longlong_as_double((long long)__shfl(m, 0, 64));
+}
 __device__ __forceinline__ void smax_clear(SmaxWays* w) {
 #pragma unroll
   for (int i = 0; i < kSmaxWays; ++i) w->line[i].v = 0ull;

@@ -346,11 +346,14 @@ int fvm_block_launch_tile(fvm_block *blk, int *tile_id);
 /* Stage-kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only wall-clocks the whole solve,
  * src/solve/class_output.f90:266-334).  enable = n >= 1 brackets the stage
- * launches of every n-th step (events cost a few microseconds per launch,
- * so a sample keeps them out of short stages' wall time); 0 turns it off.
- * Enabling resets the counters; fvm_timing_read returns the summed
- * stage-kernel milliseconds of the timed launches, their number and the cell
- * updates they performed (launches of steps skipped past t_end excluded). */
+ * launches of every n-th step with one event pair (an event costs the stream
+ * a few microseconds, so none sits between the stages and a sample keeps
+ * them out of short stages' wall time); 0 turns it off.  Enabling resets the
+ * counters; fvm_timing_read returns the summed milliseconds of the timed
+ * steps' launches (stage kernels, with the viscous-residual and patch
+ * launches between them where those are on), the number of stage launches
+ * they hold and the cell updates they performed (steps skipped past t_end
+ * excluded). */
 int fvm_timing_enable(fvm_ctx *ctx, int enable);
 int fvm_timing_read(fvm_ctx *ctx, double *stage_ms, int64_t *launches, double *cell_updates);
 
