@@ -5,8 +5,9 @@ Default workload: 3-D Euler isentropic vortex, 128^3 vertices on a periodic
 with the van Albada limiter + Roe flux, classical RK4, CFL 0.8, FP64.
 One bench step = one full RK4 time step (4 fused stage launches + the CFL
 dt update) over the whole grid; the state is resident in HBM throughout.
-Setup before the W warm-up steps: the launch-shape autotuner and 0.5 s of
-untimed steps while the GPU's clocks settle.  Numerics: the exact family
+Setup before the W warm-up steps: 0.5 s of untimed steps while the GPU's
+clocks settle, then the launch-shape autotuner (on settled clocks, so the
+shapes it ranks are timed alike).  Numerics: the exact family
 (bit for bit the oracle), the one that meets the 1000-step tolerance here.
 
 Metric: FP64 cell-updates per second per RK stage
@@ -201,12 +202,12 @@ def bench_vortex(args):
     blk.set_periodic()
     blk.upload(q)
     ctx.numerics(numerics_code(fvmgpu, args))
+    settle(ctx, cfl)
     if not args.no_tune:
         # shapes within a few per cent: long enough windows to rank them
         ctx.autotune(reps=getattr(args, "tune_reps", 10))
     tile = blk.launch_shape()
     reg = blk.launch_tile() >= 4          # the register-reconstruction kernel
-    settle(ctx, cfl)
     if args.warmup:
         ctx.step(args.warmup, cfl=cfl)
     ctx.sync()
