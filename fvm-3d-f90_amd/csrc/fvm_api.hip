@@ -1019,9 +1019,9 @@ int grid_for(fvm_ctx* ctx, int ntiles) {
   return std::min(ntiles, 2 * ctx->num_cu);
 }
 
-// One RK stage over every block: a single launch over the whole tile list
-// (batched or persistent), or one launch per block.
-int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
+// The viscous residual of every block's stage input (one launch per block);
+// with connected faces the kernels read across them through the table.
+int launch_viscous_all(fvm_ctx* ctx, int in_buf, const BlockDesc* tab) {
   ViscArgs v{};
   v.V.mu_ref = ctx->visc_mu_ref;
   v.V.t_ref = ctx->visc_t_ref;
@@ -1032,7 +1032,11 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
   v.V.rr_gas = 1.0 / ctx->visc_r;
   v.V.rt_ref = 1.0 / ctx->visc_t_ref;
   v.V.sfac = ctx->visc_t_ref + ctx->visc_s;
-  for (fvm_block* b : ctx->blocks) {
+  v.tab = ctx->has_conn ? tab : nullptr;
+  v.in_buf = in_buf;
+  for (size_t bi = 0; bi < ctx->blocks.size(); ++bi) {
+    fvm_block* b = ctx->blocks[bi];
+    v.self = (int)bi;
     v.g = b->g;
     v.q = b->q[in_buf];
     v.rv = b->rv;
@@ -1048,10 +1052,12 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf) {
   return FVM_OK;
 }
 
+// One RK stage over every block: a single launch over the whole tile list
+// (batched or persistent), or one launch per block.
 int launch_stage_all(fvm_ctx* ctx, StageArgs a, const BlockDesc* tab = nullptr) {
-  if (ctx->visc_mu_ref > 0.0)
-    if (int rc = launch_viscous_all(ctx, a.in_buf)) return rc;
   a.blocks = tab ? tab : ctx->d_desc;
+  if (ctx->visc_mu_ref > 0.0)
+    if (int rc = launch_viscous_all(ctx, a.in_buf, a.blocks)) return rc;
   a.nblocks = (int)ctx->blocks.size();
   if (ctx->launch_mode != FVM_LAUNCH_PER_BLOCK && batchable(ctx)) {
     a.tile0 = 0;
@@ -1523,8 +1529,6 @@ int fvm_block_set_bc_state(fvm_block* b, int face, int kind, const double prim[5
 int fvm_block_connect(fvm_block* a, int face, fvm_block* b, int shared) {
   if (!a || !b) return fail(FVM_EINVAL, "null block");
   if (a->ctx != b->ctx) return fail(FVM_EINVAL, "blocks belong to different contexts");
-  if (a->ctx->visc_mu_ref > 0.0)
-    return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
   if (face < 0 || face > 5) return fail(FVM_EINVAL, "face %d out of range 0..5", face);
   if (a->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
   const int d = face / 2, bface = face ^ 1;
@@ -1579,7 +1583,7 @@ int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t rang
   if (recv->dim != donor->dim) return fail(FVM_EINVAL, "a 2-D and a 3-D block cannot be connected");
   fvm_ctx* ctx = recv->ctx;
   if (ctx->visc_mu_ref > 0.0)
-    return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
+    return fail(FVM_EINVAL, "viscous terms on blocks with connection patches are not supported");
   const int dim = recv->dim;
   const int nr[3] = {recv->g.nx, recv->g.ny, recv->g.nz};
   const int nd[3] = {donor->g.nx, donor->g.ny, donor->g.nz};
@@ -2009,8 +2013,9 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
     return fail(FVM_EINVAL, "viscous parameters need t_ref, prandtl, r_gas > 0 and s >= 0");
   for (fvm_block* b : ctx->blocks)
     for (int f = 0; f < 6; ++f)
-      if (b->g.bc[f] >= BC_CONN)
-        return fail(FVM_EINVAL, "viscous terms on connected blocks are not supported");
+      if (b->g.bc[f] == BC_PATCH)
+        return fail(FVM_EINVAL, "viscous terms on blocks with connection patches or wall, symmetry, "
+                    "inflow or farfield faces are not supported");
   if (int rc = set_device(ctx)) return rc;
   for (fvm_block* b : ctx->blocks)
     if (int rc = alloc_rv(b)) return rc;

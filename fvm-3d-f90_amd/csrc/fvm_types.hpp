@@ -132,6 +132,11 @@ struct ViscArgs {
   double* rv;                      // out: viscous residual, NC-1 components
   int tiles_x, tiles_y, chunk;     // launch decomposition (3-D: k-chunks of `chunk` planes)
   ViscParams V;
+  // connected blocks: the context's table, this block's index in it and the
+  // stage input's state array index; stencil points beyond a connected face
+  // are read from the neighbour (null table: a single block)
+  const BlockDesc* tab;
+  int self, in_buf;
 };
 
 // RK stage kinds: the combine each launch applies

@@ -7,7 +7,10 @@
 //   - per vertex: u, v, w, T = p / (rho R), mu = Sutherland(T);
 //   - a stencil point outside the block maps into it per direction
 //     (periodic: wrap by the nodal period n; otherwise clamp to the boundary
-//     vertex), so no edge or corner ghost is read;
+//     vertex), so no edge or corner ghost is read; across a face connected
+//     to another block the point is that block's vertex (directions taken in
+//     the order i, j, k, each in the block the previous one reached), so a
+//     cut grid sees the values of the grid it was cut from;
 //   - face between a and b = a + e_d: means of the two vertices, normal
 //     derivative (phi_b - phi_a) / dr_d, tangential derivatives the mean of
 //     the central differences at a and b, /(4 dr_t);
@@ -62,18 +65,63 @@ __device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
   return x;
 }
 
-// u, v, w, T, mu of the conserved state at vertex (i, j, k) (already mapped)
+// A stencil point resolved to the vertex it reads: state array, element
+// offset of the vertex, component stride.
+struct VPoint {
+  const double* q;
+  int64_t c, cs;
+};
+
+__device__ __forceinline__ int64_t vidx(const BlockGeom& g, int i, int j, int k) {
+  return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
+}
+
+// The vertex stencil point (x, y, z) of the launch's block reads.
+template <int DIM>
+__device__ __forceinline__ VPoint resolve(const ViscArgs& a, int x, int y, int z) {
+  const BlockGeom& g = a.g;
+  if (!a.tab) {
+    const int mi = vmap(x, g.nx, g.bc[0], g.bc[1]), mj = vmap(y, g.ny, g.bc[2], g.bc[3]);
+    const int mk = DIM == 3 ? vmap(z, g.nz, g.bc[4], g.bc[5]) : 0;
+    return VPoint{a.q, vidx(g, mi, mj, mk), g.cs};
+  }
+  int b = a.self;
+  int c[3] = {x, y, DIM == 3 ? z : 0};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const BlockDesc& B = a.tab[b];
+    const int n = d == 0 ? B.g.nx : (d == 1 ? B.g.ny : B.g.nz);
+    if (c[d] >= 0 && c[d] < n) continue;
+    const int f = 2 * d + (c[d] < 0 ? 0 : 1);
+    if (B.g.bc[f] == BC_CONN) {
+      // shared interface vertex: vertex n-1 of the low block is vertex 0 of
+      // the high one; abutting blocks: one spacing apart
+      const int nb = B.nbr[f], sh = B.nbr_shared[f];
+      const BlockGeom& gn = a.tab[nb].g;
+      const int nn = d == 0 ? gn.nx : (d == 1 ? gn.ny : gn.nz);
+      c[d] = c[d] < 0 ? c[d] + nn - sh : c[d] - n + sh;
+      b = nb;
+    } else {
+      c[d] = vmap(c[d], n, B.g.bc[2 * d], B.g.bc[2 * d + 1]);
+    }
+  }
+  const BlockDesc& B = a.tab[b];
+  return VPoint{B.q[a.in_buf], vidx(B.g, c[0], c[1], c[2]), B.g.cs};
+}
+
+// u, v, w, T, mu of the conserved state at a resolved vertex
 template <int NC>
-__device__ __forceinline__ void prims(const double* __restrict__ q, const BlockGeom& g, int i, int j,
-                                      int k, const ViscParams& V, const Rec& R, double out[5]) {
-  const int64_t c = (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
+__device__ __forceinline__ void prims(const VPoint& pt, const ViscParams& V, const Rec& R,
+                                      double out[5]) {
+  const double* __restrict__ q = pt.q;
+  const int64_t c = pt.c, cs = pt.cs;
   const double rho = q[c];
   if constexpr (kFastV) {
     // one reciprocal of rho; reciprocals of R and T_ref from the host
     const double rr = frcp(rho);
-    const double u = q[c + g.cs] * rr, v = q[c + 2 * g.cs] * rr;
-    const double w = NC == 5 ? q[c + 3 * g.cs] * rr : 0.0;
-    const double e = q[c + (NC - 1) * g.cs];
+    const double u = q[c + cs] * rr, v = q[c + 2 * cs] * rr;
+    const double w = NC == 5 ? q[c + 3 * cs] * rr : 0.0;
+    const double e = q[c + (NC - 1) * cs];
     const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
     const double t = p * (rr * V.rr_gas);
     const double tr = t * V.rt_ref;
@@ -85,9 +133,9 @@ __device__ __forceinline__ void prims(const double* __restrict__ q, const BlockG
     return;
   }
   const Recip ir = erecip(rho);
-  const double u = ediv(q[c + g.cs], ir), v = ediv(q[c + 2 * g.cs], ir);
-  const double w = NC == 5 ? ediv(q[c + 3 * g.cs], ir) : 0.0;
-  const double e = q[c + (NC - 1) * g.cs];
+  const double u = ediv(q[c + cs], ir), v = ediv(q[c + 2 * cs], ir);
+  const double w = NC == 5 ? ediv(q[c + 3 * cs], ir) : 0.0;
+  const double e = q[c + (NC - 1) * cs];
   // class_flow_field.f90:383-396 operation order
   const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
   const double t = ediv(p, rho * V.r_gas);
@@ -188,12 +236,9 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
   const visc::Rec R = visc::make_rec(g, V);
 
   auto load_plane = [&](int k, int slot) {
-    const int mk = visc::vmap(k, g.nz, g.bc[4], g.bc[5]);
     for (int e = t; e < PN; e += NT) {
-      const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
-      const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
       double v[5];
-      visc::prims<5>(a.q, g, mi, mj, mk, V, R, v);
+      visc::prims<5>(visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k), V, R, v);
 #pragma unroll
       for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
     }
@@ -255,10 +300,8 @@ __global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
   const int i = i0 + tx, j = j0 + ty;
   const visc::Rec R = visc::make_rec(g, V);
   for (int e = t; e < PN; e += NT) {
-    const int mi = visc::vmap(i0 - 1 + e % W, g.nx, g.bc[0], g.bc[1]);
-    const int mj = visc::vmap(j0 - 1 + e / W, g.ny, g.bc[2], g.bc[3]);
     double v[5];
-    visc::prims<4>(a.q, g, mi, mj, 0, V, R, v);
+    visc::prims<4>(visc::resolve<2>(a, i0 - 1 + e % W, j0 - 1 + e / W, 0), V, R, v);
 #pragma unroll
     for (int m = 0; m < 5; ++m) S[m][e] = v[m];
   }

@@ -302,7 +302,8 @@ class Context:
         Returns {(a, b, c): (block, (i0, j0, k0))}."""
         nk, nj, ni = q.shape[1:]
         ncut = tuple(ncut) + (1,) * (3 - len(ncut))
-        ranges = [cut_uniform(n, c) for n, c in zip((ni, nj, nk), ncut)]
+        # (a 2-D grid's single k plane is not cut)
+        ranges = [cut_uniform(n, c) if n > 1 else [(0, 0)] for n, c in zip((ni, nj, nk), ncut)]
         blocks = {}
         for a, (x0, x1) in enumerate(ranges[0]):
             for b, (y0, y1) in enumerate(ranges[1]):

@@ -299,7 +299,11 @@ int fvm_step_noalloc(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt
  * is an empty stub and its call at src/solve/class_time_discret.f90:182 is
  * commented out).  mu = mu_ref (T/t_ref)^(3/2) (t_ref + s)/(T + s),
  * T = p/(rho r_gas), heat conductivity mu cp/prandtl, cp = gamma r_gas/(gamma-1).
- * mu_ref <= 0 switches the terms off.  Not available on connected blocks. */
+ * mu_ref <= 0 switches the terms off.  On blocks joined by fvm_block_connect
+ * a stencil point beyond a connected face is the neighbour's vertex (edge and
+ * corner points through the neighbour's neighbour), so a cut grid gives the
+ * results of the grid it was cut from.  Not with connection patches
+ * (fvm_conn_add) or wall, symmetry, inflow or farfield faces. */
 int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland_s,
                     double prandtl, double r_gas);
 

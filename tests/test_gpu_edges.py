@@ -73,8 +73,9 @@ def test_invalid_requests_are_errors():
             ctx.step(1, dt=0.0, cfl=0.0)                  # neither dt nor CFL
         c = ctx.block((8, 8, 8), (0.1, 0.1, 0.1))
         a.connect(1, c)
+        ctx.viscous(0.01)                                  # viscous on connected blocks: fine
         with pytest.raises(fvmgpu.FvmError):
-            ctx.viscous(0.01)                              # viscous on connected blocks
+            c.conn_add((8, 1, 1), (8, 8, 8), a, (1, 1, 1), (1, 8, 8), (1, 2, 3))   # a patch: not
     with fvmgpu.Context() as ctx:
         ctx.scheme(recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE, rk=O.RK_SSP3)
         b2 = ctx.block((8, 8, 1), (0.1, 0.1, 1.0))
