@@ -137,6 +137,8 @@ struct fvm_ctx {
                                          // stages) are refilled before a download with ghosts
   int total_tiles = 0;
   bool has_conn = false;                 // some block face is connected to another block
+  bool has_wall = false;                 // some block face is a wall (slip or no-slip)
+  bool has_noslip = false;               // some block face is a no-slip wall
   // central viscous terms (fvm_viscous_set); mu_ref <= 0: inviscid
   double visc_mu_ref = 0.0, visc_t_ref = 1.0, visc_s = 0.0, visc_pr = 0.72, visc_r = 1.0;
   SolveSlot* d_slot = nullptr;            // device time-stepping state (CFL mode)
@@ -279,7 +281,7 @@ __global__ void patch_copy_kernel(const BlockDesc* __restrict__ tab, const Patch
       double sg = 1.0;
       if (c >= 1 && c <= P.nc - 2) {
         sc = 1 + P.map_dir[c - 1];
-        sg = P.map_sign[c - 1];
+        sg = P.neg_all ? -1.0 : P.map_sign[c - 1];
       }
       dst[ri + c * gr.cs] = sg * src[di + sc * gd.cs];
     }
@@ -786,14 +788,24 @@ void unpack_state(const fvm_block* b, const double* stage, double* q, int host_n
   }
 }
 
-// Upload the descriptor table (blocks in context order, tiles numbered
-// block after block).
+// The wall kind of a boundary kind whose face mirrors the block onto itself.
+int wall_kind(int bc) {
+  switch (bc) {
+    case FVM_BC_WALL: case FVM_BC_WALL_INVISCID: case FVM_BC_SYMMETRY: return WALL_SLIP;
+    case FVM_BC_WALL_VISCOUS: return WALL_NOSLIP;
+  }
+  return WALL_NONE;
+}
+
 // Descriptor entries of the context's blocks into h[0 .. blocks), tile bases
-// and the tile total.
+// and the tile total (blocks in context order, tiles numbered block after
+// block).
 void fill_desc(fvm_ctx* ctx, BlockDesc* h, const double* dt_ptr = nullptr) {
   if (!dt_ptr) dt_ptr = ctx->d_dt;
   int base = 0;
   ctx->has_conn = false;
+  ctx->has_wall = false;
+  ctx->has_noslip = false;
   for (size_t i = 0; i < ctx->blocks.size(); ++i) {
     fvm_block* b = ctx->blocks[i];
     BlockDesc& d = h[i];
@@ -808,6 +820,15 @@ void fill_desc(fvm_ctx* ctx, BlockDesc* h, const double* dt_ptr = nullptr) {
     d.smax = b->smax_dev;
     d.halt = b->halt_dev;
     d.rv = ctx->visc_mu_ref > 0.0 ? b->rv : nullptr;
+    for (const fvm_ctx::Patch& q : ctx->patches) {
+      if (q.recv != b || q.donor != b) continue;
+      const int w = wall_kind(q.bc);
+      if (w == WALL_NONE) continue;
+      d.wall[q.face] = w;
+      if (w == WALL_NOSLIP) d.noslip |= 1 << q.face;
+      ctx->has_wall = true;
+      ctx->has_noslip = ctx->has_noslip || w == WALL_NOSLIP;
+    }
     b->tile_base = base;
     base += b->ntiles;
   }
@@ -841,6 +862,7 @@ int sync_patches(fvm_ctx* ctx) {
     }
     d.start = start;
     d.fixed_state = q.fixed ? 1 : 0;
+    d.neg_all = wall_kind(q.bc) == WALL_NOSLIP ? 1 : 0;
     if (q.fixed) {
       // p -> q in the order of src/solve/class_flow_field.f90:353-377
       const double rho = q.prim[0], u = q.prim[1], v = q.prim[2], w = q.prim[3], pr = q.prim[4];
@@ -961,19 +983,23 @@ int fill_ghosts_all(fvm_ctx* ctx, int buf) {
   return launch_patches(ctx, buf);
 }
 
+// The stage kernels' CONN variant writes connected faces' ghosts and applies
+// the no-slip wall condition; the other variant carries neither.
+bool conn_variant(const fvm_ctx* ctx) { return ctx->has_conn || ctx->has_noslip; }
+
 template <class Fam>
 hipError_t launch_family(fvm_ctx* ctx, int dim, int tile, const StageArgs& a, int grid) {
   if (dim == 2) {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_MUSCL: return Fam::m2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_WENO: return Fam::w2(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_SIMPLE: return Fam::s2(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m2(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w2(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
     }
   } else {
     switch (ctx->recon) {
-      case RECON_SIMPLE: return Fam::s3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_MUSCL: return Fam::m3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
-      case RECON_WENO: return Fam::w3(ctx->flux, tile, ctx->has_conn, a, grid, ctx->stream);
+      case RECON_SIMPLE: return Fam::s3(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
+      case RECON_MUSCL: return Fam::m3(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
+      case RECON_WENO: return Fam::w3(ctx->flux, tile, conn_variant(ctx), a, grid, ctx->stream);
     }
   }
   return hipErrorInvalidValue;
@@ -1032,7 +1058,7 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf, const BlockDesc* tab) {
   v.V.rr_gas = 1.0 / ctx->visc_r;
   v.V.rt_ref = 1.0 / ctx->visc_t_ref;
   v.V.sfac = ctx->visc_t_ref + ctx->visc_s;
-  v.tab = ctx->has_conn ? tab : nullptr;
+  v.tab = ctx->has_conn || ctx->has_wall ? tab : nullptr;
   v.in_buf = in_buf;
   for (size_t bi = 0; bi < ctx->blocks.size(); ++bi) {
     fvm_block* b = ctx->blocks[bi];
@@ -1454,7 +1480,9 @@ int set_face_kind(fvm_block* b, int face, int kind, const double* prim) {
   if (b->dim == 2 && face >= 4) return fail(FVM_EINVAL, "2-D block has no k faces");
   const BcClass cls = bc_class(kind);
   if (cls == BCC_UNKNOWN) return fail(FVM_EINVAL, "unknown BC kind %d", kind);
-  if (cls == BCC_NOSLIP) return fail(FVM_EINVAL, "no-slip walls (BC kind %d) are not supported", kind);
+  if (kind == FVM_BC_WALL_VISCOUS_ISOTHERMAL)
+    return fail(FVM_EINVAL, "isothermal no-slip walls (BC kind %d) are not supported; kind %d is the "
+                "adiabatic no-slip wall", kind, FVM_BC_WALL_VISCOUS);
   if (cls == BCC_STATE && !prim)
     return fail(FVM_EINVAL, "BC kind %d needs a state: fvm_block_set_bc_state", kind);
   if (cls != BCC_STATE && prim) return fail(FVM_EINVAL, "BC kind %d takes no state", kind);
@@ -1463,10 +1491,10 @@ int set_face_kind(fvm_block* b, int face, int kind, const double* prim) {
   if (face_connected(ctx, b, face)) return fail(FVM_EINVAL, "face %d is connected to another block", face);
   if (cls == BCC_PERIODIC && face_connected(ctx, b, other))
     return fail(FVM_EINVAL, "face %d is connected to another block", other);
-  if ((cls == BCC_MIRROR || cls == BCC_STATE) && ctx->visc_mu_ref > 0.0)
-    return fail(FVM_EINVAL, "wall, symmetry, inflow and farfield faces are not supported with viscous terms");
+  if (cls == BCC_STATE && ctx->visc_mu_ref > 0.0)
+    return fail(FVM_EINVAL, "inflow and farfield faces are not supported with viscous terms");
   const int n[3] = {b->g.nx, b->g.ny, b->g.nz};
-  if (cls == BCC_MIRROR && n[d] < kGhost + 1)
+  if ((cls == BCC_MIRROR || cls == BCC_NOSLIP) && n[d] < kGhost + 1)
     return fail(FVM_EINVAL, "a mirrored face needs >= %d vertices across the block", kGhost + 1);
   if (cls == BCC_STATE) {
     for (int c = 0; c < 5; ++c)
@@ -2011,11 +2039,10 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
   if (!(t_ref > 0.0) || !(prandtl > 0.0) || !(r_gas > 0.0) || !(sutherland_s >= 0.0) ||
       !std::isfinite(mu_ref))
     return fail(FVM_EINVAL, "viscous parameters need t_ref, prandtl, r_gas > 0 and s >= 0");
-  for (fvm_block* b : ctx->blocks)
-    for (int f = 0; f < 6; ++f)
-      if (b->g.bc[f] == BC_PATCH)
-        return fail(FVM_EINVAL, "viscous terms on blocks with connection patches or wall, symmetry, "
-                    "inflow or farfield faces are not supported");
+  for (const fvm_ctx::Patch& q : ctx->patches)
+    if (wall_kind(q.bc) == WALL_NONE)
+      return fail(FVM_EINVAL, "viscous terms on blocks with connection patches or inflow or farfield "
+                  "faces are not supported");
   if (int rc = set_device(ctx)) return rc;
   for (fvm_block* b : ctx->blocks)
     if (int rc = alloc_rv(b)) return rc;

@@ -345,11 +345,32 @@ __device__ __forceinline__ int64_t gidx_of(Tab tab, int blk, int i, int j, int k
 // Write one output vertex and, across connected faces, the neighbour block's
 // ghosts it donates to (shared interface vertex: vertex n-1-l of this block is
 // ghost -l of the next; abutting blocks: vertex n-l is ghost -l).  The block's
-// own ghosts are not written: readers map such points with bmap.
+// own ghosts are not written: readers map such points with bmap.  A vertex on
+// a no-slip wall first loses its momentum and the momentum's kinetic energy
+// (v is updated in place, so the caller's CFL wave speed sees the stored
+// state).  Both belong to the CONN variant, which launches run whenever a
+// block has a connected face or a no-slip wall: the other variant carries
+// neither's code.
 template <int NC, int NDIR, bool CONN, class Tab, class Desc>
 __device__ __forceinline__ void store_with_ghosts(const BlockGeom& g, Tab tab, const Desc& B, int out_buf,
                                                   double* __restrict__ out, int i, int j, int k,
-                                                  const double v[NC]) {
+                                                  double v[NC]) {
+  if (CONN && B.noslip) {   // uniform over the block
+    const int nn[3] = {g.nx, g.ny, g.nz}, xx[3] = {i, j, k};
+    bool on = false;
+#pragma unroll
+    for (int d = 0; d < NDIR; ++d)
+      on = on || (xx[d] == 0 && ((B.noslip >> (2 * d)) & 1)) ||
+           (xx[d] == nn[d] - 1 && ((B.noslip >> (2 * d + 1)) & 1));
+    if (on) {
+      double m2 = v[1] * v[1] + v[2] * v[2];
+      if constexpr (NC == 5) m2 = m2 + v[3] * v[3];
+      const double ke = 0.5 * m2 / v[0];
+      v[NC - 1] = v[NC - 1] - ke;
+#pragma unroll
+      for (int m = 1; m < NC - 1; ++m) v[m] = 0.0;
+    }
+  }
   const uint32_t base = gidx32(g, i, j, k);
 #pragma unroll
   for (int c = 0; c < NC; ++c) st_comp(out, g.cs, c, base, v[c]);

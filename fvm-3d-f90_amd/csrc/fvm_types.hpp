@@ -54,8 +54,18 @@ struct PatchDesc {
   int64_t rmin[3], dmin[3];        // corresponding corners
   int64_t start;                   // first point of this patch in the launch's point list
   int fixed_state;                 // 1: the ghosts take `fixed` (farfield / inflow faces)
+  int neg_all;                     // 1: every momentum component negated (no-slip wall
+                                   // mirror), else each along map_sign
   double fixed[5];                 // conserved state, device component order
 };
+
+// Wall faces: a whole-face mirror patch of the block onto itself.  Slip
+// (wall, inviscid wall, symmetry): the momentum normal to the face is
+// negated in the ghosts.  No-slip (adiabatic viscous wall): every momentum
+// component is negated in the ghosts, and each stage sets the wall vertices'
+// momentum to zero, taking its kinetic energy out of the total energy
+// (pressure, density and temperature unchanged).
+enum : int { WALL_NONE = 0, WALL_SLIP = 1, WALL_NOSLIP = 2 };
 
 // CFL wave-speed maximum of one solve (bits of a non-negative double, or of
 // a NaN: as unsigned integers those order like the values and NaN wins),
@@ -115,6 +125,8 @@ struct BlockDesc {
   const int* halt;                 // per-block: if set, the block's tiles are skipped
   const double* rv;                // viscous residual of the stage input (NC-1 components:
                                    // momentum, energy; stride g.cs), or null: inviscid
+  int wall[6];                     // WALL_* per face
+  int noslip;                      // bit f: face f is a no-slip wall
 };
 
 // Central viscous terms: Sutherland viscosity, T = p / (rho R), k = mu cp/Pr.

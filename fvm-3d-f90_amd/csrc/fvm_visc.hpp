@@ -10,7 +10,11 @@
 //     vertex), so no edge or corner ghost is read; across a face connected
 //     to another block the point is that block's vertex (directions taken in
 //     the order i, j, k, each in the block the previous one reached), so a
-//     cut grid sees the values of the grid it was cut from;
+//     cut grid sees the values of the grid it was cut from; across a wall
+//     face the point is the mirror vertex (vertex l for ghost -l) with the
+//     normal velocity negated (slip wall, symmetry) or every velocity
+//     component negated (no-slip wall), each crossing flipping its
+//     components again;
 //   - face between a and b = a + e_d: means of the two vertices, normal
 //     derivative (phi_b - phi_a) / dr_d, tangential derivatives the mean of
 //     the central differences at a and b, /(4 dr_t);
@@ -70,6 +74,7 @@ __device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
 struct VPoint {
   const double* q;
   int64_t c, cs;
+  int flip;          // bit d: velocity component d negated (wall crossings)
 };
 
 __device__ __forceinline__ int64_t vidx(const BlockGeom& g, int i, int j, int k) {
@@ -83,9 +88,9 @@ __device__ __forceinline__ VPoint resolve(const ViscArgs& a, int x, int y, int z
   if (!a.tab) {
     const int mi = vmap(x, g.nx, g.bc[0], g.bc[1]), mj = vmap(y, g.ny, g.bc[2], g.bc[3]);
     const int mk = DIM == 3 ? vmap(z, g.nz, g.bc[4], g.bc[5]) : 0;
-    return VPoint{a.q, vidx(g, mi, mj, mk), g.cs};
+    return VPoint{a.q, vidx(g, mi, mj, mk), g.cs, 0};
   }
-  int b = a.self;
+  int b = a.self, flip = 0;
   int c[3] = {x, y, DIM == 3 ? z : 0};
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -101,12 +106,22 @@ __device__ __forceinline__ VPoint resolve(const ViscArgs& a, int x, int y, int z
       const int nn = d == 0 ? gn.nx : (d == 1 ? gn.ny : gn.nz);
       c[d] = c[d] < 0 ? c[d] + nn - sh : c[d] - n + sh;
       b = nb;
+    } else if (B.wall[f] != WALL_NONE) {
+      c[d] = c[d] < 0 ? -c[d] : 2 * (n - 1) - c[d];
+      flip ^= B.wall[f] == WALL_NOSLIP ? (1 << DIM) - 1 : 1 << d;
     } else {
       c[d] = vmap(c[d], n, B.g.bc[2 * d], B.g.bc[2 * d + 1]);
     }
   }
   const BlockDesc& B = a.tab[b];
-  return VPoint{B.q[a.in_buf], vidx(B.g, c[0], c[1], c[2]), B.g.cs};
+  return VPoint{B.q[a.in_buf], vidx(B.g, c[0], c[1], c[2]), B.g.cs, flip};
+}
+
+// A wall crossing's velocity signs (VPoint::flip)
+__device__ __forceinline__ void apply_flip(int flip, double out[5]) {
+  if (flip & 1) out[0] = -out[0];
+  if (flip & 2) out[1] = -out[1];
+  if (flip & 4) out[2] = -out[2];
 }
 
 // u, v, w, T, mu of the conserved state at a resolved vertex
@@ -238,7 +253,9 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
   auto load_plane = [&](int k, int slot) {
     for (int e = t; e < PN; e += NT) {
       double v[5];
-      visc::prims<5>(visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k), V, R, v);
+      const visc::VPoint pt = visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k);
+      visc::prims<5>(pt, V, R, v);
+      visc::apply_flip(pt.flip, v);
 #pragma unroll
       for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
     }
@@ -301,7 +318,9 @@ __global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
   const visc::Rec R = visc::make_rec(g, V);
   for (int e = t; e < PN; e += NT) {
     double v[5];
-    visc::prims<4>(visc::resolve<2>(a, i0 - 1 + e % W, j0 - 1 + e / W, 0), V, R, v);
+    const visc::VPoint pt = visc::resolve<2>(a, i0 - 1 + e % W, j0 - 1 + e / W, 0);
+    visc::prims<4>(pt, V, R, v);
+    visc::apply_flip(pt.flip, v);
 #pragma unroll
     for (int m = 0; m < 5; ++m) S[m][e] = v[m];
   }

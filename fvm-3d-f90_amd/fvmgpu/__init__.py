@@ -29,6 +29,7 @@ IC_MAX_PARAMS = 176
 BC_INFLOW, BC_INFLOW_SUBSONIC, BC_INFLOW_SUPERSONIC = 10, 11, 12
 BC_OUTFLOW, BC_OUTFLOW_SUBSONIC, BC_OUTFLOW_SUPERSONIC = 20, 21, 22
 BC_WALL, BC_WALL_INVISCID, BC_FARFIELD, BC_EXTRAPOLATE, BC_SYMMETRY = 30, 31, 40, 41, 50
+BC_WALL_VISCOUS_ISOTHERMAL, BC_WALL_VISCOUS = 32, 33
 NUMERICS_EXACT, NUMERICS_FAST = 0, 1
 LAUNCH_PER_BLOCK, LAUNCH_BATCHED, LAUNCH_PERSISTENT = 0, 1, 2
 # snapshot variables: the reference's output variables

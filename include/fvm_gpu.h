@@ -62,6 +62,11 @@ extern "C" {
 #define FVM_BC_OUTFLOW_SUPERSONIC 22  /* extrapolation                               */
 #define FVM_BC_WALL               30  /* inviscid (slip) wall: mirror, normal momentum negated */
 #define FVM_BC_WALL_INVISCID      31  /* as FVM_BC_WALL                              */
+#define FVM_BC_WALL_VISCOUS_ISOTHERMAL 32  /* refused                                  */
+#define FVM_BC_WALL_VISCOUS       33  /* adiabatic no-slip wall: mirror, every momentum
+                                         component negated; each stage sets the wall
+                                         vertices' momentum to 0 (kinetic energy taken
+                                         out of the total energy)                   */
 #define FVM_BC_FARFIELD           40  /* fixed free-stream ghost state (fvm_block_set_bc_state) */
 #define FVM_BC_EXTRAPOLATE        41  /* extrapolation                               */
 #define FVM_BC_SYMMETRY           50  /* mirror, normal momentum negated             */
@@ -138,11 +143,17 @@ int fvm_block_destroy(fvm_block *blk);
  * Default FVM_BC_EXTRAP (src/base/defs_block_data.f90:226-256).  Periodic
  * applies to both faces of the direction.  Wall and symmetry faces mirror
  * the block about the boundary vertex plane (ghost l = vertex l inside,
- * momentum normal to the face negated); outflow and extrapolate kinds copy
- * the boundary vertex.  Inflow and farfield faces take a state:
- * fvm_block_set_bc_state.  A face's kind replaces its previous one
+ * momentum normal to the face negated); the no-slip wall (FVM_BC_WALL_VISCOUS)
+ * mirrors with every momentum component negated and sets the wall vertices'
+ * momentum to zero after every stage, removing its kinetic energy from the
+ * total energy (the isothermal no-slip kind 32 is refused); outflow and
+ * extrapolate kinds copy the boundary vertex.  Inflow and farfield faces take
+ * a state: fvm_block_set_bc_state.  A face's kind replaces its previous one
  * (connected faces excepted).  Wall, symmetry and state faces are refilled
- * on the device after every stage; not with viscous terms. */
+ * on the device after every stage.  With viscous terms the stencil crosses a
+ * wall face to the mirror vertex, its normal velocity (slip, symmetry) or
+ * every velocity component (no-slip) negated; inflow and farfield faces are
+ * not available with viscous terms. */
 int fvm_block_set_bc(fvm_block *blk, int face, int kind);
 
 /* Inflow / farfield face (FVM_BC_INFLOW*, FVM_BC_FARFIELD): every ghost
@@ -302,8 +313,9 @@ int fvm_step_noalloc(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt
  * mu_ref <= 0 switches the terms off.  On blocks joined by fvm_block_connect
  * a stencil point beyond a connected face is the neighbour's vertex (edge and
  * corner points through the neighbour's neighbour), so a cut grid gives the
- * results of the grid it was cut from.  Not with connection patches
- * (fvm_conn_add) or wall, symmetry, inflow or farfield faces. */
+ * results of the grid it was cut from.  Wall and symmetry faces mirror the
+ * stencil (fvm_block_set_bc).  Not with connection patches (fvm_conn_add) or
+ * inflow or farfield faces. */
 int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland_s,
                     double prandtl, double r_gas);
 

@@ -482,9 +482,20 @@ static void q_to_p_point(double gamma, const double *q, int64_t stride, double *
     p[4] = (gamma - 1.0) * (q[4 * stride] - 0.5 * rho * (u * u + v * v + w * w));
 }
 
+/* Wall kind of a face code: 0 none, 1 slip (normal velocity negated),
+ * 2 no-slip (every velocity component negated) */
+static int wall_of(int kind)
+{
+    if (kind == ORC_BC_WALL || kind == ORC_BC_WALL_INVISCID || kind == ORC_BC_SYMMETRY) return 1;
+    if (kind == ORC_BC_NOSLIP) return 2;
+    return 0;
+}
+
 /* Ghost fill of p and q on the six face regions (edges/corners untouched).
  * Extrapolation copies the boundary vertex (defs_block_data.f90:226-256);
- * periodic copies vertex (r mod n) for real index r outside [0, n). */
+ * periodic copies vertex (r mod n) for real index r outside [0, n); a wall
+ * copies the mirror vertex (ghost -l <- vertex l) and negates the normal
+ * velocity (slip) or the first dim velocity components (no-slip). */
 static void fill_ghosts(blk_t *b, const int bc[6], int dim)
 {
     int64_t cs = b->ng[0] * b->ng[1] * b->ng[2];
@@ -500,8 +511,11 @@ static void fill_ghosts(blk_t *b, const int bc[6], int dim)
             for (int64_t l = 1; l <= g; ++l) {
                 int64_t gpos = side == 0 ? g - l : g + b->n[d] - 1 + l;
                 int64_t spos;
+                int wall = wall_of(kind);
                 if (kind == ORC_BC_PERIODIC)
                     spos = side == 0 ? g + b->n[d] - l : g + l - 1;
+                else if (wall)
+                    spos = side == 0 ? g + l : g + b->n[d] - 1 - l;
                 else
                     spos = side == 0 ? g : g + b->n[d] - 1;
                 int64_t l2[3] = {lo[0], lo[1], lo[2]}, h2[3] = {hi[0], hi[1], hi[2]};
@@ -514,6 +528,11 @@ static void fill_ghosts(blk_t *b, const int bc[6], int dim)
                             int64_t dst = gidx(b, i, j, k), src = gidx(b, s[0], s[1], s[2]);
                             memcpy(b->p + 5 * dst, b->p + 5 * src, 5 * sizeof(double));
                             for (int e = 0; e < NEQ; ++e) b->q[e * cs + dst] = b->q[e * cs + src];
+                            for (int e = 0; e < dim; ++e)
+                                if (wall == 2 || (wall == 1 && e == d)) {
+                                    b->p[5 * dst + 1 + e] = -b->p[5 * dst + 1 + e];
+                                    b->q[(1 + e) * cs + dst] = -b->q[(1 + e) * cs + dst];
+                                }
                         }
             }
         }
@@ -672,18 +691,45 @@ static void visc_prims(const int64_t n[3], const orc_config *c, const double *p_
     }
 }
 
-/* face flux between vertex a and a + e_d; xs = the vertex index triples
- * resolved through vmap by the caller (pts[which][..]) */
+/* The primitive quantities stencil point (x, y, z) reads, per direction:
+ * periodic wrap, wall mirror (vertex l for ghost -l; the normal velocity
+ * negated for a slip wall, the first dim velocity components for a no-slip
+ * wall, each crossing flipping its components again), else clamp to the
+ * boundary vertex. */
+static vprim_t vget(const vprim_t *P, const int64_t n[3], const orc_config *c, int64_t x,
+                    int64_t y, int64_t z)
+{
+    int64_t r[3] = {x, y, c->dim == 3 ? z : 0};
+    int flip = 0;
+    for (int d = 0; d < c->dim; ++d) {
+        if (r[d] >= 0 && r[d] < n[d]) continue;
+        int kind = c->bc[2 * d + (r[d] < 0 ? 0 : 1)];
+        int wall = wall_of(kind);
+        if (kind == ORC_BC_PERIODIC) {
+            r[d] = vmap(r[d], n[d], ORC_BC_PERIODIC, ORC_BC_PERIODIC);
+        } else if (wall) {
+            r[d] = r[d] < 0 ? -r[d] : 2 * (n[d] - 1) - r[d];
+            flip ^= wall == 2 ? (1 << c->dim) - 1 : 1 << d;
+        } else {
+            r[d] = vmap(r[d], n[d], ORC_BC_EXTRAP, ORC_BC_EXTRAP);
+        }
+    }
+    vprim_t o = P[r[0] + n[0] * (r[1] + n[1] * r[2])];
+    if (flip & 1) o.u = -o.u;
+    if (flip & 2) o.v = -o.v;
+    if (flip & 4) o.w = -o.w;
+    return o;
+}
+
+/* face flux between vertex a and a + e_d */
 static void visc_face(const vprim_t *P, const int64_t n[3], const orc_config *c, int d,
                       const int64_t a[3], double F[4])
 {
     int dim = c->dim;
     int64_t b[3] = {a[0], a[1], a[2]};
     b[d] += 1;
-#define VIDX(x, y, z) (vmap((x), n[0], c->bc[0], c->bc[1]) + n[0] * (vmap((y), n[1], c->bc[2], c->bc[3]) \
-                       + n[1] * (dim == 3 ? vmap((z), n[2], c->bc[4], c->bc[5]) : 0)))
-    const vprim_t *A = P + VIDX(a[0], a[1], a[2]);
-    const vprim_t *B = P + VIDX(b[0], b[1], b[2]);
+    const vprim_t Av = vget(P, n, c, a[0], a[1], a[2]), Bv = vget(P, n, c, b[0], b[1], b[2]);
+    const vprim_t *A = &Av, *B = &Bv;
     double g[3][4];   /* g[e][q]: d/dx_e of q in (u, v, w, T) at the face */
     for (int e = 0; e < 3; ++e)
         for (int q = 0; q < 4; ++q) g[e][q] = 0.0;
@@ -696,14 +742,14 @@ static void visc_face(const vprim_t *P, const int64_t n[3], const orc_config *c,
         int64_t ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]};
         int64_t bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
         ap[e] += 1; am[e] -= 1; bp[e] += 1; bm[e] -= 1;
-        const vprim_t *Ap = P + VIDX(ap[0], ap[1], ap[2]), *Am = P + VIDX(am[0], am[1], am[2]);
-        const vprim_t *Bp = P + VIDX(bp[0], bp[1], bp[2]), *Bm = P + VIDX(bm[0], bm[1], bm[2]);
+        const vprim_t Apv = vget(P, n, c, ap[0], ap[1], ap[2]), Amv = vget(P, n, c, am[0], am[1], am[2]);
+        const vprim_t Bpv = vget(P, n, c, bp[0], bp[1], bp[2]), Bmv = vget(P, n, c, bm[0], bm[1], bm[2]);
+        const vprim_t *Ap = &Apv, *Am = &Amv, *Bp = &Bpv, *Bm = &Bmv;
         const double vap[4] = {Ap->u, Ap->v, Ap->w, Ap->t}, vam[4] = {Am->u, Am->v, Am->w, Am->t};
         const double vbp[4] = {Bp->u, Bp->v, Bp->w, Bp->t}, vbm[4] = {Bm->u, Bm->v, Bm->w, Bm->t};
         for (int q = 0; q < 4; ++q)
             g[e][q] = ((vap[q] - vam[q]) + (vbp[q] - vbm[q])) / (4.0 * c->dr[e]);
     }
-#undef VIDX
     const double mu = 0.5 * (A->mu + B->mu);
     const double uf[3] = {0.5 * (A->u + B->u), 0.5 * (A->v + B->v), 0.5 * (A->w + B->w)};
     const double div = g[0][0] + g[1][1] + g[2][2];
@@ -836,6 +882,34 @@ static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, do
                       double *rhs_out, double *res_out, double *dt_out, int *steps_out,
                       double *t_out);
 
+/* No-slip walls after a stage's combine: a vertex on such a face keeps rho,
+ * loses its momentum, and its total energy loses the momentum's kinetic
+ * energy 0.5 |rho u|^2 / rho (pressure and temperature unchanged).  q:
+ * component-major real vertices (the stage output). */
+static void noslip_fix(const int64_t n[3], const orc_config *c, double *q)
+{
+    int64_t nreal = n[0] * n[1] * n[2];
+    int any = 0;
+    for (int f = 0; f < 2 * c->dim; ++f) any = any || c->bc[f] == ORC_BC_NOSLIP;
+    if (!any) return;
+    for (int64_t k = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t x[3] = {i, j, k};
+                int on = 0;
+                for (int d = 0; d < c->dim; ++d)
+                    on = on || (x[d] == 0 && c->bc[2 * d] == ORC_BC_NOSLIP) ||
+                         (x[d] == n[d] - 1 && c->bc[2 * d + 1] == ORC_BC_NOSLIP);
+                if (!on) continue;
+                int64_t v = i + n[0] * (j + n[1] * k);
+                double m2 = q[nreal + v] * q[nreal + v] + q[2 * nreal + v] * q[2 * nreal + v];
+                if (c->dim == 3) m2 = m2 + q[3 * nreal + v] * q[3 * nreal + v];
+                double ke = 0.5 * m2 / q[v];
+                q[4 * nreal + v] = q[4 * nreal + v] - ke;
+                for (int e = 1; e <= c->dim; ++e) q[e * nreal + v] = 0.0;
+            }
+}
+
 int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
               int nsteps, const double *p_real, double *q_out, double *rhs_out,
               double *res_out, double *dt_out)
@@ -908,6 +982,7 @@ static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, do
             } else {
                 orc_runge_kutta(s, dt, NEQ * nreal, q0, qm, rhs);
             }
+            noslip_fix(n, c, qm);
             scatter_real(&b, qm);
             update_primitives(&b, c->gamma);
             fill_ghosts(&b, c->bc, c->dim);

@@ -27,6 +27,15 @@
 
 #define ORC_BC_EXTRAP     0     /* zeroth-order copy of the boundary vertex */
 #define ORC_BC_PERIODIC   1     /* nodal periodic: vertex n == vertex 0     */
+/* Walls (the reference's codes, type_defs_bc.f90:11-30, which it never
+ * applies; this build's definition): ghost -l is the mirror vertex l */
+#define ORC_BC_WALL       30    /* slip: normal velocity negated            */
+#define ORC_BC_WALL_INVISCID 31 /* as ORC_BC_WALL                           */
+#define ORC_BC_NOSLIP     33    /* adiabatic no-slip: every velocity component
+                                   negated; after each stage the wall vertices'
+                                   momentum is set to 0 and its kinetic energy
+                                   taken out of the total energy             */
+#define ORC_BC_SYMMETRY   50    /* as ORC_BC_WALL                           */
 
 /* ---- per-pencil scheme routines (same argument meaning as the reference) */
 void orc_recon_simple(int64_t nface, int64_t dghost, int64_t npg,

@@ -20,6 +20,7 @@ SIMPLE, MUSCL, WENO = 200, 201, 202
 LIM_NONE, LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA = 250, 251, 252, 253
 ROE, VANLEER, AUSM, AUSMPLUS = 300, 301, 303, 1303
 BC_EXTRAP, BC_PERIODIC = 0, 1
+BC_WALL, BC_NOSLIP, BC_SYMMETRY = 30, 33, 50      # slip wall, adiabatic no-slip wall
 
 
 class OrcConfig(ctypes.Structure):

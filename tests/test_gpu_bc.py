@@ -142,13 +142,21 @@ def test_bc_kinds_api():
             a.set_bc_state(0, fvmgpu.BC_FARFIELD, (-1.0, 0, 0, 0, 1.0))
         with pytest.raises(fvmgpu.FvmError):
             a.set_bc_state(0, fvmgpu.BC_SYMMETRY, (1.0, 0, 0, 0, 1.0))
-        with pytest.raises(fvmgpu.FvmError, match="no-slip"):
-            a.set_bc(0, 33)
+        with pytest.raises(fvmgpu.FvmError, match="isothermal"):
+            a.set_bc(0, fvmgpu.BC_WALL_VISCOUS_ISOTHERMAL)
+        a.set_bc(0, fvmgpu.BC_WALL_VISCOUS)     # the adiabatic no-slip wall
+        a.set_bc(0, fvmgpu.BC_EXTRAP)
         with pytest.raises(fvmgpu.FvmError):
             a.set_bc(0, 999)
         a.set_bc(2, fvmgpu.BC_SYMMETRY)
+        ctx.viscous(1e-3)                        # walls and symmetry planes take viscous terms
         with pytest.raises(fvmgpu.FvmError, match="viscous"):
-            ctx.viscous(1e-3)
+            a.set_bc_state(3, fvmgpu.BC_FARFIELD, (1.0, 0, 0, 0, 1.0))
+        ctx.viscous(0.0)
+        a.set_bc_state(3, fvmgpu.BC_FARFIELD, (1.0, 0, 0, 0, 1.0))
+        with pytest.raises(fvmgpu.FvmError, match="viscous"):
+            ctx.viscous(1e-3)                    # a fixed-state face: not with viscous terms
+        a.set_bc(3, fvmgpu.BC_OUTFLOW)
         a.set_bc(2, fvmgpu.BC_OUTFLOW)          # replaced: extrapolation again
         a.set_bc(1, fvmgpu.BC_WALL)
         a.connect(1, b)                          # a wall face may become connected
