@@ -253,6 +253,15 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
       cons_to_prim<NC>(qR, R, P.gamma);
       return;
     }
+    if (!kFast && P.weno_textbook) {
+      // textbook: the right state is the mirror image of the stencil; both
+      // faces share the smoothness indicators
+#pragma unroll
+      for (int c = 0; c < NC; ++c) weno_js_pair_exact(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], qL[c], qR[c]);
+      cons_to_prim<NC>(qL, L, P.gamma);
+      cons_to_prim<NC>(qR, R, P.gamma);
+      return;
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       qL[c] = weno_js(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], P.weno_textbook);

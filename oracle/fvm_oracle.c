@@ -112,7 +112,12 @@ void orc_recon_muscl(int64_t nface, int64_t dghost, double kappa, int limiter,
 
 /* Jiang-Shu weights on a 5-point stencil, value at the right face of u[2];
  * scheme_WENO.f90:87-121.  textbook = 0 keeps the reference's IS_1 term
- * (u2 + u4)**2 (scheme_WENO.f90:107-108); textbook = 1 uses (u2 - u4)**2. */
+ * (u2 + u4)**2 (scheme_WENO.f90:107-108) and its operation order;
+ * textbook = 1 uses (u2 - u4)**2 and forms the smoothness terms in an order
+ * symmetric under stencil reversal, t = (a + c) - 2 b and s = (3 c + a) - 4 b
+ * (the weight-3 point first), so the mirrored stencil's indicators are the
+ * forward ones reversed, bit for bit (the GPU shares them between a vertex's
+ * two faces). */
 static double weno_js(const double u[5], int textbook)
 {
     const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
@@ -122,12 +127,22 @@ static double weno_js(const double u[5], int textbook)
     double q0 = c13 * u[0] - c76 * u[1] + c116 * u[2];
     double q1 = -(c16 * u[1]) + c56 * u[2] + c13 * u[3];
     double q2 = c13 * u[2] + c56 * u[3] - c16 * u[4];
-    double t0 = u[0] - 2.0 * u[1] + u[2];
-    double s0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
-    double t1 = u[1] - 2.0 * u[2] + u[3];
-    double s1 = textbook ? (u[1] - u[3]) : (u[1] + u[3]);
-    double t2 = u[2] - 2.0 * u[3] + u[4];
-    double s2 = 3.0 * u[2] - 4.0 * u[3] + u[4];
+    double t0, s0, t1, s1, t2, s2;
+    if (textbook) {
+        t0 = (u[0] + u[2]) - 2.0 * u[1];
+        s0 = (3.0 * u[2] + u[0]) - 4.0 * u[1];
+        t1 = (u[1] + u[3]) - 2.0 * u[2];
+        s1 = u[1] - u[3];
+        t2 = (u[2] + u[4]) - 2.0 * u[3];
+        s2 = (3.0 * u[2] + u[4]) - 4.0 * u[3];
+    } else {
+        t0 = u[0] - 2.0 * u[1] + u[2];
+        s0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
+        t1 = u[1] - 2.0 * u[2] + u[3];
+        s1 = u[1] + u[3];
+        t2 = u[2] - 2.0 * u[3] + u[4];
+        s2 = 3.0 * u[2] - 4.0 * u[3] + u[4];
+    }
     double IS_0 = c1312 * (t0 * t0) + c14 * (s0 * s0);
     double IS_1 = c1312 * (t1 * t1) + c14 * (s1 * s1);
     double IS_2 = c1312 * (t2 * t2) + c14 * (s2 * s2);
