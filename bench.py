@@ -344,6 +344,7 @@ def bench_riemann(args):
     blk = ctx.block((n, n, 1), (dx, dx, 1.0))
     blk.upload(q)
     ctx.numerics(numerics_code(fvmgpu, args))
+    settle(ctx, cfl)
     if not args.no_tune:
         ctx.autotune(reps=3)
     tile = blk.launch_shape()
