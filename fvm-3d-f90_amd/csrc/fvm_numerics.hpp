@@ -337,13 +337,23 @@ __device__ __forceinline__ void weno_js_pair_exact(double u0, double u1, double 
   const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
   const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
   const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
-  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
-  const double a1 = c_1 / E1;
+  // the five weights: each divisor's reciprocal serves its two quotients at once
+  double a0, a2, b0, b2;
+  {
+    const Recip r = erecip(e0 * e0);
+    a0 = ediv(c_0, r);
+    b2 = ediv(c_2, r);
+  }
+  {
+    const Recip r = erecip(e2 * e2);
+    a2 = ediv(c_2, r);
+    b0 = ediv(c_0, r);
+  }
+  const double a1 = ediv(c_1, e1 * e1);
   {
     const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
     const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
     const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
-    const double a0 = c_0 / E0, a2 = c_2 / E2;
     const Recip ras = erecip(a0 + a1 + a2);
     right = ediv(a0, ras) * q0 + ediv(a1, ras) * q1 + ediv(a2, ras) * q2;
   }
@@ -351,7 +361,6 @@ __device__ __forceinline__ void weno_js_pair_exact(double u0, double u1, double 
     const double q0 = c13 * u4 - c76 * u3 + c116 * u2;
     const double q1 = -(c16 * u3) + c56 * u2 + c13 * u1;
     const double q2 = c13 * u2 + c56 * u1 - c16 * u0;
-    const double b0 = c_0 / E2, b2 = c_2 / E0;
     const Recip rbs = erecip(b0 + a1 + b2);
     left = ediv(b0, rbs) * q0 + ediv(a1, rbs) * q1 + ediv(b2, rbs) * q2;
   }

@@ -212,7 +212,7 @@ struct ReconSimple {   // scheme_reconstruct_simple.f90:42-85
   static constexpr int RV = 0, H = 1;
   static constexpr bool CONS = false, SAME = true;
   template <int NC>
-  __device__ static void vertex(const double (*V)[NC], const Params&, double L[NC], double R[NC]) {
+  __device__ __forceinline__ static void vertex(const double (*V)[NC], const Params&, double L[NC], double R[NC]) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) { L[c] = V[0][c]; R[c] = V[0][c]; }
   }
@@ -222,12 +222,12 @@ struct ReconMuscl {    // scheme_reconstruct_MUSCL.f90:47-226
   static constexpr int RV = 1, H = 2;
   static constexpr bool CONS = false, SAME = false;
   template <int NC, int LIM>
-  __device__ static void vertex_lim(const double (*V)[NC], double kappa, double L[NC], double R[NC]) {
+  __device__ __forceinline__ static void vertex_lim(const double (*V)[NC], double kappa, double L[NC], double R[NC]) {
     muscl_vertex_all<LIM, NC>(V, kappa, L, R);
   }
   // one uniform branch on the limiter per vertex, none per component
   template <int NC>
-  __device__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
+  __device__ __forceinline__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
     switch (P.limiter) {
       case LIM_MINMOD: vertex_lim<NC, LIM_MINMOD>(V, P.kappa, L, R); break;
       case LIM_VANLEER: vertex_lim<NC, LIM_VANLEER>(V, P.kappa, L, R); break;
@@ -241,7 +241,7 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
   static constexpr int RV = 2, H = 3;
   static constexpr bool CONS = true, SAME = false;
   template <int NC>
-  __device__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
+  __device__ __forceinline__ static void vertex(const double (*V)[NC], const Params& P, double L[NC], double R[NC]) {
     double qL[NC], qR[NC];
     // textbook WENO: both faces of the vertex share the smoothness
     // indicators (in the 3-D stage this spills a few registers, and is still
