@@ -77,17 +77,19 @@ def test_walls_2d(bc, scheme):
             assert np.all(wall == 0.0)
 
 
-@pytest.mark.parametrize("bc", [(N,) * 6, (N, N, P, P, W, N), (S, E, N, N, N, W)])
-def test_walls_3d(bc):
+@pytest.mark.parametrize("bc,scheme", [((N,) * 6, MUSCL_ROE), ((N, N, P, P, W, N), MUSCL_ROE),
+                                       ((S, E, N, N, N, W), MUSCL_ROE), ((N, W, N, N, S, N), WENO_AUSMP)])
+def test_walls_3d(bc, scheme):
     """Walls on several faces of a 3-D block, corners where no-slip walls
-    meet (each crossing flips the velocity again), viscous terms."""
+    meet (each crossing flips the velocity again), viscous terms; WENO's
+    five-point stencil reads three mirror layers."""
     p = field((12, 13, 14), seed=12)
     sp = (1 / 13, 1 / 12, 1 / 11)
-    print(f"case walls 3-D 14x13x12 bc {bc}", flush=True)
-    q, _ = gpu(p, sp, bc, 4, **MUSCL_ROE)
-    qo, _ = oracle(p, sp, bc, 4, **MUSCL_ROE)
+    print(f"case walls 3-D 14x13x12 bc {bc} {scheme['recon']}/{scheme['flux']}", flush=True)
+    q, _ = gpu(p, sp, bc, 4, **scheme)
+    qo, _ = oracle(p, sp, bc, 4, **scheme)
     assert np.array_equal(q, qo)
-    qf, _ = gpu(p, sp, bc, 4, numerics=1, **MUSCL_ROE)
+    qf, _ = gpu(p, sp, bc, 4, numerics=1, **scheme)
     assert rel_linf(qf, qo) <= 1e-12
 
 
