@@ -1058,8 +1058,10 @@ int launch_viscous_all(fvm_ctx* ctx, int in_buf, const BlockDesc* tab) {
   v.V.rr_gas = 1.0 / ctx->visc_r;
   v.V.rt_ref = 1.0 / ctx->visc_t_ref;
   v.V.sfac = ctx->visc_t_ref + ctx->visc_s;
-  v.tab = ctx->has_conn || ctx->has_wall ? tab : nullptr;
+  v.tab = ctx->has_conn || ctx->has_wall || !ctx->patches.empty() ? tab : nullptr;
   v.in_buf = in_buf;
+  v.patches = ctx->d_patch;
+  v.npatch = (int)ctx->patches.size();
   for (size_t bi = 0; bi < ctx->blocks.size(); ++bi) {
     fvm_block* b = ctx->blocks[bi];
     v.self = (int)bi;
@@ -1610,8 +1612,6 @@ int fvm_conn_add(fvm_block* recv, const int64_t range_min[3], const int64_t rang
   // (a context holds blocks of one dimension; kept for the patch kernel's sake)
   if (recv->dim != donor->dim) return fail(FVM_EINVAL, "a 2-D and a 3-D block cannot be connected");
   fvm_ctx* ctx = recv->ctx;
-  if (ctx->visc_mu_ref > 0.0)
-    return fail(FVM_EINVAL, "viscous terms on blocks with connection patches are not supported");
   const int dim = recv->dim;
   const int nr[3] = {recv->g.nx, recv->g.ny, recv->g.nz};
   const int nd[3] = {donor->g.nx, donor->g.ny, donor->g.nz};
@@ -2040,9 +2040,8 @@ int fvm_viscous_set(fvm_ctx* ctx, double mu_ref, double t_ref, double sutherland
       !std::isfinite(mu_ref))
     return fail(FVM_EINVAL, "viscous parameters need t_ref, prandtl, r_gas > 0 and s >= 0");
   for (const fvm_ctx::Patch& q : ctx->patches)
-    if (wall_kind(q.bc) == WALL_NONE)
-      return fail(FVM_EINVAL, "viscous terms on blocks with connection patches or inflow or farfield "
-                  "faces are not supported");
+    if (q.fixed)
+      return fail(FVM_EINVAL, "viscous terms on blocks with inflow or farfield faces are not supported");
   if (int rc = set_device(ctx)) return rc;
   for (fvm_block* b : ctx->blocks)
     if (int rc = alloc_rv(b)) return rc;

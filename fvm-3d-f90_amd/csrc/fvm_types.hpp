@@ -149,6 +149,8 @@ struct ViscArgs {
   // are read from the neighbour (null table: a single block)
   const BlockDesc* tab;
   int self, in_buf;
+  const PatchDesc* patches;        // connection patches (stencil points beyond a
+  int npatch;                      // patched face are read from the donor)
 };
 
 // RK stage kinds: the combine each launch applies

@@ -14,7 +14,9 @@
 //     face the point is the mirror vertex (vertex l for ghost -l) with the
 //     normal velocity negated (slip wall, symmetry) or every velocity
 //     component negated (no-slip wall), each crossing flipping its
-//     components again;
+//     components again; across a connection patch it is the donor vertex
+//     the patch's index map gives, its velocity rotated into this block's
+//     axes;
 //   - face between a and b = a + e_d: means of the two vertices, normal
 //     derivative (phi_b - phi_a) / dr_d, tangential derivatives the mean of
 //     the central differences at a and b, /(4 dr_t);
@@ -70,58 +72,111 @@ __device__ __forceinline__ int vmap(int x, int n, int lo_bc, int hi_bc) {
 }
 
 // A stencil point resolved to the vertex it reads: state array, element
-// offset of the vertex, component stride.
+// offset of the vertex, component stride, and the velocity's frame: the
+// launch block's velocity component r is sign(r) x the read vertex's
+// component axis(r) (axis(r) in bits 2r..2r+1 of `axes`, sign(r) negative
+// where bit r of `neg` is set) -- crossed patches permute and reverse the
+// axes, walls negate components.
 struct VPoint {
   const double* q;
   int64_t c, cs;
-  int flip;          // bit d: velocity component d negated (wall crossings)
+  int axes, neg;
 };
+constexpr int kAxesIdentity = 0 | (1 << 2) | (2 << 4);
 
 __device__ __forceinline__ int64_t vidx(const BlockGeom& g, int i, int j, int k) {
   return (int64_t)(k + g.oz) * g.pxy + (int64_t)(j + g.oy) * g.px + (i + g.ox);
 }
 
-// The vertex stencil point (x, y, z) of the launch's block reads.
+__device__ __forceinline__ int ext(const BlockGeom& g, int d) { return d == 0 ? g.nx : (d == 1 ? g.ny : g.nz); }
+
+// The vertex stencil point (x, y, z) of the launch's block reads.  Per
+// direction out of range: a connected face continues in the neighbour (the
+// directions are taken again there), a wall mirrors, a periodic face wraps,
+// any other face clamps; a connection-patch face is crossed last, through
+// the patch covering the point's (clamped) position on the face, whose
+// linear index map d = dmin + S (r - rmin) carries the whole point into the
+// donor.
 template <int DIM>
 __device__ __forceinline__ VPoint resolve(const ViscArgs& a, int x, int y, int z) {
   const BlockGeom& g = a.g;
   if (!a.tab) {
     const int mi = vmap(x, g.nx, g.bc[0], g.bc[1]), mj = vmap(y, g.ny, g.bc[2], g.bc[3]);
     const int mk = DIM == 3 ? vmap(z, g.nz, g.bc[4], g.bc[5]) : 0;
-    return VPoint{a.q, vidx(g, mi, mj, mk), g.cs, 0};
+    return VPoint{a.q, vidx(g, mi, mj, mk), g.cs, kAxesIdentity, 0};
   }
-  int b = a.self, flip = 0;
+  int b = a.self, neg = 0;
   int c[3] = {x, y, DIM == 3 ? z : 0};
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
+  int axis[3] = {0, 1, 2};          // launch-block velocity component r = current block's axis[r]
+  for (int hop = 0; hop <= DIM; ++hop) {
     const BlockDesc& B = a.tab[b];
-    const int n = d == 0 ? B.g.nx : (d == 1 ? B.g.ny : B.g.nz);
-    if (c[d] >= 0 && c[d] < n) continue;
-    const int f = 2 * d + (c[d] < 0 ? 0 : 1);
-    if (B.g.bc[f] == BC_CONN) {
-      // shared interface vertex: vertex n-1 of the low block is vertex 0 of
-      // the high one; abutting blocks: one spacing apart
-      const int nb = B.nbr[f], sh = B.nbr_shared[f];
-      const BlockGeom& gn = a.tab[nb].g;
-      const int nn = d == 0 ? gn.nx : (d == 1 ? gn.ny : gn.nz);
-      c[d] = c[d] < 0 ? c[d] + nn - sh : c[d] - n + sh;
-      b = nb;
-    } else if (B.wall[f] != WALL_NONE) {
-      c[d] = c[d] < 0 ? -c[d] : 2 * (n - 1) - c[d];
-      flip ^= B.wall[f] == WALL_NOSLIP ? (1 << DIM) - 1 : 1 << d;
-    } else {
-      c[d] = vmap(c[d], n, B.g.bc[2 * d], B.g.bc[2 * d + 1]);
+    int moved = 0, patch_dir = -1;
+    for (int d = 0; d < DIM && !moved; ++d) {
+      const int n = ext(B.g, d);
+      if (c[d] >= 0 && c[d] < n) continue;
+      const int f = 2 * d + (c[d] < 0 ? 0 : 1);
+      if (B.g.bc[f] == BC_CONN) {
+        // shared interface vertex: vertex n-1 of the low block is vertex 0 of
+        // the high one; abutting blocks: one spacing apart
+        const int nb = B.nbr[f], sh = B.nbr_shared[f];
+        const int nn = ext(a.tab[nb].g, d);
+        c[d] = c[d] < 0 ? c[d] + nn - sh : c[d] - n + sh;
+        b = nb;
+        moved = 1;
+      } else if (B.wall[f] != WALL_NONE) {
+        c[d] = c[d] < 0 ? -c[d] : 2 * (n - 1) - c[d];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          if (B.wall[f] == WALL_NOSLIP ? axis[r] < DIM : axis[r] == d) neg ^= 1 << r;
+      } else if (B.g.bc[f] == BC_PATCH) {
+        if (patch_dir < 0) patch_dir = d;
+      } else {
+        c[d] = vmap(c[d], n, B.g.bc[2 * d], B.g.bc[2 * d + 1]);
+      }
     }
+    if (moved) continue;
+    if (patch_dir < 0) break;
+    // cross the patch face of direction patch_dir
+    const int d = patch_dir, side = c[d] < 0 ? 0 : 1;
+    int pi = -1;
+    for (int e = 0; e < a.npatch && pi < 0; ++e) {
+      const PatchDesc& P = a.patches[e];
+      if (P.recv != b || P.dn != d || P.side != side) continue;
+      const int ca = min(max(c[P.ta], 0), ext(B.g, P.ta) - 1);
+      const int cb = min(max(c[P.tb], 0), ext(B.g, P.tb) - 1);
+      if (ca >= P.rmin[P.ta] && ca < P.rmin[P.ta] + P.na && cb >= P.rmin[P.tb] && cb < P.rmin[P.tb] + P.nb)
+        pi = e;
+    }
+    if (pi < 0) break;                // (the step checks that patches tile their faces)
+    const PatchDesc& P = a.patches[pi];
+    int dcoord[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      dcoord[P.map_dir[e]] = (int)P.dmin[P.map_dir[e]] + P.map_sign[e] * (c[e] - (int)P.rmin[e]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      // the current block's component axis[r] is map_sign x the donor's map_dir
+      if (P.map_sign[axis[r]] < 0) neg ^= 1 << r;
+      axis[r] = P.map_dir[axis[r]];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) c[e] = dcoord[e];
+    b = P.donor;
   }
   const BlockDesc& B = a.tab[b];
-  return VPoint{B.q[a.in_buf], vidx(B.g, c[0], c[1], c[2]), B.g.cs, flip};
+  return VPoint{B.q[a.in_buf], vidx(B.g, c[0], c[1], c[2]), B.g.cs,
+                axis[0] | (axis[1] << 2) | (axis[2] << 4), neg};
 }
 
-// A wall crossing's velocity signs (VPoint::flip)
-__device__ __forceinline__ void apply_flip(int flip, double out[5]) {
-  if (flip & 1) out[0] = -out[0];
-  if (flip & 2) out[1] = -out[1];
-  if (flip & 4) out[2] = -out[2];
+// The launch block's velocity from the read vertex's (out[0..2]: u, v, w).
+__device__ __forceinline__ void apply_frame(const VPoint& pt, double out[5]) {
+  if (pt.axes == kAxesIdentity && pt.neg == 0) return;
+  const double v3[3] = {out[0], out[1], out[2]};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double v = v3[(pt.axes >> (2 * r)) & 3];
+    out[r] = (pt.neg >> r) & 1 ? -v : v;
+  }
 }
 
 // u, v, w, T, mu of the conserved state at a resolved vertex
@@ -255,7 +310,7 @@ __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
       double v[5];
       const visc::VPoint pt = visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k);
       visc::prims<5>(pt, V, R, v);
-      visc::apply_flip(pt.flip, v);
+      visc::apply_frame(pt, v);
 #pragma unroll
       for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
     }
@@ -320,7 +375,7 @@ __global__ __launch_bounds__(TX * TY) void viscous2d_kernel(ViscArgs a) {
     double v[5];
     const visc::VPoint pt = visc::resolve<2>(a, i0 - 1 + e % W, j0 - 1 + e / W, 0);
     visc::prims<4>(pt, V, R, v);
-    visc::apply_flip(pt.flip, v);
+    visc::apply_frame(pt, v);
 #pragma unroll
     for (int m = 0; m < 5; ++m) S[m][e] = v[m];
   }

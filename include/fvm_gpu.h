@@ -189,7 +189,7 @@ int fvm_block_connect(fvm_block *a, int face, fvm_block *b, int shared);
  * layer l of `recv` receives the donor vertex l planes inside its face.  The
  * patches on a face must tile it exactly; a face may mix donors.  The ghosts
  * are refilled on the device after every stage.  2-D blocks: donor_type[2]
- * = 3 and k ranges 1..1.  Not with viscous terms. */
+ * = 3 and k ranges 1..1. */
 int fvm_conn_add(fvm_block *recv, const int64_t range_min[3], const int64_t range_max[3],
                  fvm_block *donor, const int64_t donor_range_min[3],
                  const int64_t donor_range_max[3], const int donor_type[3]);
@@ -314,8 +314,9 @@ int fvm_step_noalloc(fvm_ctx *ctx, int nsteps, double dt, double cfl, double *dt
  * a stencil point beyond a connected face is the neighbour's vertex (edge and
  * corner points through the neighbour's neighbour), so a cut grid gives the
  * results of the grid it was cut from.  Wall and symmetry faces mirror the
- * stencil (fvm_block_set_bc).  Not with connection patches (fvm_conn_add) or
- * inflow or farfield faces. */
+ * stencil (fvm_block_set_bc); across a connection patch (fvm_conn_add) the
+ * point is the donor vertex the patch's index map gives, its velocity
+ * rotated into the receiver's axes.  Not with inflow or farfield faces. */
 int fvm_viscous_set(fvm_ctx *ctx, double mu_ref, double t_ref, double sutherland_s,
                     double prandtl, double r_gas);
 
