@@ -31,6 +31,7 @@ module fvm_gpu
   integer(c_int), parameter, public :: FVM_BC_OUTFLOW_SUPERSONIC = 22, FVM_BC_WALL = 30
   integer(c_int), parameter, public :: FVM_BC_WALL_INVISCID = 31, FVM_BC_FARFIELD = 40
   integer(c_int), parameter, public :: FVM_BC_EXTRAPOLATE = 41, FVM_BC_SYMMETRY = 50
+  integer(c_int), parameter, public :: FVM_BC_WALL_VISCOUS_ISOTHERMAL = 32, FVM_BC_WALL_VISCOUS = 33
   integer(c_int), parameter, public :: FVM_NUMERICS_EXACT = 0, FVM_NUMERICS_FAST = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PER_BLOCK = 0, FVM_LAUNCH_BATCHED = 1
   integer(c_int), parameter, public :: FVM_LAUNCH_PERSISTENT = 2

@@ -115,3 +115,19 @@ def test_fortran_request_layout(tmp_path):
     assert int(out["size"]) == ctypes.sizeof(fvmgpu.Request)
     for f in fields:
         assert int(out[f]) == getattr(fvmgpu.Request, f).offset, f
+
+
+def test_bc_codes_agree_across_header_fortran_python_and_oracle():
+    """Every FVM_BC_* code of the header has the same value in the Fortran
+    module, the Python package and, for the kinds the oracle applies, the
+    oracle (the reference's codes, src/defs/type_defs_bc.f90:11-30)."""
+    import _oracle as O
+    hdr = dict((k, int(v)) for k, v in re.findall(r"#define (FVM_BC_\w+)\s+(\d+)", open(HEADER).read()))
+    fmod = open(os.path.join(ROOT, "fvm-3d-f90_amd", "fortran", "fvm_gpu_mod.f90")).read()
+    fort = dict((k, int(v)) for k, v in re.findall(r"(FVM_BC_\w+)\s*=\s*(\d+)", fmod))
+    assert len(hdr) >= 15
+    assert hdr == fort
+    for name, val in hdr.items():
+        assert getattr(fvmgpu, name[4:]) == val, name
+    assert hdr["FVM_BC_WALL"] == O.BC_WALL and hdr["FVM_BC_WALL_VISCOUS"] == O.BC_NOSLIP
+    assert hdr["FVM_BC_SYMMETRY"] == O.BC_SYMMETRY and hdr["FVM_BC_PERIODIC"] == O.BC_PERIODIC
