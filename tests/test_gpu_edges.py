@@ -74,8 +74,9 @@ def test_invalid_requests_are_errors():
         c = ctx.block((8, 8, 8), (0.1, 0.1, 0.1))
         a.connect(1, c)
         ctx.viscous(0.01)                                  # viscous on connected blocks: fine
+        c.conn_add((8, 1, 1), (8, 8, 8), a, (1, 1, 1), (1, 8, 8), (1, 2, 3))       # and on patches
         with pytest.raises(fvmgpu.FvmError):
-            c.conn_add((8, 1, 1), (8, 8, 8), a, (1, 1, 1), (1, 8, 8), (1, 2, 3))   # a patch: not
+            b.set_bc_state(0, fvmgpu.BC_FARFIELD, (1.0, 0.5, 0.0, 0.0, 1.0))   # farfield + viscous: not
     with fvmgpu.Context() as ctx:
         ctx.scheme(recon=O.MUSCL, limiter=O.LIM_MINMOD, flux=O.ROE, rk=O.RK_SSP3)
         b2 = ctx.block((8, 8, 1), (0.1, 0.1, 1.0))
