@@ -893,9 +893,9 @@ static void residual_norms(int64_t nreal, const double *rhs, double *res)
 }
 
 static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
-                      double t_end, int nsteps, const double *p_real, double *q_out,
-                      double *rhs_out, double *res_out, double *dt_out, int *steps_out,
-                      double *t_out);
+                      double t_end, int nsteps, const double *p_real, const double *q_real,
+                      double *q_out, double *rhs_out, double *res_out, double *dt_out,
+                      int *steps_out, double *t_out);
 
 /* No-slip walls after a stage's combine: a vertex on such a face keeps rho,
  * loses its momentum, and its total energy loses the momentum's kinetic
@@ -929,22 +929,30 @@ int orc_solve(const int64_t n[3], const orc_config *c, double dt_fix, double cfl
               int nsteps, const double *p_real, double *q_out, double *rhs_out,
               double *res_out, double *dt_out)
 {
-    return solve_core(n, c, dt_fix, cfl, INFINITY, nsteps, p_real, q_out, rhs_out, res_out,
-                      dt_out, NULL, NULL);
+    return solve_core(n, c, dt_fix, cfl, INFINITY, nsteps, p_real, NULL, q_out, rhs_out,
+                      res_out, dt_out, NULL, NULL);
+}
+
+int orc_solve_q(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
+                int nsteps, const double *q_real, double *q_out, double *rhs_out,
+                double *res_out, double *dt_out)
+{
+    return solve_core(n, c, dt_fix, cfl, INFINITY, nsteps, NULL, q_real, q_out, rhs_out,
+                      res_out, dt_out, NULL, NULL);
 }
 
 int orc_solve_to(const int64_t n[3], const orc_config *c, double cfl, double t_end,
                  int max_steps, const double *p_real, double *q_out, int *steps_out,
                  double *t_out)
 {
-    return solve_core(n, c, 0.0, cfl, t_end, max_steps, p_real, q_out, NULL, NULL, NULL,
-                      steps_out, t_out);
+    return solve_core(n, c, 0.0, cfl, t_end, max_steps, p_real, NULL, q_out, NULL, NULL,
+                      NULL, steps_out, t_out);
 }
 
 static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, double cfl,
-                      double t_end, int nsteps, const double *p_real, double *q_out,
-                      double *rhs_out, double *res_out, double *dt_out, int *steps_out,
-                      double *t_out)
+                      double t_end, int nsteps, const double *p_real, const double *q_real,
+                      double *q_out, double *rhs_out, double *res_out, double *dt_out,
+                      int *steps_out, double *t_out)
 {
     blk_t b;
     for (int d = 0; d < 3; ++d) b.n[d] = n[d];
@@ -962,16 +970,22 @@ static int solve_core(const int64_t n[3], const orc_config *c, double dt_fix, do
     double *rhs = malloc(sizeof(double) * NEQ * nreal);
     if (!b.q || !b.p || !q0 || !qm || !acc || !rhs) return -1;
 
-    /* initial state: p on real vertices, q = p -> q, optional p = q -> p */
-    for (int64_t k = 0; k < n[2]; ++k)
-        for (int64_t j = 0; j < n[1]; ++j)
-            for (int64_t i = 0; i < n[0]; ++i) {
-                int64_t gi = gidx(&b, i + NG, j + NG, k + b.gz);
-                const double *src = p_real + 5 * (i + n[0] * (j + n[1] * k));
-                memcpy(b.p + 5 * gi, src, 5 * sizeof(double));
-                p_to_q_point(c->gamma, b.p + 5 * gi, b.q + gi, cs);
-            }
-    if (c->p_from_q) update_primitives(&b, c->gamma);
+    /* initial state: p on real vertices, q = p -> q, optional p = q -> p;
+     * or a conserved state, p = q -> p (what every stage ends with) */
+    if (q_real) {
+        scatter_real(&b, q_real);
+        update_primitives(&b, c->gamma);
+    } else {
+        for (int64_t k = 0; k < n[2]; ++k)
+            for (int64_t j = 0; j < n[1]; ++j)
+                for (int64_t i = 0; i < n[0]; ++i) {
+                    int64_t gi = gidx(&b, i + NG, j + NG, k + b.gz);
+                    const double *src = p_real + 5 * (i + n[0] * (j + n[1] * k));
+                    memcpy(b.p + 5 * gi, src, 5 * sizeof(double));
+                    p_to_q_point(c->gamma, b.p + 5 * gi, b.q + gi, cs);
+                }
+        if (c->p_from_q) update_primitives(&b, c->gamma);
+    }
     fill_ghosts(&b, c->bc, c->dim);
 
     int nstage = c->rk == ORC_RK4 ? 4 : 3;

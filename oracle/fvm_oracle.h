@@ -101,6 +101,15 @@ int orc_solve(const int64_t n[3], const orc_config *cfg, double dt, double cfl,
               int nsteps, const double *p_real, double *q_out, double *rhs_out,
               double *res_out, double *dt_out);
 
+/* orc_solve from a conserved state instead of primitives: q_real
+ * component-major (5, n2, n1, n0) real vertices; primitives are derived
+ * from it (the p_from_q path).  A step loop's state is its real conserved
+ * vertices, so nsteps from the q_out of an earlier call continue that run
+ * bit for bit (a long run in chunks). */
+int orc_solve_q(const int64_t n[3], const orc_config *cfg, double dt, double cfl,
+                int nsteps, const double *q_real, double *q_out, double *rhs_out,
+                double *res_out, double *dt_out);
+
 /* CFL steps to an end time, as the device time slot of a served solve takes
  * them (no reference counterpart: the reference's CFL time step is
  * unimplemented, src/solve/class_time_discret.f90:94-107): per step

@@ -60,6 +60,8 @@ def oracle():
         lib.orc_flux_ausmplus.argtypes = [i64, ci, cd, P, P, P, P]
         lib.orc_solve.argtypes = [P, ctypes.POINTER(OrcConfig), cd, cd, ci, P, P, P, P, P]
         lib.orc_solve.restype = ci
+        lib.orc_solve_q.argtypes = [P, ctypes.POINTER(OrcConfig), cd, cd, ci, P, P, P, P, P]
+        lib.orc_solve_q.restype = ci
         lib.orc_solve_to.argtypes = [P, ctypes.POINTER(OrcConfig), cd, cd, ci, P, P,
                                      ctypes.POINTER(ci), ctypes.POINTER(cd)]
         lib.orc_solve_to.restype = ci
@@ -170,6 +172,30 @@ def oracle_solve(p_real, dim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=
                             _ptr(q), _ptr(rhs), _ptr(res), _ptr(dts))
     assert rc == 0
     return {"q": q, "rhs": rhs, "res": res[:nsteps], "dt": dts[:nsteps]}
+
+
+def oracle_solve_q(q_real, dim, dr, recon=MUSCL, limiter=LIM_MINMOD, flux=ROE, rk=RK_SSP3,
+                   kappa=0.33333, entfix=0.2, gamma=1.4, dt=2e-4, cfl=0.0, nsteps=1,
+                   bc=(BC_EXTRAP,) * 6, weno_textbook=0):
+    """oracle_solve from a conserved state q_real (5, nk, nj, ni) on real
+    vertices (primitives derived from it): nsteps from an earlier run's final
+    q continue that run bit for bit.  Returns dict as oracle_solve's, plus
+    q_real, the final real vertices."""
+    q_real = np.ascontiguousarray(q_real, dtype=np.float64)
+    _, nk, nj, ni = q_real.shape
+    gz = 3 if dim == 3 else 0
+    n = np.array([ni, nj, nk], dtype=np.int64)
+    cfg = OrcConfig(dim, recon, limiter, flux, rk, weno_textbook, 1, (ci * 6)(*bc), kappa,
+                    entfix, gamma, (cd * 3)(*dr), 0.0, 0.0, 0.0, 0.0, 0.0)
+    q = np.zeros((5, nk + 2 * gz, nj + 6, ni + 6))
+    rhs = np.zeros((5, nk, nj, ni))
+    res = np.zeros((max(nsteps, 1), 6))
+    dts = np.zeros(max(nsteps, 1))
+    rc = oracle().orc_solve_q(_ptr(n), ctypes.byref(cfg), dt, cfl, nsteps, _ptr(q_real),
+                              _ptr(q), _ptr(rhs), _ptr(res), _ptr(dts))
+    assert rc == 0
+    return {"q": q, "q_real": q[:, gz:gz + nk, 3:3 + nj, 3:3 + ni].copy(), "rhs": rhs,
+            "res": res[:nsteps], "dt": dts[:nsteps]}
 
 
 def oracle_solve_to(p_real, dim, dr, cfl, t_end, max_steps=100000, recon=MUSCL,

@@ -2,6 +2,8 @@
 against the oracle and the exact family.  Tolerances are the stated ones:
 relative L-inf per conserved component (normalised by max|q| with a floor of
 1) at most 1e-12 after one step and at most 1e-9 after 1000 steps."""
+import os
+
 import numpy as np
 import pytest
 
@@ -136,3 +138,47 @@ def test_exact_1000_steps_equals_oracle():
     assert dt_last == r["dt"][-1]
     assert np.array_equal(qg, qo)
     np.testing.assert_allclose(norms, r["res"][-1], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.skipif(not os.environ.get("FVM_LONG_TESTS"),
+                    reason="runs the host oracle over 1000 steps of a 128^3 grid (about 6 minutes "
+                           "on 16 threads); FVM_LONG_TESTS=1 runs it")
+def test_exact_1000_steps_equals_oracle_vortex128():
+    """The benchmark workload itself, the full run: the exact family on the
+    128^3 vortex (MUSCL-vanAlbada + Roe, RK4, CFL 0.8) equals the oracle bit
+    for bit after every 100 of its 1000 steps, CFL dt included.  The oracle
+    runs in 25-step chunks (orc_solve_q continues a run bit for bit) so the
+    run reports its progress."""
+    p, h, _ = cases.isentropic_vortex(128, seed=0)
+    q0 = cases.to_conserved(p)
+    marks = [100 * m for m in range(1, 11)]
+    print("case exact vs oracle 1000 steps vortex128", flush=True)
+    gpu, gpu_dt = [], []
+    with fvmgpu.Context() as ctx:
+        ctx.scheme(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4)
+        ctx.numerics(fvmgpu.NUMERICS_EXACT)
+        b = ctx.block((128, 128, 128), (h, h, h))
+        b.set_periodic()
+        b.upload(q0)
+        done = 0
+        for m in marks:
+            gpu_dt.append(ctx.step(m - done, cfl=0.8))
+            done = m
+            gpu.append(b.download())
+    O.set_threads(min(16, os.cpu_count() or 1))
+    kw = dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=O.RK4, cfl=0.8, dt=0.0,
+              bc=(O.BC_PERIODIC,) * 6)
+    q, chunk = None, 25
+    for mark, qg, dtg in zip(marks, gpu, gpu_dt):
+        for s in range(mark - 100, mark, chunk):
+            if q is None:
+                r = O.oracle_solve(p, 3, (h, h, h), nsteps=chunk, p_from_q=1, **kw)
+                q = r["q"][:, 3:-3, 3:-3, 3:-3].copy()
+            else:
+                r = O.oracle_solve_q(q, 3, (h, h, h), nsteps=chunk, **kw)
+                q = r["q_real"]
+            print(f"oracle step {s + chunk} of 1000", flush=True)
+        assert np.array_equal(qg, q), f"step {mark}: rel linf {rel_linf(qg, q)}"
+        assert dtg == r["dt"][-1]
+        print(f"step {mark}: bit-identical, dt {dtg!r}", flush=True)

@@ -285,3 +285,23 @@ def test_vortex_second_order_convergence():
         errs.append(np.abs(q[0] - exact[0]).mean())
     rate = math.log2(errs[0] / errs[1])
     assert rate > 1.6, (errs, rate)
+
+
+@pytest.mark.parametrize("rk,cfl", [(O.RK4, 0.8), (O.RK_SSP3, 0.5)])
+def test_solve_from_conserved_state_continues_a_run(rk, cfl):
+    """orc_solve_q from an earlier run's final conserved state continues that
+    run bit for bit (chunked long runs), CFL dt included; the state is the
+    run's whole memory, so 3 + 4 + 3 steps equal 10 steps."""
+    p, h, _ = cases.isentropic_vortex(12, seed=2)
+    kw = dict(recon=O.MUSCL, limiter=O.LIM_VANALBADA, flux=O.ROE, rk=rk, cfl=cfl, dt=0.0,
+              bc=(O.BC_PERIODIC,) * 6)
+    whole = O.oracle_solve(p, 3, (h, h, h), nsteps=10, p_from_q=1, **kw)
+    first = O.oracle_solve(p, 3, (h, h, h), nsteps=3, p_from_q=1, **kw)
+    q, dts = first["q"][:, 3:-3, 3:-3, 3:-3], list(first["dt"])
+    for n in (4, 3):
+        r = O.oracle_solve_q(q, 3, (h, h, h), nsteps=n, **kw)
+        q = r["q_real"]
+        dts += list(r["dt"])
+    assert np.array_equal(q, whole["q"][:, 3:-3, 3:-3, 3:-3])
+    assert dts == list(whole["dt"])
+    assert np.array_equal(r["res"][-1], whole["res"][-1])
