@@ -797,6 +797,15 @@ def extras(args):
     r = argparse.Namespace(**vars(ns))
     r.steps, r.warmup = None, 5
     wl["riemann400"] = summary(bench_riemann(r))
+    if not args.no_cpu:
+        # the reference's own Fortran scheme modules on this workload, which
+        # is the one the reference itself can run (its case/Riemann-2D setup)
+        from fvmgpu import cases
+        p, dx = cases.riemann2d(400, seed=0)
+        ref = cpu_reference(p, cases.to_conserved(p), dx, 0.5, 10, 400.0 * 400 * 3 * 10)
+        if ref is not None:
+            ref["cpu_model"] = cpu_model()
+            wl["riemann400"]["cpu_baseline"] = ref
     b = argparse.Namespace(**vars(ns))
     b.steps, b.warmup = 20, 2
     ln = bench_blob27(b)
