@@ -272,14 +272,11 @@ __device__ __forceinline__ void muscl_vertex_all(const double (*V)[NC], double k
 }
 
 // ---- WENO-JS, src/scheme/scheme_WENO.f90:87-121 ------------------------------
-// Value at the right face of u2.  textbook = 0 keeps the reference's
-// IS_1 = 13/12 (u1-2u2+u3)^2 + 1/4 (u1 + u3)^2 (scheme_WENO.f90:107-108) and
-// operation order; textbook = 1 (this build's variant) forms the indicator
-// terms in an order symmetric under stencil reversal, t = (a + c) - 2 b,
-// s = (3 c + a) - 4 b with the weight-3 point first, so a mirrored stencil's
-// indicators are the forward ones reversed, bit for bit (weno_js_pair_exact).
+// The reference's variant: value at the right face of u2, with
+// IS_1 = 13/12 (u1-2u2+u3)^2 + 1/4 (u1 + u3)^2 (scheme_WENO.f90:107-108), in
+// its operation order.
 __device__ __forceinline__ double weno_js(double u0, double u1, double u2, double u3,
-                                          double u4, int textbook) {
+                                          double u4) {
   const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
   const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
   const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
@@ -288,22 +285,12 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
   const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
   // a - 2b and a - 4b as FMAs: products by 2 and 4 are exact, so each FMA
   // rounds once, as the separate subtraction does
-  double t0, s0, t1, s1, t2, s2;
-  if (textbook) {
-    t0 = fma(-2.0, u1, u0 + u2);
-    s0 = fma(-4.0, u1, 3.0 * u2 + u0);
-    t1 = fma(-2.0, u2, u1 + u3);
-    s1 = u1 - u3;
-    t2 = fma(-2.0, u3, u2 + u4);
-    s2 = fma(-4.0, u3, 3.0 * u2 + u4);
-  } else {
-    t0 = fma(-2.0, u1, u0) + u2;
-    s0 = fma(-4.0, u1, u0) + 3.0 * u2;
-    t1 = fma(-2.0, u2, u1) + u3;
-    s1 = u1 + u3;
-    t2 = fma(-2.0, u3, u2) + u4;
-    s2 = fma(-4.0, u3, 3.0 * u2) + u4;
-  }
+  const double t0 = fma(-2.0, u1, u0) + u2;
+  const double s0 = fma(-4.0, u1, u0) + 3.0 * u2;
+  const double t1 = fma(-2.0, u2, u1) + u3;
+  const double s1 = u1 + u3;
+  const double t2 = fma(-2.0, u3, u2) + u4;
+  const double s2 = fma(-4.0, u3, 3.0 * u2) + u4;
   const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
   const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
   const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
@@ -321,48 +308,46 @@ __device__ __forceinline__ double weno_js(double u0, double u1, double u2, doubl
   return w0 * q0 + w1 * q1 + w2 * q2;
 }
 
-// Textbook WENO5-JS at both faces of u2, exact family: the value at the right
-// face (stencil u0..u4) and at the left face (the mirrored stencil u4..u0),
-// each bit for bit weno_js(.., 1) of its stencil.  The mirrored stencil's
-// indicators are e2, e1, e0 (the symmetric order makes them identical), and
-// its middle weight c_1 / e1^2 is the forward one, so both are formed once.
+// Textbook WENO5-JS (Jiang & Shu, JCP 126 (1996); IS_1 with (u1 - u3)^2) at
+// both faces of u2, exact family: the value at the right face (stencil
+// u0..u4) and at the left face (the mirrored stencil u4..u0).  The indicator
+// terms are formed in an order symmetric under stencil reversal,
+// t = (a + c) - 2 b and s = (3 c + a) - 4 b (the weight-3 point first), so the
+// mirrored stencil's indicators are the forward ones reversed, bit for bit,
+// and are formed once.  The weights alpha_k = c_k / e_k^2,
+// w_k = alpha_k / sum alpha, are taken in product form, scaled by E0 E1 E2
+// (E_k = e_k^2): w_k = b_k / (b0 + b1 + b2), b0 = c_0 E1 E2, b1 = c_1 E0 E2,
+// b2 = c_2 E0 E1, one IEEE division per face; the mirrored face's b are the
+// same products in reverse order (b1 is shared).  Every e_k is at least
+// 1e-6, and the products stay normal doubles while the e_k stay below ~1e76.
+// Products by 2, 4 and 1/4 are exact; the other sums of products are FMAs.
 __device__ __forceinline__ void weno_js_pair_exact(double u0, double u1, double u2, double u3,
                                                    double u4, double& right, double& left) {
   const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
   const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
-  const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+  const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0;
   const double t0 = fma(-2.0, u1, u0 + u2), s0 = fma(-4.0, u1, 3.0 * u2 + u0);
   const double t1 = fma(-2.0, u2, u1 + u3), s1 = u1 - u3;
   const double t2 = fma(-2.0, u3, u2 + u4), s2 = fma(-4.0, u3, 3.0 * u2 + u4);
-  const double e0 = c1312 * (t0 * t0) + c14 * (s0 * s0) + eps;
-  const double e1 = c1312 * (t1 * t1) + c14 * (s1 * s1) + eps;
-  const double e2 = c1312 * (t2 * t2) + c14 * (s2 * s2) + eps;
-  // the five weights: each divisor's reciprocal serves its two quotients at once
-  double a0, a2, b0, b2;
+  const double e0 = fma(c1312, t0 * t0, fma(0.25 * s0, s0, eps));
+  const double e1 = fma(c1312, t1 * t1, fma(0.25 * s1, s1, eps));
+  const double e2 = fma(c1312, t2 * t2, fma(0.25 * s2, s2, eps));
+  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+  const double P12 = E1 * E2, P01 = E0 * E1;
+  const double b1 = c_1 * (E0 * E2);
   {
-    const Recip r = erecip(e0 * e0);
-    a0 = ediv(c_0, r);
-    b2 = ediv(c_2, r);
+    const double q0 = fma(c116, u2, fma(-c76, u1, c13 * u0));
+    const double q1 = fma(c13, u3, fma(c56, u2, -(c16 * u1)));
+    const double q2 = fma(-c16, u4, fma(c56, u3, c13 * u2));
+    const double b0 = c_0 * P12, b2 = c_2 * P01;
+    right = ediv(fma(b2, q2, fma(b1, q1, b0 * q0)), (b0 + b1) + b2);
   }
   {
-    const Recip r = erecip(e2 * e2);
-    a2 = ediv(c_2, r);
-    b0 = ediv(c_0, r);
-  }
-  const double a1 = ediv(c_1, e1 * e1);
-  {
-    const double q0 = c13 * u0 - c76 * u1 + c116 * u2;
-    const double q1 = -(c16 * u1) + c56 * u2 + c13 * u3;
-    const double q2 = c13 * u2 + c56 * u3 - c16 * u4;
-    const Recip ras = erecip(a0 + a1 + a2);
-    right = ediv(a0, ras) * q0 + ediv(a1, ras) * q1 + ediv(a2, ras) * q2;
-  }
-  {
-    const double q0 = c13 * u4 - c76 * u3 + c116 * u2;
-    const double q1 = -(c16 * u3) + c56 * u2 + c13 * u1;
-    const double q2 = c13 * u2 + c56 * u1 - c16 * u0;
-    const Recip rbs = erecip(b0 + a1 + b2);
-    left = ediv(b0, rbs) * q0 + ediv(a1, rbs) * q1 + ediv(b2, rbs) * q2;
+    const double q0 = fma(c116, u2, fma(-c76, u3, c13 * u4));
+    const double q1 = fma(c13, u1, fma(c56, u2, -(c16 * u3)));
+    const double q2 = fma(-c16, u0, fma(c56, u1, c13 * u2));
+    const double b0 = c_0 * P01, b2 = c_2 * P12;
+    left = ediv(fma(b2, q2, fma(b1, q1, b0 * q0)), (b0 + b1) + b2);
   }
 }
 

@@ -262,19 +262,13 @@ struct ReconWeno {     // scheme_WENO.f90:28-121, then q -> p at the face
       cons_to_prim<NC>(qR, R, P.gamma);
       return;
     }
+    // reference: the right state of a face is the next face's left state
+    // (scheme_WENO.f90:54-56)
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      qL[c] = weno_js(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c], P.weno_textbook);
-      // reference: the right state of a face is the next face's left state
-      // (scheme_WENO.f90:54-56); textbook: mirror image of the stencil
-      qR[c] = P.weno_textbook ? weno_js(V[4][c], V[3][c], V[2][c], V[1][c], V[0][c], 1) : qL[c];
-    }
+    for (int c = 0; c < NC; ++c) qL[c] = weno_js(V[0][c], V[1][c], V[2][c], V[3][c], V[4][c]);
     cons_to_prim<NC>(qL, L, P.gamma);
-    if (P.weno_textbook) cons_to_prim<NC>(qR, R, P.gamma);
-    else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) R[c] = L[c];
-    }
+    for (int c = 0; c < NC; ++c) R[c] = L[c];
   }
 };
 template <int FLUX, int NC, int D>

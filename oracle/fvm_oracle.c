@@ -110,15 +110,10 @@ void orc_recon_muscl(int64_t nface, int64_t dghost, double kappa, int limiter,
     }
 }
 
-/* Jiang-Shu weights on a 5-point stencil, value at the right face of u[2];
- * scheme_WENO.f90:87-121.  textbook = 0 keeps the reference's IS_1 term
- * (u2 + u4)**2 (scheme_WENO.f90:107-108) and its operation order;
- * textbook = 1 uses (u2 - u4)**2 and forms the smoothness terms in an order
- * symmetric under stencil reversal, t = (a + c) - 2 b and s = (3 c + a) - 4 b
- * (the weight-3 point first), so the mirrored stencil's indicators are the
- * forward ones reversed, bit for bit (the GPU shares them between a vertex's
- * two faces). */
-static double weno_js(const double u[5], int textbook)
+/* Jiang-Shu weights on a 5-point stencil, value at the right face of u[2]:
+ * the reference's variant, scheme_WENO.f90:87-121, with its IS_1 term
+ * (u2 + u4)**2 (scheme_WENO.f90:107-108) and its operation order. */
+static double weno_js_ref(const double u[5])
 {
     const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
     const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
@@ -127,22 +122,12 @@ static double weno_js(const double u[5], int textbook)
     double q0 = c13 * u[0] - c76 * u[1] + c116 * u[2];
     double q1 = -(c16 * u[1]) + c56 * u[2] + c13 * u[3];
     double q2 = c13 * u[2] + c56 * u[3] - c16 * u[4];
-    double t0, s0, t1, s1, t2, s2;
-    if (textbook) {
-        t0 = (u[0] + u[2]) - 2.0 * u[1];
-        s0 = (3.0 * u[2] + u[0]) - 4.0 * u[1];
-        t1 = (u[1] + u[3]) - 2.0 * u[2];
-        s1 = u[1] - u[3];
-        t2 = (u[2] + u[4]) - 2.0 * u[3];
-        s2 = (3.0 * u[2] + u[4]) - 4.0 * u[3];
-    } else {
-        t0 = u[0] - 2.0 * u[1] + u[2];
-        s0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
-        t1 = u[1] - 2.0 * u[2] + u[3];
-        s1 = u[1] + u[3];
-        t2 = u[2] - 2.0 * u[3] + u[4];
-        s2 = 3.0 * u[2] - 4.0 * u[3] + u[4];
-    }
+    double t0 = u[0] - 2.0 * u[1] + u[2];
+    double s0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
+    double t1 = u[1] - 2.0 * u[2] + u[3];
+    double s1 = u[1] + u[3];
+    double t2 = u[2] - 2.0 * u[3] + u[4];
+    double s2 = 3.0 * u[2] - 4.0 * u[3] + u[4];
     double IS_0 = c1312 * (t0 * t0) + c14 * (s0 * s0);
     double IS_1 = c1312 * (t1 * t1) + c14 * (s1 * s1);
     double IS_2 = c1312 * (t2 * t2) + c14 * (s2 * s2);
@@ -155,6 +140,45 @@ static double weno_js(const double u[5], int textbook)
     double omega_1 = alpha_1 / alpha_sum;
     double omega_2 = alpha_2 / alpha_sum;
     return omega_0 * q0 + omega_1 * q1 + omega_2 * q2;
+}
+
+/* Textbook WENO5-JS (Jiang & Shu, JCP 126 (1996)), value at the right face
+ * of u[2]; this build's variant beside the reference's, with
+ * IS_1 = 13/12 (u1 - 2 u2 + u3)^2 + 1/4 (u1 - u3)^2.  The indicator terms are
+ * formed in an order symmetric under stencil reversal, t = (a + c) - 2 b and
+ * s = (3 c + a) - 4 b (the weight-3 point first), so the mirrored stencil's
+ * indicators are the forward ones reversed, bit for bit (the GPU forms them
+ * once for a vertex's two faces).  The weights alpha_k = c_k / e_k^2,
+ * w_k = alpha_k / sum alpha, are taken in product form, scaled by
+ * E_0 E_1 E_2 (E_k = e_k^2): w_k = b_k / (b_0 + b_1 + b_2) with
+ * b_0 = c_0 E_1 E_2, b_1 = c_1 E_0 E_2, b_2 = c_2 E_0 E_1, one division per
+ * face.  Products by 2, 4 and 1/4 are exact; the other sums of products are
+ * single-rounding fma's. */
+static double weno_js_textbook(const double u[5])
+{
+    const double c_0 = 0.1, c_1 = 0.6, c_2 = 0.3, eps = 1.0e-6;
+    const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0;
+    const double c16 = 1.0 / 6.0, c56 = 5.0 / 6.0, c1312 = 13.0 / 12.0;
+    double q0 = fma(c116, u[2], fma(-c76, u[1], c13 * u[0]));
+    double q1 = fma(c13, u[3], fma(c56, u[2], -(c16 * u[1])));
+    double q2 = fma(-c16, u[4], fma(c56, u[3], c13 * u[2]));
+    double t0 = (u[0] + u[2]) - 2.0 * u[1];
+    double s0 = (3.0 * u[2] + u[0]) - 4.0 * u[1];
+    double t1 = (u[1] + u[3]) - 2.0 * u[2];
+    double s1 = u[1] - u[3];
+    double t2 = (u[2] + u[4]) - 2.0 * u[3];
+    double s2 = (3.0 * u[2] + u[4]) - 4.0 * u[3];
+    double e0 = fma(c1312, t0 * t0, fma(0.25 * s0, s0, eps));
+    double e1 = fma(c1312, t1 * t1, fma(0.25 * s1, s1, eps));
+    double e2 = fma(c1312, t2 * t2, fma(0.25 * s2, s2, eps));
+    double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+    double b0 = c_0 * (E1 * E2), b1 = c_1 * (E0 * E2), b2 = c_2 * (E0 * E1);
+    return fma(b2, q2, fma(b1, q1, b0 * q0)) / ((b0 + b1) + b2);
+}
+
+static double weno_js(const double u[5], int textbook)
+{
+    return textbook ? weno_js_textbook(u) : weno_js_ref(u);
 }
 
 /* scheme_WENO.f90:28-72 on conserved variables qg(npg, neq) column-major.

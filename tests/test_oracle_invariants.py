@@ -159,6 +159,50 @@ def test_weno_textbook_right_state_is_mirror():
     np.testing.assert_array_equal(R, Lm[::-1])
 
 
+def _weno_js_published(u):
+    """Jiang & Shu, JCP 126 (1996), eqs. (2.10), (3.2)-(3.4), in its published
+    quotient form: alpha_k = d_k / (eps + IS_k)^2, w_k = alpha_k / sum alpha."""
+    u0, u1, u2, u3, u4 = u
+    q = (u0 / 3 - 7 * u1 / 6 + 11 * u2 / 6,
+         -u1 / 6 + 5 * u2 / 6 + u3 / 3,
+         u2 / 3 + 5 * u3 / 6 - u4 / 6)
+    IS = (13 / 12 * (u0 - 2 * u1 + u2) ** 2 + (u0 - 4 * u1 + 3 * u2) ** 2 / 4,
+          13 / 12 * (u1 - 2 * u2 + u3) ** 2 + (u1 - u3) ** 2 / 4,
+          13 / 12 * (u2 - 2 * u3 + u4) ** 2 + (3 * u2 - 4 * u3 + u4) ** 2 / 4)
+    a = [d / (1e-6 + s) ** 2 for d, s in zip((0.1, 0.6, 0.3), IS)]
+    return sum(ai * qi for ai, qi in zip(a, q)) / sum(a)
+
+
+def test_weno_textbook_matches_published_weights():
+    """The textbook variant's product-form weights (one division per face)
+    are Jiang-Shu's quotient form: random stencils with jumps of every size,
+    so each weight matters somewhere; a weight paired with the wrong
+    indicator product, the reference's (u1 + u3) term or an unsquared
+    indicator each miss by far more than rounding."""
+    rng = np.random.default_rng(11)
+    n = 400
+    base = rng.uniform(0.5, 2.0, n)
+    jumps = rng.choice([0.0, 1e-3, 0.1, 1.0, 10.0], n) * (rng.uniform(size=n) < 0.3)
+    col = base + np.cumsum(jumps)
+    qg = np.repeat(col[:, None], 5, axis=1)
+    nface = n - 5
+    L = O.recon(O.oracle(), O.WENO, qg, nface, 3, -1, textbook=1)[:, 0]
+    R = O.recon(O.oracle(), O.WENO, qg, nface, 3, 1, textbook=1)[:, 0]
+    f = np.arange(1, nface + 1)
+    ig = f + 1                       # vertex left of face f (0-based; 3 ghosts)
+    wantL = np.array([_weno_js_published(col[i - 2:i + 3]) for i in ig])
+    wantR = np.array([_weno_js_published(col[i - 1:i + 4][::-1]) for i in ig])
+    tol = 1e-13 * np.abs(col).max()
+    np.testing.assert_allclose(L, wantL, rtol=0, atol=tol)
+    np.testing.assert_allclose(R, wantR, rtol=0, atol=tol)
+    # not vacuous: the nonlinear weights move the values well away from the
+    # linear (smooth-data) combination at the jumps
+    lin = np.array([0.1 * (col[i - 2] / 3 - 7 * col[i - 1] / 6 + 11 * col[i] / 6)
+                    + 0.6 * (-col[i - 1] / 6 + 5 * col[i] / 6 + col[i + 1] / 3)
+                    + 0.3 * (col[i] / 3 + 5 * col[i + 1] / 6 - col[i + 2] / 6) for i in ig])
+    assert np.abs(L - lin).max() > 0.1
+
+
 def test_weno_reference_right_equals_next_left():
     """scheme_WENO.f90:54-56: the right state of face f is the left state of f+1."""
     rng = np.random.default_rng(6)
