@@ -43,6 +43,17 @@ __device__ __forceinline__ double frcp(double x) {
   }
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so neighbouring tiles share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  const int per = (nblocks + nx - 1) / nx;
+  const int xcd = bid % nx, local = bid / nx;
+  const int full = nblocks % nx == 0 ? nx : nblocks % nx;   // XCDs with `per` blocks
+  const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + local;
+}
+
 // ---- exact family: IEEE FP64 division and square root, shortened -----------
 // The compiler expands an IEEE FP64 division a / b into two v_div_scale, a
 // v_rcp estimate, four FMAs (two Newton steps on the reciprocal), a multiply,

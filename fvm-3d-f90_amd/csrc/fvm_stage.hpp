@@ -50,17 +50,6 @@ __device__ __forceinline__ void st_comp(double* __restrict__ q, int64_t cs, int 
   *(double*)((char*)(q + c * cs) + (idx << 3)) = v;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so neighbouring tiles share an L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  const int nx = 8;
-  const int per = (nblocks + nx - 1) / nx;
-  const int xcd = bid % nx, local = bid / nx;
-  const int full = nblocks % nx == 0 ? nx : nblocks % nx;   // XCDs with `per` blocks
-  const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
-  return base + local;
-}
-
 // ---------------------------------------------------------------- RK combine
 struct RKCoef { double a, b, c, d; };
 

@@ -179,19 +179,25 @@ __device__ __forceinline__ void apply_frame(const VPoint& pt, double out[5]) {
   }
 }
 
-// u, v, w, T, mu of the conserved state at a resolved vertex
+// The NC conserved components of a resolved vertex.
 template <int NC>
-__device__ __forceinline__ void prims(const VPoint& pt, const ViscParams& V, const Rec& R,
-                                      double out[5]) {
-  const double* __restrict__ q = pt.q;
-  const int64_t c = pt.c, cs = pt.cs;
-  const double rho = q[c];
+__device__ __forceinline__ void load_cons(const VPoint& pt, double q[NC]) {
+  const double* __restrict__ p = pt.q;
+#pragma unroll
+  for (int m = 0; m < NC; ++m) q[m] = p[pt.c + m * pt.cs];
+}
+
+// u, v, w, T, mu of a conserved state q[0..NC-1]
+template <int NC>
+__device__ __forceinline__ void prims_of(const double q[NC], const ViscParams& V, const Rec& R,
+                                         double out[5]) {
+  const double rho = q[0];
   if constexpr (kFastV) {
     // one reciprocal of rho; reciprocals of R and T_ref from the host
     const double rr = frcp(rho);
-    const double u = q[c + cs] * rr, v = q[c + 2 * cs] * rr;
-    const double w = NC == 5 ? q[c + 3 * cs] * rr : 0.0;
-    const double e = q[c + (NC - 1) * cs];
+    const double u = q[1] * rr, v = q[2] * rr;
+    const double w = NC == 5 ? q[3] * rr : 0.0;
+    const double e = q[NC - 1];
     const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
     const double t = p * (rr * V.rr_gas);
     const double tr = t * V.rt_ref;
@@ -203,9 +209,9 @@ __device__ __forceinline__ void prims(const VPoint& pt, const ViscParams& V, con
     return;
   }
   const Recip ir = erecip(rho);
-  const double u = ediv(q[c + cs], ir), v = ediv(q[c + 2 * cs], ir);
-  const double w = NC == 5 ? ediv(q[c + 3 * cs], ir) : 0.0;
-  const double e = q[c + (NC - 1) * cs];
+  const double u = ediv(q[1], ir), v = ediv(q[2], ir);
+  const double w = NC == 5 ? ediv(q[3], ir) : 0.0;
+  const double e = q[NC - 1];
   // class_flow_field.f90:383-396 operation order
   const double p = (V.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w));
   const double t = ediv(p, rho * V.r_gas);
@@ -215,6 +221,15 @@ __device__ __forceinline__ void prims(const VPoint& pt, const ViscParams& V, con
   out[2] = w;
   out[3] = t;
   out[4] = ediv(V.mu_ref * tr * esqrt(tr) * (V.t_ref + V.s), t + V.s);
+}
+
+// u, v, w, T, mu of the conserved state at a resolved vertex
+template <int NC>
+__device__ __forceinline__ void prims(const VPoint& pt, const ViscParams& V, const Rec& R,
+                                      double out[5]) {
+  double q[NC];
+  load_cons<NC>(pt, q);
+  prims_of<NC>(q, V, R, out);
 }
 
 // Face flux in direction D between vertex A (offset O along D from the
@@ -291,36 +306,65 @@ __device__ __forceinline__ void residual(const At& at, const BlockGeom& g, const
 }  // namespace visc
 
 // 3-D: tile TX x TY columns marching over `chunk` k-planes, three-plane ring.
+// The conserved values of plane k+2 are loaded while plane k is computed
+// (registers: NPL entries per thread), and converted into the ring at the
+// start of the next plane.  Tiles are dealt to the XCDs in contiguous ranges
+// so the halos neighbouring tiles share stay in one L2.
 template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void viscous3d_kernel(ViscArgs a) {
   constexpr int NT = TX * TY, W = TX + 2, HH = TY + 2, PN = W * HH;
+  constexpr int NPL = (PN + NT - 1) / NT;   // ring entries per thread and plane
   __shared__ double S[3][5][PN];
   const BlockGeom& g = a.g;
   const ViscParams& V = a.V;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
   const int tiles_xy = a.tiles_x * a.tiles_y;
-  const int tk = blockIdx.x / tiles_xy, rem = blockIdx.x % tiles_xy;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tk = bid / tiles_xy, rem = bid % tiles_xy;
   const int i0 = (rem % a.tiles_x) * TX, j0 = (rem / a.tiles_x) * TY;
   const int k0 = tk * a.chunk, k1 = min(k0 + a.chunk, g.nz);
   const int i = i0 + tx, j = j0 + ty;
   const visc::Rec R = visc::make_rec(g, V);
 
-  auto load_plane = [&](int k, int slot) {
-    for (int e = t; e < PN; e += NT) {
-      double v[5];
-      const visc::VPoint pt = visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k);
-      visc::prims<5>(pt, V, R, v);
-      visc::apply_frame(pt, v);
+  // conserved values of this thread's ring entries of plane k, with the
+  // velocity frame of each resolved point
+  double raw[NPL][5];
+  int frm[NPL][2];
+  auto fetch = [&](int k) {
 #pragma unroll
-      for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
+    for (int p = 0; p < NPL; ++p) {
+      const int e = t + p * NT;
+      if (e < PN) {
+        const visc::VPoint pt = visc::resolve<3>(a, i0 - 1 + e % W, j0 - 1 + e / W, k);
+        visc::load_cons<5>(pt, raw[p]);
+        frm[p][0] = pt.axes;
+        frm[p][1] = pt.neg;
+      }
     }
   };
-  load_plane(k0 - 1, 0);
-  load_plane(k0, 1);
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+      const int e = t + p * NT;
+      if (e < PN) {
+        double v[5];
+        visc::prims_of<5>(raw[p], V, R, v);
+        visc::apply_frame(visc::VPoint{nullptr, 0, 0, frm[p][0], frm[p][1]}, v);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) S[slot][m][e] = v[m];
+      }
+    }
+  };
+  fetch(k0 - 1);
+  commit(0);
+  fetch(k0);
+  commit(1);
+  fetch(k0 + 1);
   int sm = 0, sc = 1, sp = 2;   // ring slots of k-1, k, k+1
   double Fzm[4] = {0.0, 0.0, 0.0, 0.0};   // k- face flux of the owned column
   for (int k = k0; k < k1; ++k) {
-    load_plane(k + 1, sp);
+    commit(sp);                  // plane k+1
+    if (k + 1 < k1) fetch(k + 2);
     __syncthreads();
     if (i < g.nx && j < g.ny) {
       const int ctr = (ty + 1) * W + (tx + 1);
