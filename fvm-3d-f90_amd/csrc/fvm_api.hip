@@ -2136,8 +2136,12 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
   hipEvent_t e0 = res.e0, e1 = res.e1;
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_slot, 1.0e-6);
   HIP_TRY(hipGetLastError());
-  const int chunks3[] = {4, 8, 16, 32}, chunks2[] = {2, 4, 8, 16};
+  // 3-D k-chunks: besides the powers of two, 6, 10 and 12 (on blocks of
+  // 30-90 vertices a chunk that divides the extents more evenly can save a
+  // few per cent: the 27-block blob runs 2.5 % faster at 10 than at 8)
+  const int chunks3[] = {4, 6, 8, 10, 12, 16, 32}, chunks2[] = {2, 4, 8, 16};
   const int dim = ctx->blocks[0]->dim;
+  const int nchunk = dim == 3 ? 7 : 4;
   const bool together = ctx->launch_mode != FVM_LAUNCH_PER_BLOCK;
   const size_t groups = together ? 1 : ctx->blocks.size();
   for (size_t gi = 0; gi < groups; ++gi) {
@@ -2151,7 +2155,7 @@ int fvm_autotune(fvm_ctx* ctx, int reps) {
     // are not timed on a cold device
     for (int pass = 0; pass < 2; ++pass)
     for (int tile = 0; tile < ntiles; ++tile)
-      for (int ci = 0; ci < 4; ++ci) {
+      for (int ci = 0; ci < nchunk; ++ci) {
         if (dim == 2 && tile > 0 && ci > 0) break;   // a 2-D tile does not march
         if (dim == 3 && !tile3d_ok(ctx->recon, tile)) break;
         const int chunk = dim == 3 ? chunks3[ci] : chunks2[ci];
