@@ -779,6 +779,8 @@ def summary(ln):
            "numerics": ln["config"].get("numerics")}
     if r:
         out.update(roofline_frac=r["frac"], avg_launch_us=r["avg_launch_us"], kernel=r["kernel"])
+    if "tile_xy_chunk" in ln["config"]:
+        out["tile_xy_chunk"] = ln["config"]["tile_xy_chunk"]
     return out
 
 
@@ -807,12 +809,12 @@ def extras(args):
             ref["cpu_model"] = cpu_model()
             wl["riemann400"]["cpu_baseline"] = ref
     b = argparse.Namespace(**vars(ns))
-    b.steps, b.warmup = 20, 2
+    b.steps, b.warmup = 60, 5
     ln = bench_blob27(b)
     wl["blob27"] = summary(ln)
     wl["blob27"]["speedup_vs_per_block"] = ln["config"]["speedup_vs_per_block"]
     t = argparse.Namespace(**vars(ns))
-    t.steps, t.warmup = 10, 2
+    t.steps, t.warmup = 30, 3
     wl["tgv192"] = summary(bench_tgv(t))
     out["workloads"] = wl
     sv = argparse.Namespace(**vars(ns))
