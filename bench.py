@@ -836,7 +836,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         sys.exit(2)
     fn = {"vortex128": bench_vortex, "riemann400": bench_riemann, "blob27": bench_blob27,

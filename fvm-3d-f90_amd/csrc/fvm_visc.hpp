@@ -2,7 +2,7 @@
 // numerics, selected by FVM_FAST as for the stage kernels).
 //
 // The reference reserves viscous_flux_sub (src/solve/class_space_discret.f90:
-// 671-675) and leaves it empty; this is the build's own definition of the
+// 671-675) and leaves it empty; this is the library's own definition of the
 // term, the same one oracle/fvm_oracle.c restates on the CPU:
 //   - per vertex: u, v, w, T = p / (rho R), mu = Sutherland(T);
 //   - a stencil point outside the block maps into it per direction

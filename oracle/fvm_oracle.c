@@ -1,6 +1,6 @@
 /*
  * fvm_oracle.c -- CPU restatement of FVM-3D-F90's explicit residual + RK
- * stage path, used as the checker for the HIP implementation.
+ * stage path, against which the HIP implementation is tested.
  *
  * TEST INFRASTRUCTURE ONLY (see fvm_oracle.h).  Compiled with
  * -ffp-contract=off: every expression below keeps the reference's operation
@@ -143,7 +143,7 @@ static double weno_js_ref(const double u[5])
 }
 
 /* Textbook WENO5-JS (Jiang & Shu, JCP 126 (1996)), value at the right face
- * of u[2]; this build's variant beside the reference's, with
+ * of u[2]; this library's variant beside the reference's, with
  * IS_1 = 13/12 (u1 - 2 u2 + u3)^2 + 1/4 (u1 - u3)^2.  The indicator terms are
  * formed in an order symmetric under stencil reversal, t = (a + c) - 2 b and
  * s = (3 c + a) - 4 b (the weight-3 point first), so the mirrored stencil's
@@ -672,7 +672,7 @@ static void inviscid_residual(blk_t *b, const orc_config *c, double *rhs)
 /* ======================================================================== */
 /* The reference reserves viscous_flux_sub (class_space_discret.f90:671-675)
  * but leaves it empty and its call commented out (class_time_discret.f90:
- * 182).  This restatement defines the build's viscous term:
+ * 182).  This restatement defines the library's viscous term:
  *   - primitives u, v, w, T = p / (rho R) and mu = Sutherland(T) per vertex;
  *     a stencil point outside the block maps into it per direction (periodic:
  *     wrap by n, the nodal period; otherwise clamp to the boundary vertex),

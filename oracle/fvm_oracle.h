@@ -1,11 +1,11 @@
 /*
  * fvm_oracle.h -- CPU restatement of the FVM-3D-F90 explicit residual + RK
  * path.  TEST INFRASTRUCTURE ONLY: the test suite, the smoke check and the
- * CPU-baseline leg of bench.py load liboracle.so as the checker.  Nothing in
+ * CPU-baseline leg of bench.py load liboracle.so to check results.  Nothing in
  * the product links or calls it.
  *
  * Scheme codes are the reference's (src/defs/type_defs_scheme.f90:11-36);
- * codes >= 1000 are additions of this build with no reference counterpart.
+ * codes >= 1000 are additions of this library with no reference counterpart.
  */
 #ifndef FVM_ORACLE_H
 #define FVM_ORACLE_H
@@ -28,7 +28,7 @@
 #define ORC_BC_EXTRAP     0     /* zeroth-order copy of the boundary vertex */
 #define ORC_BC_PERIODIC   1     /* nodal periodic: vertex n == vertex 0     */
 /* Walls (the reference's codes, type_defs_bc.f90:11-30, which it never
- * applies; this build's definition): ghost -l is the mirror vertex l */
+ * applies; this library's definition): ghost -l is the mirror vertex l */
 #define ORC_BC_WALL       30    /* slip: normal velocity negated            */
 #define ORC_BC_WALL_INVISCID 31 /* as ORC_BC_WALL                           */
 #define ORC_BC_NOSLIP     33    /* adiabatic no-slip: every velocity component

@@ -1,7 +1,7 @@
 """ctypes bindings for the oracle (oracle/liboracle.so) and, where it has been
 built in this checkout, the reference harness (oracle/_ref/libfvmref.so).
 
-Test infrastructure only: these libraries are the checker, never the product.
+Test infrastructure only: these libraries check results, never produce them.
 """
 import ctypes
 import os
@@ -80,7 +80,7 @@ def oracle():
 
 
 def ref():
-    """The reference harness, or None where it is not built (GPU boxes)."""
+    """The reference harness, or None where it is not built (no reference sources)."""
     global _ref
     if _ref is None and os.path.exists(REF_SO):
         lib = ctypes.CDLL(REF_SO)

@@ -1,5 +1,5 @@
 """The oracle's wall faces (CPU).  The reference lists wall codes
-(src/defs/type_defs_bc.f90:11-30) but never applies them; this build defines
+(src/defs/type_defs_bc.f90:11-30) but never applies them; this library defines
 them: ghost -l is the mirror vertex l with the normal velocity negated (slip
 wall 30/31, symmetry 50) or every velocity component negated (adiabatic
 no-slip wall 33); a no-slip wall's vertices lose their momentum (and its
