@@ -69,6 +69,24 @@ def test_tile_shapes_bitwise(tile, chunk, monkeypatch):
     assert np.array_equal(qg, qo)
 
 
+@pytest.mark.parametrize("tile", ["1", "3"])
+@pytest.mark.parametrize("chunk", ["6", "10", "12"])
+def test_weno_chunks_bitwise(tile, chunk, monkeypatch):
+    """The autotuner's k-chunks that are not powers of two, with the WENO5
+    stencil (five-plane k-queue) and a k extent that leaves a partial last
+    chunk, give the oracle's bits."""
+    monkeypatch.setenv("FVM_TILE", tile)
+    monkeypatch.setenv("FVM_CHUNK", chunk)
+    p = cases.smooth_3d((29, 13, 37), seed=4)
+    sp = (1 / 36, 1 / 12, 1 / 28)
+    print(f"case weno chunks tile={tile} chunk={chunk} 37x13x29", flush=True)
+    sch = dict(recon=O.WENO, limiter=O.LIM_NONE, flux=O.AUSMPLUS, rk=O.RK_SSP3, ref_compat=0)
+    with fvmgpu.Context() as ctx:
+        qg, _, _, _ = run_gpu(ctx, p, sp, 2, dt=1e-3, bc=(1, 1, 0, 0, 1, 1), **sch)
+    qo, _ = run_oracle(p, sp, 2, dt=1e-3, bc=(1, 1, 0, 0, 1, 1), **sch)
+    assert np.array_equal(qg, qo)
+
+
 @pytest.mark.parametrize("tile2", ["1", "2"])
 @pytest.mark.parametrize("recon,lim,flux", [(O.SIMPLE, O.LIM_NONE, O.ROE),
                                             (O.MUSCL, O.LIM_MINMOD, O.ROE),
